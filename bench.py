@@ -1,0 +1,193 @@
+"""Benchmark: batched P-SV forward + adjoint gradient on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 3a|2|1|3b|4|5]
+
+One "step" is one gradient pass of the workload: forward run of every shot
+(with adjoint snapshots), misfit + adjoint sources, adjoint run with kernel
+accumulation.  Inputs (model, sources, observed data) are resident on the
+GPU before the timed region.  Throughput = sum over shots of nx*nz*nt*2
+cell-updates / wall time (forward and adjoint both counted, nothing
+recomputed).
+
+The JSON line adds
+  roofline:     the dominant kernel (the fused step kernel) against the
+                8 TB/s HBM peak: algorithmic bytes per launch (8 B per state
+                field per shot cell + 4 B per material field per grid cell)
+                over its mean launch time from HIP events on the library's
+                stream, in a forward pass timed on its own;
+  cpu_baseline: the oracle's unfused reference kernel sequence (one shot,
+                a fixed window of steps on the same grid) on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def build_problem(name):
+    from seispie_amd import configs
+    if name == "3a":
+        true = configs.psv_config3()
+        init = configs.psv_config3()
+        init.grids[0].fields = [np.full_like(f, f.mean()) for f in true.grids[0].fields]
+    elif name == "3b":
+        true = configs.psv_config3_mixed()
+        init = configs.psv_config3_mixed()
+        for g in init.grids:
+            g.fields = [np.full_like(f, f.mean()) for f in g.fields]
+    elif name == "2":
+        true = configs.psv_config2(true_model=True)
+        init = configs.psv_config2(true_model=False)
+    elif name == "1":
+        true = configs.sh_config1()
+        init = configs.sh_config1()
+        init.grids[0].fields = [np.full_like(f, f.max()) for f in true.grids[0].fields]
+    elif name == "4":
+        true = configs.spin_config4()
+        init = configs.spin_config4()
+        g = init.grids[0]
+        g.fields = [np.full_like(f, f.mean()) for f in g.fields]
+    elif name == "5":
+        true = configs.fwi_config5(true_model=True)
+        init = configs.fwi_config5(true_model=False)
+    else:
+        raise SystemExit("unknown config %s" % name)
+    return true, init
+
+
+def observed_data(true):
+    from seispie_amd import _lib, configs
+    ctx = _lib.Context(0)
+    ids = configs.load_problem(ctx, true)
+    ctx.forward(0)
+    data = [ctx.records(i) for i in ids]
+    ctx.close()
+    return data
+
+
+def cpu_baseline(prob, warm=20, nsteps=200):
+    """Oracle (port of the reference kernels) on the host, one shot."""
+    from oracle import oracle as O
+    from tests.helpers import oracle_sim, oracle_stf
+    g = prob.grids[0]
+    shot = g.shots[0]
+    sim = oracle_sim(prob, g, shot)
+    stf = oracle_stf(prob, shot)
+    ncores = min(os.cpu_count() or 1, 16)
+    out = {}
+    for nth in (ncores, 1):
+        steps = nsteps if nth > 1 else max(nsteps // 8, 10)
+        sec = sim.cpu_bench(stf, warm if nth > 1 else 2, steps, nth)
+        out[nth] = (g.nx * g.nz * steps / sec / 1e9, steps)
+    return ncores, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="3a")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--order", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(shots would be split across GPUs with one kernel all-reduce per gradient).", file=sys.stderr)
+        return 2
+
+    from seispie_amd import _lib, configs
+    true, init = build_problem(args.config)
+    t0 = time.time()
+    data = observed_data(true)
+    print("# observed data for %s in %.1fs" % (true.name, time.time() - t0), file=sys.stderr, flush=True)
+
+    ctx = _lib.Context(0)
+    if args.chunk:
+        ctx.set_option(_lib.OPT_CHUNK, args.chunk)
+    ctx.set_option(_lib.OPT_ORDER, args.order)
+    ids = configs.load_problem(ctx, init)
+    for i, d in zip(ids, data):
+        ctx.set_data(i, d)
+    del data
+    sae = init.sae
+
+    for w in range(args.warmup):
+        ctx.gradient(sae)
+        ctx.sync()
+        print("# warmup %d done" % w, file=sys.stderr, flush=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        norms = ctx.gradient(sae)
+        print("# step %d misfit %.6e" % (s, norms.sum()), file=sys.stderr, flush=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    tim = ctx.timing()
+
+    # dominant kernel alone: a forward pass without snapshots
+    ctx.forward(0)
+    ctx.sync()
+    ft = ctx.timing()
+    launch_ms = ft["total_ms"] / ft["step_launches"]
+    bytes_per_launch = ft["alg_bytes"] / ft["step_launches"]
+    achieved = bytes_per_launch / (launch_ms * 1e-3)
+
+    cells = init.cell_updates(passes=2)
+    value = cells * args.steps / wall / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf)).get(init.name)
+        if t:
+            traffic = t["hbm_bytes_per_launch"]
+
+    cpu = None
+    if not args.no_cpu:
+        ncores, cb = cpu_baseline(init)
+        g = init.grids[0]
+        cpu = {"value": round(cb[ncores][0], 4), "unit": "Gcell-upd/s", "cores": ncores, "kind": "port",
+               "sample": "oracle (unfused reference kernel order, float32 storage / float64 math) on %dx%d, "
+                         "one shot, %d forward steps after 20 warm-up; 1 core: %.4f Gcell-upd/s"
+                         % (g.nx, g.nz, cb[ncores][1], cb[1][0])}
+
+    line = {
+        "metric": "Gcell-upd/s (% HBM peak); time-to-misfit, learned vs plain precond",
+        "value": round(value, 3),
+        "unit": "Gcell-upd/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage, f64 arithmetic",
+        "data": "synthetic",
+        "config": {"workload": init.name + " forward+adjoint gradient",
+                   "grids": [[g.nx, g.nz] for g in init.grids], "shots": init.nshots, "nt": init.nt,
+                   "adjoint_interval": sae, "cell_updates_per_step": cells,
+                   "hbm_roofline_frac_of_whole_pass": round(init.alg_bytes_per_step() * init.nt * 2 * args.steps
+                                                            / wall / HBM_PEAK, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "kernel": "k_step<P-SV>", "launch_ms": round(launch_ms, 4),
+                     "alg_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

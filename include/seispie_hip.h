@@ -1,0 +1,137 @@
+/*
+ * seispie_hip -- C ABI of the MI355X (gfx950) solver library.
+ *
+ * The reference (icui/seispie) has no native code: its solver is the
+ * Python class seispie/solver/fd2d/fd2d.py driving Numba-CUDA kernels in
+ * seispie/solver/fd2d/solver.py.  Each entry point below names the
+ * reference method or kernel sequence it replaces; the Python plugin
+ * seispie_amd.solver.fd2d binds them through ctypes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero code on failure;
+ *     sp_last_error(ctx) then holds a message.  Nothing throws across the ABI.
+ *   - Grid arrays are float32 with k = i*nz + j (z fastest), as the
+ *     reference (solver.py:14-19).
+ *   - Host buffers are borrowed for the duration of the call.  Arguments
+ *     named *_on_device == 1 take device pointers on the context's device
+ *     (ordered on the context's stream) instead.
+ *   - A context is not thread safe; use one per host thread.
+ */
+#ifndef SEISPIE_HIP_H
+#define SEISPIE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sp_ctx sp_ctx;
+
+/* physics: solver config keys sh / psv / spin (fd2d.py:116-118) */
+#define SP_SH 1
+#define SP_PSV 2
+#define SP_SPIN 3
+
+/* model fields of a grid (fd2d.py:172, 203-216) */
+#define SP_F_RHO 0
+#define SP_F_LAM 1
+#define SP_F_MU 2
+#define SP_F_NU 3
+#define SP_F_J 4
+#define SP_F_LAMC 5
+#define SP_F_MUC 6
+#define SP_F_NUC 7
+
+/* sensitivity kernels of a grid.  SP_K_MU is the reference's k_mu
+ * (solver.py:159-163); the others are this library's P-SV / spin
+ * definitions.  SP_K_VP/VS/RHOP are the vp/vs/rho kernels derived from
+ * lam/mu/rho by sp_kernels_vps. */
+#define SP_K_LAM 0
+#define SP_K_MU 1
+#define SP_K_RHO 2
+#define SP_K_NU 3
+#define SP_K_J 4
+#define SP_K_MUC 5
+#define SP_K_VP 6
+#define SP_K_VS 7
+#define SP_K_RHOP 8
+#define SP_NKERN 9
+
+/* tuning options (sp_set_option) */
+#define SP_OPT_CHUNK 1      /* rows per wavefront item in the step kernel */
+#define SP_OPT_ORDER 2      /* 0: shot-outer item order, 1: shot-inner */
+#define SP_OPT_GRAPH 3      /* 1: replay time steps from a captured hipGraph */
+#define SP_OPT_SNAP_MB 4    /* snapshot memory budget in MiB (0 = auto) */
+
+/* Context and device.  Replaces fd2d.setup (fd2d.py:111-120). */
+int sp_create(int device, sp_ctx **out);
+int sp_destroy(sp_ctx *ctx);
+const char *sp_last_error(const sp_ctx *ctx);
+int sp_set_stream(sp_ctx *ctx, void *hip_stream); /* NULL: the context's own stream */
+int sp_sync(sp_ctx *ctx);
+int sp_set_option(sp_ctx *ctx, int key, long long value);
+
+/* Time axis and physics: solver keys nt, dt, sh/psv/spin (fd2d.py:114-118). */
+int sp_set_physics(sp_ctx *ctx, int physics, int nt, double dt);
+
+/* Model grid.  Replaces import_model's device part (fd2d.py:184-245):
+ * param 0 takes fields {vp, vs, rho} and applies vps2lm (fd2d.py:21-33);
+ * param 1 takes the reference's spin list {lambda, mu, nu, j, lambda_c,
+ * mu_c, nu_c, rho} (fd2d.py:172); param 2 takes {lambda, mu, rho}. */
+int sp_add_grid(sp_ctx *ctx, int nx, int nz, float dx, float dz, int param,
+                const float *const *fields, int *grid_out);
+/* set_bound: fd2d.py:47-68, 218-228 */
+int sp_set_sponge(sp_ctx *ctx, int grid, int width, double alpha, int left, int right,
+                  int bottom, int top);
+/* update_model (fd2d.py:442-443) for any field, and read-back */
+int sp_set_field(sp_ctx *ctx, int grid, int field, const float *src, int src_on_device);
+int sp_get_field(sp_ctx *ctx, int grid, int field, float *dst, int dst_on_device);
+
+/* One simulation ("shot") on a grid.  Replaces import_sources /
+ * import_stations (fd2d.py:122-165) and the isrc selection of run_forward
+ * (solver.py:180-183): the caller lists the sources that fire in this
+ * shot.  src_k/rec_k are grid indices; stf is [ncomp_src][nsrc][nt]
+ * (SH: y; P-SV and spin: x then z). */
+int sp_add_shot(sp_ctx *ctx, int grid, int nsrc, const int *src_k, const float *stf, int nrec,
+                const int *rec_k, int *shot_out);
+int sp_clear_shots(sp_ctx *ctx);
+int sp_num_components(const sp_ctx *ctx); /* records per receiver: SH 1, P-SV 2, spin 3 */
+
+/* run_forward (solver.py:166-287) for every shot, batched.  Records the
+ * displacement at the receivers.  sae > 0 also keeps the adjoint snapshots
+ * (solver.py:193-203, 236-242) on the device for sp_adjoint. */
+int sp_forward(sp_ctx *ctx, int sae);
+/* records of a shot: [ncomp][nrec][nt] (SH y; P-SV x, z; spin x, z, y) */
+int sp_get_records(sp_ctx *ctx, int shot, float *dst, int dst_on_device);
+/* observed data ("syn" in fd2d.py:303-309), same layout as the records */
+int sp_set_data(sp_ctx *ctx, int shot, const float *src, int src_on_device);
+/* misfit/waveform.py:5-34 for every shot and misfit component (SH y,
+ * P-SV and spin x and z): writes the adjoint sources on the device and
+ * the L2 norm of each tapered residual to out[shot*ncomp_misfit + c]. */
+int sp_misfit(sp_ctx *ctx, double *norms_out);
+/* run_adjoint + interaction (solver.py:290-321) for every shot, using the
+ * snapshots of the last sp_forward(sae); kernels are cleared first. */
+int sp_adjoint(sp_ctx *ctx, int sae);
+/* run_kernel(adj=1) minus smoothing (solver.py:332-394): forward, misfit
+ * and adjoint for every shot in snapshot-memory-sized sub-batches.
+ * norms_out as sp_misfit.  compute_misfit (adj=0) is sp_forward(ctx, 0)
+ * followed by sp_misfit. */
+int sp_gradient(sp_ctx *ctx, int sae, double *norms_out);
+/* kernels (SP_K_*) of a grid, [N] */
+int sp_get_kernel(sp_ctx *ctx, int grid, int which, float *dst, int dst_on_device);
+/* vp/vs/rho kernels from lam/mu/rho kernels (chain rule of vps2lm) */
+int sp_kernels_vps(sp_ctx *ctx, int grid);
+/* smooth (fd2d.py:71-108, 349-352): in place on a kernel of a grid */
+int sp_smooth_kernel(sp_ctx *ctx, int grid, int which, int sigma);
+/* the same smoothing on a caller buffer of nx*nz floats */
+int sp_smooth(sp_ctx *ctx, float *field, int nx, int nz, int sigma, int on_device);
+
+/* Timing of the last sp_forward / sp_adjoint / sp_gradient on the
+ * context's stream: step-kernel launches and their total milliseconds
+ * (HIP events), and the algorithmic bytes those launches move. */
+int sp_last_timing(const sp_ctx *ctx, long long *step_launches, double *step_ms,
+                   double *total_ms, double *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

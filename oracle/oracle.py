@@ -1,0 +1,294 @@
+"""CPU oracle for seispie's fd2d solver hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: tests/, the smoke
+check in __graft_entry__ and the cpu_baseline leg of bench.py may call it;
+the product package (seispie_amd/) never imports it.
+
+It wraps oracle/_build/liboracle.so (oracle.c, a kernel-by-kernel C
+restatement of seispie/solver/fd2d/solver.py and fd2d.py) and restates the
+reference's small host-side pieces in NumPy:
+
+  ricker            seispie/solver/source/ricker.py:7-11
+  source/receiver   seispie/solver/fd2d/fd2d.py:122-165 (grid index rounding)
+  grid derivation   seispie/solver/fd2d/fd2d.py:189-196
+  run_kernel        seispie/solver/fd2d/solver.py:332-404 (shot loop, misfit
+                    per component, adjoint, smoothing)
+
+Two precisions: "f32" reproduces the reference numerics (float32 storage,
+float64 arithmetic where Numba promotes); "f64" replays the same stencils
+with float64 storage.
+
+Pinning (tests/test_oracle_*.py): the stencil operators against polynomial
+closed forms and D- = -(D+)^T, the Ricker/taper/sponge/vps2lm closed forms,
+smoothing against a brute-force double sum, forward runs against the
+reference-held SH trace examples/inverse-checker/traces/vy_000000.npy,
+linearity/superposition/reciprocity/energy invariants, f32 against f64, and
+the adjoint kernels against finite differences of the misfit.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+SH, PSV, SPIN = 1, 2, 3
+NSNAP = {SH: 2, PSV: 4, SPIN: 6}
+NKERN = {SH: 1, PSV: 3, SPIN: 6}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int), ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nt", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_double),
+        ("rho", ctypes.c_void_p), ("lam", ctypes.c_void_p), ("mu", ctypes.c_void_p),
+        ("nu", ctypes.c_void_p), ("jj", ctypes.c_void_p), ("mu_c", ctypes.c_void_p),
+        ("nu_c", ctypes.c_void_p), ("bound", ctypes.c_void_p),
+    ]
+
+
+class _Geom(ctypes.Structure):
+    _fields_ = [
+        ("nsrc", ctypes.c_int), ("src_id", ctypes.c_void_p), ("isrc", ctypes.c_int),
+        ("nrec", ctypes.c_int), ("rec_id", ctypes.c_void_p),
+    ]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp = ctypes.c_void_p
+        for sfx in ("f32", "f64"):
+            f = getattr(L, "orc_forward_" + sfx)
+            f.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 6 + [ctypes.c_int, vp]
+            f.restype = ctypes.c_int
+            f = getattr(L, "orc_adjoint_" + sfx)
+            f.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 3 + [ctypes.c_int] + [vp] * 7
+            f.restype = ctypes.c_int
+            f = getattr(L, "orc_waveform_" + sfx)
+            f.argtypes = [vp] * 4 + [ctypes.c_int, ctypes.c_double, ctypes.c_int]
+            f.restype = ctypes.c_double
+            f = getattr(L, "orc_smooth_" + sfx)
+            f.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            f.restype = None
+        L.orc_set_bound.argtypes = [vp, ctypes.c_int, ctypes.c_double] + [ctypes.c_int] * 6
+        L.orc_set_bound.restype = None
+        L.orc_vps2lm.argtypes = [vp, vp, vp, ctypes.c_long]
+        L.orc_vps2lm.restype = None
+        L.orc_bench_forward_f32.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 3 + [ctypes.c_int] * 3
+        L.orc_bench_forward_f32.restype = ctypes.c_double
+        L.orc_max_threads.restype = ctypes.c_int
+        L.orc_diff_f64.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_diff_f64.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- host pieces
+
+def ricker(t, f0, t0, ang, amp):
+    """seispie/solver/source/ricker.py:7-11 (float64, returns x, y, z)."""
+    a = (np.pi * f0) ** 2
+    stf = -amp * (1.0 - 2.0 * a * (t - t0) ** 2) * np.exp(-a * (t - t0) ** 2)
+    ang_rad = np.deg2rad(ang)
+    return stf * np.sin(ang_rad), stf, -stf * np.cos(ang_rad)
+
+
+def make_stf(src, nt, dt):
+    """fd2d.py:127-138: float32 [nsrc*nt] arrays for x, y, z."""
+    nsrc = src.shape[0]
+    out = [np.zeros(nsrc * nt, dtype="float32") for _ in range(3)]
+    for isrc in range(nsrc):
+        for it in range(nt):
+            vals = ricker(it * dt, *src[isrc][3:])
+            for c in range(3):
+                out[c][isrc * nt + it] = vals[c]
+    return out
+
+
+def grid_index(xz, dx, dz, nz):
+    """fd2d.py:132-134 / :157-159: round(x/dx)*nz + round(z/dz), int32."""
+    ids = np.zeros(xz.shape[0], dtype="int32")
+    for r in range(xz.shape[0]):
+        ids[r] = int(np.round(xz[r][0] / dx)) * nz + int(np.round(xz[r][1] / dz))
+    return ids
+
+
+def grid_shape(x, z, npt):
+    """fd2d.py:189-196.  x, z float32 arrays; dx, dz stay float32 scalars."""
+    lx = x.max() - x.min()
+    lz = z.max() - z.min()
+    nx = int(np.rint(np.sqrt(npt * lx / lz)))
+    nz = int(np.rint(np.sqrt(npt * lz / lx)))
+    return nx, nz, lx / (nx - 1), lz / (nz - 1)
+
+
+def set_bound(nx, nz, width, alpha, left, right, bottom, top):
+    b = np.zeros(nx * nz, dtype="float32")
+    lib().orc_set_bound(_p(b), int(width), float(alpha), int(left), int(right), int(bottom), int(top), nx, nz)
+    return b
+
+
+def vps2lm(vp, vs, rho):
+    lam = np.ascontiguousarray(vp, dtype="float32").copy()
+    mu = np.ascontiguousarray(vs, dtype="float32").copy()
+    rho = np.ascontiguousarray(rho, dtype="float32")
+    lib().orc_vps2lm(_p(lam), _p(mu), _p(rho), lam.size)
+    return lam, mu
+
+
+def diff(v, nx, nz, dx, dz, which):
+    """D-x, D+x, D-z, D+z (which = 0..3) of a float64 [nx, nz] field."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    out = np.zeros_like(v)
+    lib().orc_diff_f64(_p(v), _p(out), nx, nz, float(dx), float(dz), int(which))
+    return out.reshape(nx, nz)
+
+
+# ------------------------------------------------------------- simulations
+
+class Sim:
+    """One grid + model + sponge + geometry, as the reference solver holds it.
+
+    model: dict of float32 [nx*nz] arrays with keys rho, lam, mu and, for
+    spin, nu, j, mu_c, nu_c (already in the lambda-mu parameterisation).
+    """
+
+    def __init__(self, physics, nx, nz, dx, dz, dt, nt, model, bound, src_id, rec_id, isrc=-1):
+        self.physics, self.nx, self.nz, self.nt = physics, nx, nz, nt
+        self.dx, self.dz, self.dt = np.float32(dx), np.float32(dz), float(dt)
+        n = nx * nz
+        f32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype="float32").reshape(n)
+        self.model = {k: f32(v) for k, v in model.items()}
+        self.bound = f32(bound)
+        self.src_id = np.ascontiguousarray(src_id, dtype="int32")
+        self.rec_id = np.ascontiguousarray(rec_id, dtype="int32")
+        self.isrc = int(isrc)
+
+    def _structs(self):
+        m = self.model
+        M = _Model(self.physics, self.nx, self.nz, self.nt, float(self.dx), float(self.dz), self.dt,
+                   _p(m["rho"]), _p(m.get("lam")), _p(m["mu"]), _p(m.get("nu")), _p(m.get("j")),
+                   _p(m.get("mu_c")), _p(m.get("nu_c")), _p(self.bound))
+        G = _Geom(self.src_id.size, _p(self.src_id), self.isrc, self.rec_id.size, _p(self.rec_id))
+        return M, G
+
+    def forward(self, stf, sae=0, precision="f32"):
+        """run_forward: returns (obs dict of [nrec, nt], snapshots or None).
+
+        stf: (stf_x, stf_y, stf_z) float arrays [nsrc*nt].
+        """
+        dt_ = np.float32 if precision == "f32" else np.float64
+        n, nt, nrec = self.nx * self.nz, self.nt, self.rec_id.size
+        stf = [np.ascontiguousarray(s, dtype=dt_) for s in stf]
+        obs = [np.zeros(nrec * nt, dtype=dt_) for _ in range(3)]
+        snaps = None
+        if sae:
+            snaps = np.zeros((nt // sae, NSNAP[self.physics], n), dtype=dt_)
+        M, G = self._structs()
+        fn = getattr(lib(), "orc_forward_" + precision)
+        rc = fn(ctypes.byref(M), ctypes.byref(G), _p(stf[0]), _p(stf[1]), _p(stf[2]),
+                _p(obs[0]), _p(obs[1]), _p(obs[2]), int(sae), _p(snaps))
+        assert rc == 0
+        out = {}
+        if self.physics == SH:
+            out["y"] = obs[1].reshape(nrec, nt)
+        else:
+            out["x"] = obs[0].reshape(nrec, nt)
+            out["z"] = obs[2].reshape(nrec, nt)
+            if self.physics == SPIN:
+                out["y"] = obs[1].reshape(nrec, nt)
+        return out, snaps
+
+    def adjoint(self, adstf, sae, snaps, kernels=None, precision="f32"):
+        """run_adjoint with kernel accumulation.  adstf: dict comp -> [nrec, nt].
+
+        Returns kernels dict (lam, mu, rho, nu, j, muc), accumulated in place
+        into `kernels` when given.
+        """
+        dt_ = np.float32 if precision == "f32" else np.float64
+        n, nt, nrec = self.nx * self.nz, self.nt, self.rec_id.size
+        z = np.zeros(nrec * nt, dtype=dt_)
+        ad = [np.ascontiguousarray(adstf.get(c, z), dtype=dt_).reshape(-1) for c in ("x", "y", "z")]
+        if kernels is None:
+            kernels = {k: np.zeros(n, dtype=dt_) for k in ("lam", "mu", "rho", "nu", "j", "muc")}
+        snaps = np.ascontiguousarray(snaps, dtype=dt_)
+        M, G = self._structs()
+        fn = getattr(lib(), "orc_adjoint_" + precision)
+        rc = fn(ctypes.byref(M), ctypes.byref(G), _p(ad[0]), _p(ad[1]), _p(ad[2]), int(sae), _p(snaps),
+                *[_p(kernels[k]) for k in ("lam", "mu", "rho", "nu", "j", "muc")])
+        if rc != 0:
+            raise ValueError("nt must be a multiple of adjoint_interval")
+        return kernels
+
+    def cpu_bench(self, stf, warm, nsteps, nthreads):
+        """Wall seconds of nsteps unfused forward steps on nthreads threads."""
+        stf = [np.ascontiguousarray(s, dtype="float32") for s in stf]
+        M, G = self._structs()
+        return lib().orc_bench_forward_f32(ctypes.byref(M), ctypes.byref(G), _p(stf[0]), _p(stf[1]),
+                                           _p(stf[2]), int(warm), int(nsteps), int(nthreads))
+
+
+def waveform(syn, obs, nt, dt, precision="f32"):
+    """misfit/waveform.py:28-34.  syn = data, obs = synthetics, [nrec, nt].
+
+    Returns (norm of the tapered residual, adjoint source [nrec, nt],
+    residual [nrec, nt]).
+    """
+    dt_ = np.float32 if precision == "f32" else np.float64
+    syn = np.ascontiguousarray(syn, dtype=dt_).reshape(-1)
+    obs = np.ascontiguousarray(obs, dtype=dt_).reshape(-1)
+    nrec = syn.size // nt
+    ad = np.zeros_like(syn)
+    res = np.zeros_like(syn)
+    ss = getattr(lib(), "orc_waveform_" + precision)(_p(syn), _p(obs), _p(ad), _p(res), nt, float(dt), nrec)
+    return float(np.sqrt(ss)), ad.reshape(nrec, nt), res.reshape(nrec, nt)
+
+
+def smooth(data, nx, nz, sigma, precision="f32"):
+    dt_ = np.float32 if precision == "f32" else np.float64
+    d = np.ascontiguousarray(data, dtype=dt_).reshape(-1).copy()
+    getattr(lib(), "orc_smooth_" + precision)(_p(d), nx, nz, int(sigma))
+    return d
+
+
+def run_kernel(sim, stf, data, sae, adj=True, sigma=0, precision="f32", combine=False):
+    """solver.py:332-404 without MPI: per shot forward, misfit per component,
+    adjoint; then smooth.  data: list over shots of dict comp -> [nrec, nt].
+
+    Returns (misfit, kernels dict) -- kernels unsmoothed unless sigma > 0,
+    in which case every kernel is smoothed (the reference smooths k_mu).
+    """
+    nsrc = sim.src_id.size
+    comps = ["y"] if sim.physics == SH else ["x", "z"]
+    misfit = 0.0
+    kern = None
+    for i in range(nsrc):
+        sim.isrc = -1 if combine else i
+        obs, snaps = sim.forward(stf, sae=sae if adj else 0, precision=precision)
+        ad = {}
+        for c in comps:
+            m, a, _ = waveform(data[i][c], obs[c], sim.nt, sim.dt, precision)
+            misfit += m
+            ad[c] = a
+        if adj:
+            kern = sim.adjoint(ad, sae, snaps, kern, precision)
+    if adj and sigma:
+        for k in kern:
+            kern[k] = smooth(kern[k], sim.nx, sim.nz, sigma, precision)
+    return misfit, kern

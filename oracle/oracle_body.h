@@ -1,0 +1,503 @@
+/*
+ * Oracle body: a CPU restatement of seispie's fd2d kernel sequence.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (seispie_amd/) links,
+ * loads or calls this code; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, and only as the checker.
+ *
+ * This header is included twice by oracle.c:
+ *   T = float   (suffix f32): the reference's numerics -- every array is
+ *               float32, arithmetic is done in the precision Numba's typing
+ *               rules give each expression (mostly float64, a few float32
+ *               products), and every store rounds to float32;
+ *   T = double  (suffix f64): the same stencils replayed with float64 storage.
+ * Each loop is one reference kernel, run in the reference's launch order
+ * (seispie/solver/fd2d/solver.py:205-234), so the structure is unfused on
+ * purpose.  Expression order and the int64*float32 -> float64 promotions
+ * follow the reference source line by line; the file must be compiled with
+ * -ffp-contract=off so that no multiply-add is fused.
+ */
+
+#define CAT_(a, b) a##_##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, SFX)
+
+/* D-x: solver.py:22-27 (diff_x).  9*(a)/(8dx) - (b)/(24dx), a and b are
+ * float32 differences, the rest float64; 0 outside 2 <= i < nx-2. */
+static inline double FN(dxm)(const T *v, int i, long k, double dx, int nx, long nz)
+{
+    if (i >= 2 && i < nx - 2)
+        return 9.0 * (double)(T)(v[k] - v[k - nz]) / (8.0 * dx)
+             - (double)(T)(v[k + nz] - v[k - 2 * nz]) / (24.0 * dx);
+    return 0.0;
+}
+
+/* D+x: solver.py:30-35 (diff_x1); 0 outside 1 <= i < nx-2. */
+static inline double FN(dxp)(const T *v, int i, long k, double dx, int nx, long nz)
+{
+    if (i >= 1 && i < nx - 2)
+        return 9.0 * (double)(T)(v[k + nz] - v[k]) / (8.0 * dx)
+             - (double)(T)(v[k + 2 * nz] - v[k - nz]) / (24.0 * dx);
+    return 0.0;
+}
+
+/* D-z: solver.py:38-43 (diff_z); 0 outside 2 <= j < nz-2. */
+static inline double FN(dzm)(const T *v, int j, long k, double dz, int nz)
+{
+    if (j >= 2 && j < nz - 2)
+        return 9.0 * (double)(T)(v[k] - v[k - 1]) / (8.0 * dz)
+             - (double)(T)(v[k + 1] - v[k - 2]) / (24.0 * dz);
+    return 0.0;
+}
+
+/* D+z: solver.py:46-51 (diff_z1); 0 outside 1 <= j < nz-2. */
+static inline double FN(dzp)(const T *v, int j, long k, double dz, int nz)
+{
+    if (j >= 1 && j < nz - 2)
+        return 9.0 * (double)(T)(v[k + 1] - v[k]) / (8.0 * dz)
+             - (double)(T)(v[k + 2] - v[k - 1]) / (24.0 * dz);
+    return 0.0;
+}
+
+typedef struct {
+    long n;
+    /* SH (fd2d.py:232-233) */
+    T *vy, *uy, *sxy, *szy, *dsy, *dvydx, *dvydz;
+    /* P-SV (fd2d.py:235-239) */
+    T *vx, *vz, *ux, *uz, *sxx, *szz, *sxz, *dsx, *dsz, *dvxdx, *dvxdz, *dvzdx, *dvzdz;
+    /* micropolar spin (fd2d.py:241-242) */
+    T *vy_c, *uy_c, *syx_c, *syy_c, *syz_c, *dsy_c, *dvydx_c, *dvydz_c;
+    /* adjoint-side scratch for the interaction (fd2d.py:275-287) */
+    T *ax, *az, *bx, *bz, *cx, *cz, *ex, *ez;
+    /* material, promoted to T */
+    T *rho, *lam, *mu, *nu, *jj, *mu_c, *nu_c, *bound;
+} FN(state);
+
+static T *FN(zalloc)(long n) { return (T *)calloc((size_t)(n > 0 ? n : 1), sizeof(T)); }
+
+static void FN(state_init)(FN(state) *s, const orc_model *m)
+{
+    long n = (long)m->nx * m->nz, k;
+    memset(s, 0, sizeof(*s));
+    s->n = n;
+#define ALLOC(f) s->f = FN(zalloc)(n)
+    ALLOC(vy); ALLOC(uy); ALLOC(sxy); ALLOC(szy); ALLOC(dsy); ALLOC(dvydx); ALLOC(dvydz);
+    ALLOC(vx); ALLOC(vz); ALLOC(ux); ALLOC(uz); ALLOC(sxx); ALLOC(szz); ALLOC(sxz);
+    ALLOC(dsx); ALLOC(dsz); ALLOC(dvxdx); ALLOC(dvxdz); ALLOC(dvzdx); ALLOC(dvzdz);
+    ALLOC(vy_c); ALLOC(uy_c); ALLOC(syx_c); ALLOC(syy_c); ALLOC(syz_c); ALLOC(dsy_c);
+    ALLOC(dvydx_c); ALLOC(dvydz_c);
+    ALLOC(ax); ALLOC(az); ALLOC(bx); ALLOC(bz); ALLOC(cx); ALLOC(cz); ALLOC(ex); ALLOC(ez);
+    ALLOC(rho); ALLOC(lam); ALLOC(mu); ALLOC(nu); ALLOC(jj); ALLOC(mu_c); ALLOC(nu_c); ALLOC(bound);
+#undef ALLOC
+    for (k = 0; k < n; k++) {
+        s->rho[k] = (T)m->rho[k];
+        s->lam[k] = m->lam ? (T)m->lam[k] : 0;
+        s->mu[k] = (T)m->mu[k];
+        s->nu[k] = m->nu ? (T)m->nu[k] : 0;
+        s->jj[k] = m->jj ? (T)m->jj[k] : 0;
+        s->mu_c[k] = m->mu_c ? (T)m->mu_c[k] : 0;
+        s->nu_c[k] = m->nu_c ? (T)m->nu_c[k] : 0;
+        s->bound[k] = (T)m->bound[k];
+    }
+}
+
+static void FN(state_free)(FN(state) *s)
+{
+    T **p = &s->vy, **e = &s->bound;
+    for (; p <= e; p++) free(*p);
+}
+
+/* clear_wavefields: fd2d.py:318-341 */
+static void FN(clear)(FN(state) *s)
+{
+    size_t b = (size_t)s->n * sizeof(T);
+    memset(s->vy, 0, b); memset(s->uy, 0, b); memset(s->sxy, 0, b); memset(s->szy, 0, b);
+    memset(s->vx, 0, b); memset(s->vz, 0, b); memset(s->ux, 0, b); memset(s->uz, 0, b);
+    memset(s->sxx, 0, b); memset(s->szz, 0, b); memset(s->sxz, 0, b);
+    memset(s->vy_c, 0, b); memset(s->uy_c, 0, b); memset(s->syx_c, 0, b);
+    memset(s->syy_c, 0, b); memset(s->syz_c, 0, b);
+}
+
+/* div_sy: solver.py:54-58 */
+static void FN(div_sy)(T *dsy, const T *sxy, const T *szy, double dx, double dz, int nx, int nz)
+{
+    long n = (long)nx * nz, k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz);
+        dsy[k] = (T)(FN(dxm)(sxy, i, k, dx, nx, nz) + FN(dzm)(szy, j, k, dz, nz));
+    }
+}
+
+/* div_sxz: solver.py:61-66 */
+static void FN(div_sxz)(T *dsx, T *dsz, const T *sxx, const T *szz, const T *sxz,
+                        double dx, double dz, int nx, int nz)
+{
+    long n = (long)nx * nz, k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz);
+        dsx[k] = (T)(FN(dxm)(sxx, i, k, dx, nx, nz) + FN(dzm)(sxz, j, k, dz, nz));
+        dsz[k] = (T)(FN(dxm)(sxz, i, k, dx, nx, nz) + FN(dzm)(szz, j, k, dz, nz));
+    }
+}
+
+/* div_sxyz_c: solver.py:69-75 */
+static void FN(div_sxyz_c)(T *dsx, T *dsz, T *dsy_c, const T *syy_c, double dx, double dz,
+                           int nx, int nz)
+{
+    long n = (long)nx * nz, k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz);
+        dsx[k] = (T)((double)dsx[k] + FN(dzm)(syy_c, j, k, dz, nz));
+        dsz[k] = (T)((double)dsz[k] - FN(dxm)(syy_c, i, k, dx, nx, nz));
+        dsy_c[k] = (T)((double)dsy_c[k] + 2.0 * (double)syy_c[k]);
+    }
+}
+
+/* stf_dsy / stf_dsxz: solver.py:78-93.  One block per source in the
+ * reference; the sources are applied here in index order. */
+static void FN(stf_add)(T *d, const T *stf, const int *id, int nsrc, int isrc, int it, int nt)
+{
+    int b;
+    for (b = 0; b < nsrc; b++)
+        if (isrc < 0 || isrc == b) d[id[b]] = (T)(d[id[b]] + stf[(long)b * nt + it]);
+}
+
+/* add_vy: solver.py:95-100 (also used for the spin vy_c with j, :226) */
+static void FN(add_vy)(T *vy, T *uy, const T *dsy, const T *rho, const T *bound, double dt, long n)
+{
+    long k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        vy[k] = (T)((double)bound[k] * ((double)vy[k] + dt * (double)dsy[k] / (double)rho[k]));
+        uy[k] = (T)((double)uy[k] + (double)vy[k] * dt);
+    }
+}
+
+/* add_vxz: solver.py:102-109 */
+static void FN(add_vxz)(T *vx, T *vz, T *ux, T *uz, const T *dsx, const T *dsz, const T *rho,
+                        const T *bound, double dt, long n)
+{
+    long k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        vx[k] = (T)((double)bound[k] * ((double)vx[k] + dt * (double)dsx[k] / (double)rho[k]));
+        vz[k] = (T)((double)bound[k] * ((double)vz[k] + dt * (double)dsz[k] / (double)rho[k]));
+        ux[k] = (T)((double)ux[k] + (double)vx[k] * dt);
+        uz[k] = (T)((double)uz[k] + (double)vz[k] * dt);
+    }
+}
+
+/* div_vy: solver.py:112-117 */
+static void FN(div_vy)(T *dvydx, T *dvydz, const T *vy, double dx, double dz, int nx, int nz)
+{
+    long n = (long)nx * nz, k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz);
+        dvydx[k] = (T)FN(dxp)(vy, i, k, dx, nx, nz);
+        dvydz[k] = (T)FN(dzp)(vy, j, k, dz, nz);
+    }
+}
+
+/* div_vxz: solver.py:119-126 */
+static void FN(div_vxz)(T *dvxdx, T *dvxdz, T *dvzdx, T *dvzdz, const T *vx, const T *vz,
+                        double dx, double dz, int nx, int nz)
+{
+    long n = (long)nx * nz, k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz);
+        dvxdx[k] = (T)FN(dxp)(vx, i, k, dx, nx, nz);
+        dvxdz[k] = (T)FN(dzp)(vx, j, k, dz, nz);
+        dvzdx[k] = (T)FN(dxp)(vz, i, k, dx, nx, nz);
+        dvzdz[k] = (T)FN(dzp)(vz, j, k, dz, nz);
+    }
+}
+
+/* add_sy: solver.py:128-133 */
+static void FN(add_sy)(T *sxy, T *szy, const T *dvydx, const T *dvydz, const T *mu, double dt, long n)
+{
+    long k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        sxy[k] = (T)((double)sxy[k] + dt * (double)mu[k] * (double)dvydx[k]);
+        szy[k] = (T)((double)szy[k] + dt * (double)mu[k] * (double)dvydz[k]);
+    }
+}
+
+/* add_sy_c: solver.py:135-141 (lambda_c is never used by the reference) */
+static void FN(add_sy_c)(T *syx_c, T *syy_c, T *syz_c, const T *vy_c, const T *dvydx_c,
+                         const T *dvydz_c, const T *dvxdz, const T *dvzdx, const T *nu,
+                         const T *mu_c, const T *nu_c, double dt, long n)
+{
+    long k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        syy_c[k] = (T)((double)syy_c[k] + 2.0 * dt * (double)nu[k]
+                       * (-(double)vy_c[k] - 0.5 * (double)(T)(dvzdx[k] - dvxdz[k])));
+        syx_c[k] = (T)((double)syx_c[k] + dt * (double)(T)(mu_c[k] + nu_c[k]) * (double)dvydx_c[k]);
+        syz_c[k] = (T)((double)syz_c[k] + dt * (double)(T)(mu_c[k] + nu_c[k]) * (double)dvydz_c[k]);
+    }
+}
+
+/* add_sxz: solver.py:143-149.  lam*dvzdz and mu*(dvxdz+dvzdx) are float32
+ * products in the reference (float32 * float32 stays float32 in Numba). */
+static void FN(add_sxz)(T *sxx, T *szz, T *sxz, const T *dvxdx, const T *dvxdz, const T *dvzdx,
+                        const T *dvzdz, const T *lam, const T *mu, double dt, long n)
+{
+    long k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        double l2m = (double)lam[k] + 2.0 * (double)mu[k];
+        sxx[k] = (T)((double)sxx[k] + dt * (l2m * (double)dvxdx[k] + (double)(T)(lam[k] * dvzdz[k])));
+        szz[k] = (T)((double)szz[k] + dt * (l2m * (double)dvzdz[k] + (double)(T)(lam[k] * dvxdx[k])));
+        sxz[k] = (T)((double)sxz[k] + dt * (double)(T)(mu[k] * (T)(dvxdz[k] + dvzdx[k])));
+    }
+}
+
+/* save_obs: solver.py:151-156 */
+static void FN(save_obs)(T *obs, const T *u, const int *rec_id, int nrec, int it, int nt)
+{
+    int r;
+    for (r = 0; r < nrec; r++) obs[(long)r * nt + it] = u[rec_id[r]];
+}
+
+/* One time step, in the launch order of solver.py:205-234.  inj_* are the
+ * injection points and time functions (sources in the forward run,
+ * receivers with the adjoint sources in the adjoint run). */
+static void FN(step)(FN(state) *s, const orc_model *m, int ninj, const int *inj_id, int isrc,
+                     const T *fx, const T *fy, const T *fz, int it)
+{
+    int nx = m->nx, nz = m->nz, nt = m->nt;
+    long n = s->n;
+    double dx = (double)m->dx, dz = (double)m->dz, dt = m->dt;
+    if (m->physics == ORC_SH) {
+        FN(div_sy)(s->dsy, s->sxy, s->szy, dx, dz, nx, nz);
+        FN(stf_add)(s->dsy, fy, inj_id, ninj, isrc, it, nt);
+        FN(add_vy)(s->vy, s->uy, s->dsy, s->rho, s->bound, dt, n);
+        FN(div_vy)(s->dvydx, s->dvydz, s->vy, dx, dz, nx, nz);
+        FN(add_sy)(s->sxy, s->szy, s->dvydx, s->dvydz, s->mu, dt, n);
+        return;
+    }
+    FN(div_sxz)(s->dsx, s->dsz, s->sxx, s->szz, s->sxz, dx, dz, nx, nz);
+    if (m->physics == ORC_SPIN) {
+        FN(div_sy)(s->dsy_c, s->syx_c, s->syz_c, dx, dz, nx, nz);
+        FN(div_sxyz_c)(s->dsx, s->dsz, s->dsy_c, s->syy_c, dx, dz, nx, nz);
+    }
+    FN(stf_add)(s->dsx, fx, inj_id, ninj, isrc, it, nt);
+    FN(stf_add)(s->dsz, fz, inj_id, ninj, isrc, it, nt);
+    FN(add_vxz)(s->vx, s->vz, s->ux, s->uz, s->dsx, s->dsz, s->rho, s->bound, dt, n);
+    FN(div_vxz)(s->dvxdx, s->dvxdz, s->dvzdx, s->dvzdz, s->vx, s->vz, dx, dz, nx, nz);
+    FN(add_sxz)(s->sxx, s->szz, s->sxz, s->dvxdx, s->dvxdz, s->dvzdx, s->dvzdz, s->lam, s->mu, dt, n);
+    if (m->physics == ORC_SPIN) {
+        FN(add_vy)(s->vy_c, s->uy_c, s->dsy_c, s->jj, s->bound, dt, n);
+        FN(div_vy)(s->dvydx_c, s->dvydz_c, s->vy_c, dx, dz, nx, nz);
+        FN(add_sy_c)(s->syx_c, s->syy_c, s->syz_c, s->vy_c, s->dvydx_c, s->dvydz_c, s->dvxdz,
+                     s->dvzdx, s->nu, s->mu_c, s->nu_c, dt, n);
+    }
+}
+
+/* Snapshot slots per physics: SH (uy, vy); P-SV (ux, uz, vx, vz);
+ * spin (ux, uz, uy_c, vx, vz, vy_c).  u is copied before the step and v
+ * after it, as solver.py:193-203 and :236-242 do. */
+static void FN(snap_u)(const FN(state) *s, int phys, T *dst)
+{
+    size_t b = (size_t)s->n * sizeof(T);
+    if (phys == ORC_SH) { memcpy(dst, s->uy, b); return; }
+    memcpy(dst, s->ux, b); memcpy(dst + s->n, s->uz, b);
+    if (phys == ORC_SPIN) memcpy(dst + 2 * s->n, s->uy_c, b);
+}
+
+static void FN(snap_v)(const FN(state) *s, int phys, T *dst)
+{
+    size_t b = (size_t)s->n * sizeof(T);
+    if (phys == ORC_SH) { memcpy(dst + s->n, s->vy, b); return; }
+    if (phys == ORC_PSV) { memcpy(dst + 2 * s->n, s->vx, b); memcpy(dst + 3 * s->n, s->vz, b); return; }
+    memcpy(dst + 3 * s->n, s->vx, b); memcpy(dst + 4 * s->n, s->vz, b); memcpy(dst + 5 * s->n, s->vy_c, b);
+}
+
+/* run_forward: solver.py:166-242.  obs_* are [nrec][nt]; snaps (may be
+ * NULL) is [nt/sae][nsnap][N] in the reference's reversed order. */
+int FN(orc_forward)(const orc_model *m, const orc_geom *g, const T *stf_x, const T *stf_y,
+                    const T *stf_z, T *obs_x, T *obs_y, T *obs_z, int sae, T *snaps)
+{
+    FN(state) s;
+    int it, nt = m->nt, phys = m->physics;
+    int nsa = sae > 0 ? nt / sae : 0;
+    int nf = orc_nsnap(phys);
+    FN(state_init)(&s, m);
+    FN(clear)(&s);
+    for (it = 0; it < nt; it++) {
+        int isa = -1;
+        if (sae > 0 && snaps && (it + 1) % sae == 0) isa = nsa - (it + 1) / sae;
+        if (isa >= 0) FN(snap_u)(&s, phys, snaps + (long)isa * nf * s.n);
+        FN(step)(&s, m, g->nsrc, g->src_id, g->isrc, stf_x, stf_y, stf_z, it);
+        if (phys == ORC_SH) {
+            FN(save_obs)(obs_y, s.uy, g->rec_id, g->nrec, it, nt);
+        } else {
+            FN(save_obs)(obs_x, s.ux, g->rec_id, g->nrec, it, nt);
+            FN(save_obs)(obs_z, s.uz, g->rec_id, g->nrec, it, nt);
+            if (phys == ORC_SPIN) FN(save_obs)(obs_y, s.uy_c, g->rec_id, g->nrec, it, nt);
+        }
+        if (isa >= 0) FN(snap_v)(&s, phys, snaps + (long)isa * nf * s.n);
+    }
+    FN(state_free)(&s);
+    return 0;
+}
+
+/* Interaction terms, accumulated every sae adjoint steps.
+ *   SH: interaction_muy, solver.py:159-163, on D+ of the adjoint uy and of
+ *       the stored forward uy (solver.py:315-321), float32 products.
+ *   P-SV and spin have no reference implementation (solver.py:324-329 is
+ *   broken); they are this project's definitions, in the same style:
+ *     k_lam += div(u_adj) div(u) ndt
+ *     k_mu  += [2 exx_a exx + 2 ezz_a ezz + (dzux_a+dxuz_a)(dzux+dxuz)] ndt
+ *     k_rho += (vx_a vx + vz_a vz) ndt
+ *   and for spin, with E = -uy_c - (dxuz - dzux)/2:
+ *     k_nu  += 4 E_a E ndt  (syy_c = nu * 2E, so d/dnu pairs 2E_a with 2E);
+ *     k_j   += vy_c_a vy_c ndt;
+ *     k_muc += (dx phi_a dx phi + dz phi_a dz phi) ndt   (phi = uy_c;
+ *              the gradient for mu_c and for nu_c, which enter as a sum).
+ *   All of these are float64 sums of the float32 derivative fields. */
+static void FN(interact)(FN(state) *s, const orc_model *m, const T *snap, double ndt, T *klam,
+                         T *kmu, T *krho, T *knu, T *kj, T *kmuc)
+{
+    int nx = m->nx, nz = m->nz, phys = m->physics;
+    long n = s->n, k;
+    double dx = (double)m->dx, dz = (double)m->dz;
+    if (phys == ORC_SH) {
+        const T *ufw = snap;
+        FN(div_vy)(s->dvydx, s->dvydz, s->uy, dx, dz, nx, nz);
+        FN(div_vy)(s->ax, s->az, ufw, dx, dz, nx, nz);
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; k++)
+            kmu[k] = (T)((double)kmu[k]
+                         + (double)(T)((T)(s->dvydx[k] * s->ax[k]) + (T)(s->dvydz[k] * s->az[k])) * ndt);
+        return;
+    }
+    {
+        const T *uxf = snap, *uzf = snap + n;
+        const T *vxf = snap + (phys == ORC_SPIN ? 3 : 2) * n;
+        const T *vzf = snap + (phys == ORC_SPIN ? 4 : 3) * n;
+        FN(div_vxz)(s->dvxdx, s->dvxdz, s->dvzdx, s->dvzdz, s->ux, s->uz, dx, dz, nx, nz);
+        FN(div_vxz)(s->ax, s->az, s->bx, s->bz, uxf, uzf, dx, dz, nx, nz);
+        if (phys == ORC_SPIN) {
+            FN(div_vy)(s->dvydx_c, s->dvydz_c, s->uy_c, dx, dz, nx, nz);
+            FN(div_vy)(s->cx, s->cz, snap + 2 * n, dx, dz, nx, nz);
+        }
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; k++) {
+            /* adjoint: dvxdx=dx ux, dvxdz=dz ux, dvzdx=dx uz, dvzdz=dz uz;
+             * forward: ax=dx ux, az=dz ux, bx=dx uz, bz=dz uz */
+            double axx = s->dvxdx[k], axz = s->dvxdz[k], azx = s->dvzdx[k], azz = s->dvzdz[k];
+            double fxx = s->ax[k], fxz = s->az[k], fzx = s->bx[k], fzz = s->bz[k];
+            double diva = axx + azz, divf = fxx + fzz;
+            double sha = axz + azx, shf = fxz + fzx;
+            klam[k] = (T)((double)klam[k] + diva * divf * ndt);
+            kmu[k] = (T)((double)kmu[k] + (2.0 * axx * fxx + 2.0 * azz * fzz + sha * shf) * ndt);
+            krho[k] = (T)((double)krho[k]
+                          + ((double)s->vx[k] * (double)vxf[k] + (double)s->vz[k] * (double)vzf[k]) * ndt);
+            if (phys == ORC_SPIN) {
+                double ea = -(double)s->uy_c[k] - 0.5 * (azx - axz);
+                double ef = -(double)snap[2 * n + k] - 0.5 * (fzx - fxz);
+                knu[k] = (T)((double)knu[k] + 4.0 * ea * ef * ndt);
+                kj[k] = (T)((double)kj[k] + (double)s->vy_c[k] * (double)snap[5 * n + k] * ndt);
+                kmuc[k] = (T)((double)kmuc[k]
+                              + ((double)s->dvydx_c[k] * (double)s->cx[k]
+                                 + (double)s->dvydz_c[k] * (double)s->cz[k]) * ndt);
+            }
+        }
+    }
+}
+
+/* run_adjoint: solver.py:290-321 for SH; the same loop for P-SV and spin
+ * with the interaction above.  ad_* are the adjoint sources [nrec][nt],
+ * injected at the receivers for every step (isrc = -1, solver.py:311).
+ * Kernels are accumulated in place (k_* are not cleared here; the reference
+ * clears them once per run_kernel, fd2d.py:311-316). */
+int FN(orc_adjoint)(const orc_model *m, const orc_geom *g, const T *ad_x, const T *ad_y,
+                    const T *ad_z, int sae, const T *snaps, T *klam, T *kmu, T *krho, T *knu,
+                    T *kj, T *kmuc)
+{
+    FN(state) s;
+    int it, nt = m->nt, nf = orc_nsnap(m->physics);
+    double ndt = (double)sae * m->dt;
+    if (sae <= 0 || nt % sae) return -1;
+    FN(state_init)(&s, m);
+    FN(clear)(&s);
+    for (it = 0; it < nt; it++) {
+        FN(step)(&s, m, g->nrec, g->rec_id, -1, ad_x, ad_y, ad_z, it);
+        if ((it + sae) % sae == 0) {
+            int isae = (it + sae) / sae - 1;
+            FN(interact)(&s, m, snaps + (long)isae * nf * s.n, ndt, klam, kmu, krho, knu, kj, kmuc);
+        }
+    }
+    FN(state_free)(&s);
+    return 0;
+}
+
+/* diff kernel of misfit/waveform.py:5-25.  res and adstf are [nrec][nt].
+ * Returns sum(res^2) in float64 (waveform.py:34 takes its square root). */
+double FN(orc_waveform)(const T *syn, const T *obs, T *adstf, T *res, int nt, double dt, int nrec)
+{
+    double acc = 0.0;
+    int it, ir;
+    double t_end = (double)(nt - 1) * dt;
+    double tw = t_end / 10.0, t_min = tw, t_max = t_end - tw;
+    for (ir = 0; ir < nrec; ir++) {
+        for (it = 0; it < nt; it++) {
+            long kt = (long)ir * nt + it, akt = (long)ir * nt + nt - it - 1;
+            double t = (double)it * dt, taper;
+            if (t <= t_min) taper = 0.5 + 0.5 * cos(M_PI * (t_min - t) / tw);
+            else if (t >= t_max) taper = 0.5 + 0.5 * cos(M_PI * (t_max - t) / tw);
+            else taper = 1.0;
+            res[kt] = (T)((double)(T)(syn[kt] - obs[kt]) * taper);
+            adstf[akt] = (T)((double)res[kt] * taper * 2.0);
+            acc += (double)res[kt] * (double)res[kt];
+        }
+    }
+    return acc;
+}
+
+/* smooth = apply_gauxxian_x then apply_gauxxian_z with init_gausian's
+ * normaliser (fd2d.py:71-108, :349-352); in place on data. */
+void FN(orc_smooth)(T *data, int nx, int nz, int sigma)
+{
+    long n = (long)nx * nz, k;
+    T *gtmp = FN(zalloc)(n), *gsum = FN(zalloc)(n);
+    double s = (double)sigma;
+    double c = 1.0 / (sqrt(2.0 * M_PI) * s);
+#define GAUSS(x) (c * exp(-(double)((long)(x) * (long)(x)) / (2.0 * s * s)))
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz), q;
+        double sumx = 0.0, sumz = 0.0;
+        for (q = 0; q < nx; q++) sumx += GAUSS(i - q);
+        for (q = 0; q < nz; q++) sumz += GAUSS(j - q);
+        gsum[k] = (T)(sumx * sumz);
+    }
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz), q;
+        double sumx = 0.0;
+        for (q = 0; q < nx; q++) sumx += GAUSS(i - q) * (double)data[(long)q * nz + j];
+        gtmp[k] = (T)sumx;
+    }
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < n; k++) {
+        int j = (int)(k % nz), i = (int)(k / nz), q;
+        double sumz = 0.0;
+        for (q = 0; q < nz; q++) sumz += GAUSS(j - q) * (double)gtmp[(long)i * nz + q];
+        data[k] = (T)(sumz / (double)gsum[k]);
+    }
+#undef GAUSS
+    free(gtmp);
+    free(gsum);
+}
+
+#undef FN
+#undef CAT
+#undef CAT_

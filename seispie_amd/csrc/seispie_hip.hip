@@ -1,0 +1,1492 @@
+// seispie_hip -- MI355X (gfx950) solver hot path of seispie's fd2d plugin.
+//
+// What the reference does (seispie/solver/fd2d/solver.py) is a Python time
+// loop that launches 6 (SH), 7 (P-SV) or 13 (spin) Numba kernels per step,
+// one thread per cell, each kernel streaming whole arrays through HBM.
+// Here one launch advances every shot of a batch by one time step:
+//
+//   * k_step<PHYS>: each wavefront owns a strip of 64 consecutive z cells
+//     (z is the contiguous axis, k = i*nz + j) and marches along x through
+//     a chunk of rows.  It keeps a 4-row register window of the old
+//     stresses and of the new velocities, takes z neighbours with
+//     wavefront shuffles, and writes new velocity, displacement and stress
+//     for the 58 interior lanes.  Stress divergence, source injection,
+//     sponge, velocity/displacement update, velocity gradient, stress
+//     update, receiver recording and adjoint snapshots are all fused, so a
+//     cell's state is read and written once per step.  Stresses and
+//     velocities ping-pong between two buffers because neighbouring strips
+//     read the old values in their halo.
+//   * k_interact<PHYS>: kernel accumulation every sae adjoint steps.
+//   * k_misfit, k_smooth_x/z, k_vps2lm, ...: the smaller pieces.
+//
+// Numerics follow the reference: float32 storage, arithmetic in float64
+// wherever Numba promotes (int literal * float32, Python-float dt) and the
+// few float32 products it keeps, rounded to float32 at the points where
+// the reference stores an array.  The only difference is that 1/(8dx),
+// 1/(24dx) and 1/rho are applied as multiplications, which moves results
+// by at most one float32 rounding step on rare cells.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/seispie_hip.h"
+
+namespace {
+
+// ------------------------------------------------------------------ layout
+// Stress, velocity and displacement fields per physics.
+//   SH:   s = {sxy, szy}                         v = {vy}          u = {uy}
+//   P-SV: s = {sxx, szz, sxz}                    v = {vx, vz}      u = {ux, uz}
+//   spin: s = {sxx, szz, sxz, syx_c, syy_c, syz_c} v = {vx, vz, vy_c} u = {ux, uz, uy_c}
+// State slots of a shot: two stress sets, two velocity sets, one u set.
+__host__ __device__ constexpr int n_s(int p) { return p == SP_SH ? 2 : (p == SP_PSV ? 3 : 6); }
+__host__ __device__ constexpr int n_v(int p) { return p == SP_SH ? 1 : (p == SP_PSV ? 2 : 3); }
+__host__ __device__ constexpr int n_slots(int p) { return 2 * n_s(p) + 3 * n_v(p); }
+__host__ __device__ constexpr int slot_s(int p, int set, int f) { return set * n_s(p) + f; }
+__host__ __device__ constexpr int slot_v(int p, int set, int f) { return 2 * n_s(p) + set * n_v(p) + f; }
+__host__ __device__ constexpr int slot_u(int p, int f) { return 2 * n_s(p) + 2 * n_v(p) + f; }
+// snapshot fields: u (before the step) then v (after it), solver.py:193-242
+__host__ __device__ constexpr int n_snap(int p) { return 2 * n_v(p); }
+// components of the records (SH y; P-SV x, z; spin x, z, y_c), of the
+// injected sources (SH y; P-SV/spin x, z) and of the misfit.
+__host__ __device__ constexpr int n_rec(int p) { return n_v(p); }
+__host__ __device__ constexpr int n_inj(int p) { return p == SP_SH ? 1 : 2; }
+__host__ __device__ constexpr int n_mis(int p) { return p == SP_SH ? 1 : 2; }
+// material slots of a grid
+constexpr int M_RHO = SP_F_RHO, M_LAM = SP_F_LAM, M_MU = SP_F_MU, M_NU = SP_F_NU, M_J = SP_F_J;
+constexpr int M_GAM = 8;  // float32(mu_c + nu_c), as the reference adds them (solver.py:140)
+constexpr int N_MAT = 9;
+
+struct ShotDesc {
+    const float *mat;   // [N_MAT][Npad]
+    const float *bx;    // [nx] x-side sponge factors (already float32-rounded)
+    const double *bzb;  // [nz] bottom factor or 1
+    const double *bzt;  // [nz] top factor or 1
+    float *state;       // [n_slots][Npad]
+    float *rec;         // [n_rec][nrec][nt]
+    float *snap;        // [nsa][n_snap][Npad] or null
+    const float *stf;   // [n_inj][nsrc][nt]
+    float *adstf;       // [n_mis][nrec][nt]
+    const int *src_k, *src_row, *rec_k, *rec_row;  // sorted by k
+    int nsrc, nrec, nx, nz;
+    long long Npad;
+    double c1x, c2x, c1z, c2z;  // 9/(8h), 1/(24h)
+};
+
+struct StepArgs {
+    const ShotDesc *shots;
+    const int4 *items;  // (shot, j0, i0, i1) per wavefront
+    int nitems;
+    int it, nt;
+    double dt;
+    int rd;        // stress/velocity set read this step
+    int snap_idx;  // -1: no snapshot this step
+    int adjoint;   // inject adjoint sources at the receivers, record nothing
+};
+
+// float32 operations the reference performs in float32 (float32 * float32
+// stays float32 under Numba's typing); contraction off so they are never
+// folded into a float64 multiply-add.
+__device__ __forceinline__ float mul_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a - b;
+}
+
+// D- at row r from values at r-2, r-1, r, r+1 (solver.py:22-27, 38-43)
+__device__ __forceinline__ double dminus(float m2, float m1, float c0, float p1, double c1, double c2)
+{
+    return c1 * (double)sub_rn(c0, m1) - c2 * (double)sub_rn(p1, m2);
+}
+// D+ at row q from values at q-1, q, q+1, q+2 (solver.py:30-35, 46-51)
+__device__ __forceinline__ double dplus(float m1, float c0, float p1, float p2, double c1, double c2)
+{
+    return c1 * (double)sub_rn(p1, c0) - c2 * (double)sub_rn(p2, m1);
+}
+
+__device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, 64); }
+
+__device__ __forceinline__ int lower_bound_k(const int *a, int n, int key)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------ step kernel
+template <int P>
+__global__ __launch_bounds__(256) void k_step(StepArgs a)
+{
+    constexpr int NS = n_s(P), NV = n_v(P);
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= a.nitems) return;  // whole wavefront leaves together
+    const int L = threadIdx.x & 63;
+    const int4 item = a.items[wave];
+    const ShotDesc &d = a.shots[item.x];
+    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
+    const int j = item.y - 3 + L;
+    const bool jin = j >= 0 && j < nz;
+    const bool outl = jin && L >= 3 && L < 61;
+    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
+    const long long NP = d.Npad;
+    const double dt = a.dt, dt2 = 2.0 * a.dt;
+    const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
+    float *st = d.state;
+    const int rd = a.rd, wr = a.rd ^ 1;
+    const float *mat = d.mat;
+    const double bzb = jin ? d.bzb[j] : 1.0, bzt = jin ? d.bzt[j] : 1.0;
+
+    // injection list: sources (forward) or receivers with adjoint sources
+    const int *ik = a.adjoint ? d.rec_k : d.src_k;
+    const int *irow = a.adjoint ? d.rec_row : d.src_row;
+    const int ni = a.adjoint ? d.nrec : d.nsrc;
+    const float *ival = a.adjoint ? d.adstf : d.stf;
+    int pi = lower_bound_k(ik, ni, (i0 - 1) * nz);
+    int pr = lower_bound_k(d.rec_k, d.nrec, i0 * nz);
+    const bool record = !a.adjoint;
+
+    float S0[NS], S1[NS], S2[NS], S3[NS];
+    float W0[NV], W1[NV], W2[NV], W3[NV];
+#pragma unroll
+    for (int f = 0; f < NS; f++) S0[f] = S1[f] = S2[f] = S3[f] = 0.f;
+#pragma unroll
+    for (int f = 0; f < NV; f++) W0[f] = W1[f] = W2[f] = W3[f] = 0.f;
+
+    for (int m = i0 - 3; m <= i1 + 2; ++m) {
+        // ---- old stresses of row m into the window
+        const bool mrow = m >= 0 && m < nx;
+#pragma unroll
+        for (int f = 0; f < NS; f++) {
+            S0[f] = S1[f];
+            S1[f] = S2[f];
+            S2[f] = S3[f];
+            S3[f] = (mrow && jin) ? st[slot_s(P, rd, f) * NP + (long long)m * nz + j] : 0.f;
+        }
+        if (m < i0) continue;
+
+        // ---- new velocities of row r = m-1 (window rows r-2..r+1)
+        const int r = m - 1;
+        float vn[NV];
+#pragma unroll
+        for (int f = 0; f < NV; f++) vn[f] = 0.f;
+        if (r >= 0 && r < nx) {
+            const bool xm_ok = r >= 2 && r < nx - 2;
+            const long long k = (long long)r * nz + j;
+            float dsv[NV];
+            if constexpr (P == SP_SH) {
+                // div_sy (solver.py:54-58)
+                float zm2 = shfl(S2[1], L - 2), zm1 = shfl(S2[1], L - 1), zp1 = shfl(S2[1], L + 1);
+                double ddx = xm_ok ? dminus(S0[0], S1[0], S2[0], S3[0], c1x, c2x) : 0.0;
+                double ddz = zm_ok ? dminus(zm2, zm1, S2[1], zp1, c1z, c2z) : 0.0;
+                dsv[0] = (float)(ddx + ddz);
+            } else {
+                // div_sxz (solver.py:61-66)
+                float xzm2 = shfl(S2[2], L - 2), xzm1 = shfl(S2[2], L - 1), xzp1 = shfl(S2[2], L + 1);
+                float zzm2 = shfl(S2[1], L - 2), zzm1 = shfl(S2[1], L - 1), zzp1 = shfl(S2[1], L + 1);
+                double dxx = xm_ok ? dminus(S0[0], S1[0], S2[0], S3[0], c1x, c2x) : 0.0;
+                double dxxz = xm_ok ? dminus(S0[2], S1[2], S2[2], S3[2], c1x, c2x) : 0.0;
+                double dzxz = zm_ok ? dminus(xzm2, xzm1, S2[2], xzp1, c1z, c2z) : 0.0;
+                double dzzz = zm_ok ? dminus(zzm2, zzm1, S2[1], zzp1, c1z, c2z) : 0.0;
+                dsv[0] = (float)(dxx + dzxz);
+                dsv[1] = (float)(dxxz + dzzz);
+                if constexpr (P == SP_SPIN) {
+                    // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
+                    float yzm2 = shfl(S2[5], L - 2), yzm1 = shfl(S2[5], L - 1), yzp1 = shfl(S2[5], L + 1);
+                    float yym2 = shfl(S2[4], L - 2), yym1 = shfl(S2[4], L - 1), yyp1 = shfl(S2[4], L + 1);
+                    double dxyx = xm_ok ? dminus(S0[3], S1[3], S2[3], S3[3], c1x, c2x) : 0.0;
+                    double dzyz = zm_ok ? dminus(yzm2, yzm1, S2[5], yzp1, c1z, c2z) : 0.0;
+                    float dsy = (float)(dxyx + dzyz);
+                    double dzyy = zm_ok ? dminus(yym2, yym1, S2[4], yyp1, c1z, c2z) : 0.0;
+                    double dxyy = xm_ok ? dminus(S0[4], S1[4], S2[4], S3[4], c1x, c2x) : 0.0;
+                    dsv[0] = (float)((double)dsv[0] + dzyy);
+                    dsv[1] = (float)((double)dsv[1] - dxyy);
+                    dsv[2] = (float)((double)dsy + 2.0 * (double)S2[4]);
+                }
+            }
+            // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list order
+            const int kend = (r + 1) * nz, kr = r * nz + j;
+            while (pi < ni && ik[pi] < kend) {
+                if (ik[pi] == kr) {
+                    const int row = irow[pi];
+                    if constexpr (P == SP_SH) {
+                        dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + a.it]);
+                    } else {
+                        const long long nr = a.adjoint ? d.nrec : d.nsrc;
+                        dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + a.it]);
+                        dsv[1] = add_rn(dsv[1], ival[(nr + row) * a.nt + a.it]);
+                    }
+                }
+                ++pi;
+            }
+            float un[NV];
+#pragma unroll
+            for (int f = 0; f < NV; f++) un[f] = 0.f;
+            if (jin) {
+                // add_vy / add_vxz (solver.py:95-109) with the separable sponge
+                // of set_bound (fd2d.py:47-68): f32(f32(bx*bottom)*top)
+                const float b1 = (float)((double)d.bx[r] * bzb);
+                const double b = (double)(float)((double)b1 * bzt);
+                const double dtr = dt / (double)mat[M_RHO * NP + k];
+#pragma unroll
+                for (int f = 0; f < NV; f++) {
+                    double dtm = dtr;
+                    if constexpr (P == SP_SPIN) {
+                        if (f == 2) dtm = dt / (double)mat[M_J * NP + k];
+                    }
+                    const float vo = st[slot_v(P, rd, f) * NP + k];
+                    vn[f] = (float)(b * ((double)vo + (double)dsv[f] * dtm));
+                }
+                if (outl && r >= i0 && r < i1) {
+#pragma unroll
+                    for (int f = 0; f < NV; f++) {
+                        float *up = st + slot_u(P, f) * NP + k;
+                        const float uo = *up;
+                        un[f] = (float)((double)uo + (double)vn[f] * dt);
+                        *up = un[f];
+                        st[slot_v(P, wr, f) * NP + k] = vn[f];
+                        if (a.snap_idx >= 0) {
+                            float *sp = d.snap + (long long)a.snap_idx * n_snap(P) * NP + k;
+                            sp[f * NP] = uo;
+                            sp[(NV + f) * NP] = vn[f];
+                        }
+                    }
+                }
+            }
+            // save_obs (solver.py:151-156): displacement after the step
+            if (record && r >= i0 && r < i1) {
+                while (pr < d.nrec && d.rec_k[pr] < kend) {
+                    if (outl && d.rec_k[pr] == kr) {
+                        const int row = d.rec_row[pr];
+#pragma unroll
+                        for (int f = 0; f < NV; f++)
+                            d.rec[((long long)f * d.nrec + row) * a.nt + a.it] = un[f];
+                    }
+                    ++pr;
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NV; f++) {
+            W0[f] = W1[f];
+            W1[f] = W2[f];
+            W2[f] = W3[f];
+            W3[f] = vn[f];
+        }
+
+        // ---- new stresses of row q = m-3 (velocity window rows q-1..q+2)
+        const int q = m - 3;
+        if (q < i0) continue;
+        const bool xp_ok = q >= 1 && q < nx - 2;
+        const long long kq = (long long)q * nz + j;
+        if constexpr (P == SP_SH) {
+            // div_vy + add_sy (solver.py:112-117, 128-133)
+            float zm1 = shfl(W1[0], L - 1), zp1 = shfl(W1[0], L + 1), zp2 = shfl(W1[0], L + 2);
+            if (outl) {
+                const float dvdx = xp_ok ? (float)dplus(W0[0], W1[0], W2[0], W3[0], c1x, c2x) : 0.f;
+                const float dvdz = zp_ok ? (float)dplus(zm1, W1[0], zp1, zp2, c1z, c2z) : 0.f;
+                const double dtmu = dt * (double)mat[M_MU * NP + kq];
+                st[slot_s(P, wr, 0) * NP + kq] = (float)((double)S0[0] + dtmu * (double)dvdx);
+                st[slot_s(P, wr, 1) * NP + kq] = (float)((double)S0[1] + dtmu * (double)dvdz);
+            }
+        } else {
+            // div_vxz + add_sxz (solver.py:119-126, 143-149)
+            float xm1 = shfl(W1[0], L - 1), xp1 = shfl(W1[0], L + 1), xp2 = shfl(W1[0], L + 2);
+            float zm1 = shfl(W1[1], L - 1), zp1 = shfl(W1[1], L + 1), zp2 = shfl(W1[1], L + 2);
+            float cm1 = 0.f, cp1 = 0.f, cp2 = 0.f;
+            if constexpr (P == SP_SPIN) {
+                cm1 = shfl(W1[2], L - 1);
+                cp1 = shfl(W1[2], L + 1);
+                cp2 = shfl(W1[2], L + 2);
+            }
+            if (outl) {
+                const float dvxdx = xp_ok ? (float)dplus(W0[0], W1[0], W2[0], W3[0], c1x, c2x) : 0.f;
+                const float dvzdx = xp_ok ? (float)dplus(W0[1], W1[1], W2[1], W3[1], c1x, c2x) : 0.f;
+                const float dvxdz = zp_ok ? (float)dplus(xm1, W1[0], xp1, xp2, c1z, c2z) : 0.f;
+                const float dvzdz = zp_ok ? (float)dplus(zm1, W1[1], zp1, zp2, c1z, c2z) : 0.f;
+                const float lam = mat[M_LAM * NP + kq], mu = mat[M_MU * NP + kq];
+                const double l2m = (double)lam + 2.0 * (double)mu;
+                st[slot_s(P, wr, 0) * NP + kq] =
+                    (float)((double)S0[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz)));
+                st[slot_s(P, wr, 1) * NP + kq] =
+                    (float)((double)S0[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx)));
+                st[slot_s(P, wr, 2) * NP + kq] =
+                    (float)((double)S0[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+                if constexpr (P == SP_SPIN) {
+                    // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
+                    const float dcdx = xp_ok ? (float)dplus(W0[2], W1[2], W2[2], W3[2], c1x, c2x) : 0.f;
+                    const float dcdz = zp_ok ? (float)dplus(cm1, W1[2], cp1, cp2, c1z, c2z) : 0.f;
+                    const float nu = mat[M_NU * NP + kq];
+                    const double dtg = dt * (double)mat[M_GAM * NP + kq];
+                    st[slot_s(P, wr, 4) * NP + kq] =
+                        (float)((double)S0[4] + dt2 * (double)nu
+                                * (-(double)W1[2] - 0.5 * (double)sub_rn(dvzdx, dvxdz)));
+                    st[slot_s(P, wr, 3) * NP + kq] = (float)((double)S0[3] + dtg * (double)dcdx);
+                    st[slot_s(P, wr, 5) * NP + kq] = (float)((double)S0[5] + dtg * (double)dcdz);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------ interaction kernel
+struct InteractArgs {
+    const ShotDesc *shots;
+    const int *list;  // shots of this grid in the batch
+    int nlist;
+    int nx, nz;
+    long long Npad;
+    float *kern;      // [SP_NKERN][Npad]
+    int cur;          // velocity set holding the adjoint velocity
+    int isae;
+    double ndt;
+    double c1x, c2x, c1z, c2z;
+};
+
+__device__ __forceinline__ float dpx(const float *u, int i, long long k, int nx, int nz, double c1, double c2)
+{
+    if (i >= 1 && i < nx - 2) return (float)dplus(u[k - nz], u[k], u[k + nz], u[k + 2 * (long long)nz], c1, c2);
+    return 0.f;
+}
+__device__ __forceinline__ float dpz(const float *u, int j, long long k, int nz, double c1, double c2)
+{
+    if (j >= 1 && j < nz - 2) return (float)dplus(u[k - 1], u[k], u[k + 1], u[k + 2], c1, c2);
+    return 0.f;
+}
+
+// SH: interaction_muy on D+ of the adjoint and the forward uy
+// (solver.py:159-163, 315-321).  P-SV and spin: this library's kernels,
+// accumulated in float64 from the float32 derivative fields:
+//   k_lam += div_a div ndt;  k_mu += (2 exx_a exx + 2 ezz_a ezz + g_a g) ndt
+//   k_rho += (vx_a vx + vz_a vz) ndt;  spin: k_nu += 4 E_a E ndt,
+//   k_j += vyc_a vyc ndt, k_muc += grad phi_a . grad phi ndt,
+//   with g = dz ux + dx uz and E = -uy_c - (dx uz - dz ux)/2.
+template <int P>
+__global__ __launch_bounds__(256) void k_interact(InteractArgs a)
+{
+    const long long N = (long long)a.nx * a.nz;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k / a.nz), j = (int)(k - (long long)i * a.nz);
+    const int nx = a.nx, nz = a.nz;
+    const long long NP = a.Npad;
+    float *K = a.kern;
+    if constexpr (P == SP_SH) {
+        float km = K[SP_K_MU * NP + k];
+        for (int s = 0; s < a.nlist; s++) {
+            const ShotDesc &d = a.shots[a.list[s]];
+            const float *ua = d.state + slot_u(P, 0) * NP;
+            const float *uf = d.snap + (long long)a.isae * n_snap(P) * NP;
+            const float ax = dpx(ua, i, k, nx, nz, a.c1x, a.c2x), fx = dpx(uf, i, k, nx, nz, a.c1x, a.c2x);
+            const float az = dpz(ua, j, k, nz, a.c1z, a.c2z), fz = dpz(uf, j, k, nz, a.c1z, a.c2z);
+            km = (float)((double)km + (double)add_rn(mul_rn(ax, fx), mul_rn(az, fz)) * a.ndt);
+        }
+        K[SP_K_MU * NP + k] = km;
+        return;
+    }
+    double kl = K[SP_K_LAM * NP + k], kmu = K[SP_K_MU * NP + k], kr = K[SP_K_RHO * NP + k];
+    double knu = 0, kj = 0, kc = 0;
+    if constexpr (P == SP_SPIN) {
+        knu = K[SP_K_NU * NP + k];
+        kj = K[SP_K_J * NP + k];
+        kc = K[SP_K_MUC * NP + k];
+    }
+    for (int s = 0; s < a.nlist; s++) {
+        const ShotDesc &d = a.shots[a.list[s]];
+        const float *uxa = d.state + slot_u(P, 0) * NP, *uza = d.state + slot_u(P, 1) * NP;
+        const float *sn = d.snap + (long long)a.isae * n_snap(P) * NP;
+        const float *uxf = sn, *uzf = sn + NP;
+        const double axx = dpx(uxa, i, k, nx, nz, a.c1x, a.c2x), axz = dpz(uxa, j, k, nz, a.c1z, a.c2z);
+        const double azx = dpx(uza, i, k, nx, nz, a.c1x, a.c2x), azz = dpz(uza, j, k, nz, a.c1z, a.c2z);
+        const double fxx = dpx(uxf, i, k, nx, nz, a.c1x, a.c2x), fxz = dpz(uxf, j, k, nz, a.c1z, a.c2z);
+        const double fzx = dpx(uzf, i, k, nx, nz, a.c1x, a.c2x), fzz = dpz(uzf, j, k, nz, a.c1z, a.c2z);
+        const double vxa = d.state[slot_v(P, a.cur, 0) * NP + k], vza = d.state[slot_v(P, a.cur, 1) * NP + k];
+        const double vxf = sn[(n_v(P) + 0) * NP + k], vzf = sn[(n_v(P) + 1) * NP + k];
+        // the same expressions, in the same order, as the oracle
+        const double diva = axx + azz, divf = fxx + fzz, sha = axz + azx, shf = fxz + fzx;
+        kl = (double)(float)(kl + diva * divf * a.ndt);
+        kmu = (double)(float)(kmu + (2.0 * axx * fxx + 2.0 * azz * fzz + sha * shf) * a.ndt);
+        kr = (double)(float)(kr + (vxa * vxf + vza * vzf) * a.ndt);
+        if constexpr (P == SP_SPIN) {
+            const float *uca = d.state + slot_u(P, 2) * NP, *ucf = sn + 2 * NP;
+            const double ea = -(double)uca[k] - 0.5 * (azx - axz);
+            const double ef = -(double)ucf[k] - 0.5 * (fzx - fxz);
+            knu = (double)(float)(knu + 4.0 * ea * ef * a.ndt);
+            const double vca = d.state[slot_v(P, a.cur, 2) * NP + k], vcf = sn[5 * NP + k];
+            kj = (double)(float)(kj + vca * vcf * a.ndt);
+            const double cax = dpx(uca, i, k, nx, nz, a.c1x, a.c2x), caz = dpz(uca, j, k, nz, a.c1z, a.c2z);
+            const double cfx = dpx(ucf, i, k, nx, nz, a.c1x, a.c2x), cfz = dpz(ucf, j, k, nz, a.c1z, a.c2z);
+            kc = (double)(float)(kc + (cax * cfx + caz * cfz) * a.ndt);
+        }
+    }
+    K[SP_K_LAM * NP + k] = (float)kl;
+    K[SP_K_MU * NP + k] = (float)kmu;
+    K[SP_K_RHO * NP + k] = (float)kr;
+    if constexpr (P == SP_SPIN) {
+        K[SP_K_NU * NP + k] = (float)knu;
+        K[SP_K_J * NP + k] = (float)kj;
+        K[SP_K_MUC * NP + k] = (float)kc;
+    }
+}
+
+// ---------------------------------------------------------- misfit kernel
+// diff kernel of misfit/waveform.py:5-25 plus the norm's partial sums.
+// One block per (shot, component, receiver) triple of the batch.
+struct MisfitArgs {
+    const ShotDesc *shots;
+    const int4 *triples;  // (shot, comp, rec, -)
+    const float *const *data;  // per shot, [n_rec][nrec][nt]
+    double *partial;
+    int n, nt;
+    double dt;
+};
+
+__global__ __launch_bounds__(256) void k_misfit(MisfitArgs a)
+{
+    const int b = blockIdx.x;
+    if (b >= a.n) return;
+    const int4 t = a.triples[b];
+    const ShotDesc &d = a.shots[t.x];
+    const int nt = a.nt;
+    const long long base = ((long long)t.y * d.nrec + t.z) * nt;
+    const float *syn = a.data[t.x] + base;  // observed data ("syn" in fd2d.py:305)
+    const float *obs = d.rec + base;         // current synthetics
+    float *ad = d.adstf + base;
+    const double t_end = (double)(nt - 1) * a.dt;
+    const double tw = t_end / 10.0, t_min = tw, t_max = t_end - tw;
+    double acc = 0.0;
+    for (int it = threadIdx.x; it < nt; it += blockDim.x) {
+        const double tt = (double)it * a.dt;
+        double taper;
+        if (tt <= t_min) taper = 0.5 + 0.5 * cos(M_PI * (t_min - tt) / tw);
+        else if (tt >= t_max) taper = 0.5 + 0.5 * cos(M_PI * (t_max - tt) / tw);
+        else taper = 1.0;
+        const float res = (float)((double)sub_rn(syn[it], obs[it]) * taper);
+        ad[nt - it - 1] = (float)((double)res * taper * 2.0);
+        acc += (double)res * (double)res;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[b] = red[0];
+}
+
+// ------------------------------------------------------- smoothing kernels
+// apply_gauxxian_x / _z with init_gausian's normaliser (fd2d.py:71-108).
+// g[d] = exp(-d^2/(2 s^2)) / (sqrt(2 pi) s) for |i-n| = d, in float64.
+__global__ __launch_bounds__(256) void k_smooth_x(const float *data, float *gtmp, const double *g, int nx, int nz)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)nx * nz) return;
+    const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
+    double s = 0.0;
+    for (int n = 0; n < nx; n++) s += g[i > n ? i - n : n - i] * (double)data[(long long)n * nz + j];
+    gtmp[k] = (float)s;
+}
+
+__global__ __launch_bounds__(256) void k_smooth_z(float *data, const float *gtmp, const double *g,
+                                                  const double *sumx, const double *sumz, int nx, int nz)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)nx * nz) return;
+    const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
+    const float *row = gtmp + (long long)i * nz;
+    double s = 0.0;
+    for (int n = 0; n < nz; n++) s += g[j > n ? j - n : n - j] * (double)row[n];
+    const float gsum = (float)(sumx[i] * sumz[j]);
+    data[k] = (float)(s / (double)gsum);
+}
+
+// ---------------------------------------------------------- small kernels
+__global__ void k_fill(float *p, long long n, float v)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// vps2lm: fd2d.py:21-33 (lam holds vp and mu holds vs on entry)
+__global__ void k_vps2lm(float *lam, float *mu, const float *rho, long long n)
+{
+#pragma clang fp contract(off)
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float vp = lam[k], vs = mu[k];
+    if (vp > vs) lam[k] = (float)((double)rho[k] * ((double)(vp * vp) - 2.0 * (double)vs * (double)vs));
+    else lam[k] = 0.f;
+    mu[k] = rho[k] * vs * vs;
+}
+
+__global__ void k_gamma(float *gam, const float *muc, const float *nuc, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) gam[k] = add_rn(muc[k], nuc[k]);
+}
+
+// vp/vs/rho kernels from lam/mu/rho kernels: lam = rho (vp^2 - 2 vs^2),
+// mu = rho vs^2 (the parameterisation of vps2lm).
+__global__ void k_kern_vps(float *K, const float *mat, long long n, long long NP)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double rho = mat[M_RHO * NP + k], lam = mat[M_LAM * NP + k], mu = mat[M_MU * NP + k];
+    const double vp = sqrt((lam + 2.0 * mu) / rho), vs = sqrt(mu / rho);
+    const double kl = K[SP_K_LAM * NP + k], km = K[SP_K_MU * NP + k], kr = K[SP_K_RHO * NP + k];
+    K[SP_K_VP * NP + k] = (float)(2.0 * rho * vp * kl);
+    K[SP_K_VS * NP + k] = (float)(2.0 * rho * vs * (km - 2.0 * kl));
+    K[SP_K_RHOP * NP + k] = (float)((vp * vp - 2.0 * vs * vs) * kl + vs * vs * km + kr);
+}
+
+// ===================================================================== host
+struct Grid {
+    int nx = 0, nz = 0;
+    float dx = 0, dz = 0;
+    long long N = 0, Npad = 0;
+    float *mat = nullptr;   // [N_MAT][Npad]
+    float *bx = nullptr;    // [nx]
+    double *bz = nullptr;   // [2][nz]: bottom, top
+    float *kern = nullptr;  // [SP_NKERN][Npad]
+};
+
+struct Shot {
+    int grid = 0, nsrc = 0, nrec = 0;
+    int *d_idx = nullptr;  // src_k, src_row, rec_k, rec_row
+    float *stf = nullptr, *state = nullptr, *rec = nullptr, *data = nullptr, *adstf = nullptr;
+    float *snap = nullptr;  // into the context's snapshot pool
+};
+
+}  // namespace
+
+struct sp_ctx {
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    int physics = 0, nt = 0;
+    double dt = 0;
+    std::vector<Grid> grids;
+    std::vector<Shot> shots;
+    std::string err;
+    // tuning
+    int chunk = 0;         // 0: automatic
+    int order = 1;
+    int graph = 0;
+    long long snap_mb = 0;
+    // snapshot pool
+    float *snap_pool = nullptr;
+    size_t snap_pool_bytes = 0;
+    int snaps_valid = 0, snaps_sae = 0;  // snapshots of every shot held from the last sp_forward
+    // batch descriptors
+    ShotDesc *d_desc = nullptr;
+    size_t d_desc_cap = 0;
+    int4 *d_items = nullptr;
+    size_t d_items_cap = 0;
+    int *d_list = nullptr;
+    size_t d_list_cap = 0;
+    // timing of the last run
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    long long t_launches = 0;
+    double t_step_ms = 0, t_total_ms = 0, t_bytes = 0;
+};
+
+namespace {
+
+#define HIPCHECK(ctx, call)                                                                  \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                   \
+            return 2;                                                                        \
+        }                                                                                    \
+    } while (0)
+
+int fail(sp_ctx *ctx, const std::string &msg)
+{
+    ctx->err = msg;
+    return 1;
+}
+
+long long pad64(long long n) { return (n + 63) / 64 * 64; }
+
+int dcopy(sp_ctx *ctx, void *dst, const void *src, size_t bytes, int dst_dev, int src_dev)
+{
+    hipMemcpyKind kind = dst_dev ? (src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice)
+                                 : (src_dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost);
+    HIPCHECK(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    if (!dst_dev || !src_dev) HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fill(sp_ctx *ctx, float *p, long long n, float v)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n, v);
+    HIPCHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int ensure(sp_ctx *ctx, T *&p, size_t &cap, size_t n)
+{
+    if (n <= cap) return 0;
+    if (p) HIPCHECK(ctx, hipFree(p));
+    p = nullptr;
+    HIPCHECK(ctx, hipMalloc(&p, n * sizeof(T)));
+    cap = n;
+    return 0;
+}
+
+int check_ctx(sp_ctx *ctx)
+{
+    if (!ctx) return 3;
+    if (!ctx->physics) return fail(ctx, "sp_set_physics has not been called");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    return 0;
+}
+
+// sizes of one shot's buffers
+long long rec_floats(const sp_ctx *c, const Shot &s) { return (long long)n_rec(c->physics) * s.nrec * c->nt; }
+long long adj_floats(const sp_ctx *c, const Shot &s) { return (long long)n_mis(c->physics) * s.nrec * c->nt; }
+size_t snap_bytes(const sp_ctx *c, const Shot &s, int sae)
+{
+    return (size_t)(c->nt / sae) * n_snap(c->physics) * (size_t)c->grids[s.grid].Npad * sizeof(float);
+}
+
+ShotDesc make_desc(const sp_ctx *c, const Shot &s)
+{
+    const Grid &g = c->grids[s.grid];
+    ShotDesc d;
+    d.mat = g.mat;
+    d.bx = g.bx;
+    d.bzb = g.bz;
+    d.bzt = g.bz + g.nz;
+    d.state = s.state;
+    d.rec = s.rec;
+    d.snap = s.snap;
+    d.stf = s.stf;
+    d.adstf = s.adstf;
+    d.src_k = s.d_idx;
+    d.src_row = s.d_idx + s.nsrc;
+    d.rec_k = s.d_idx + 2 * s.nsrc;
+    d.rec_row = s.d_idx + 2 * s.nsrc + s.nrec;
+    d.nsrc = s.nsrc;
+    d.nrec = s.nrec;
+    d.nx = g.nx;
+    d.nz = g.nz;
+    d.Npad = g.Npad;
+    d.c1x = 9.0 / (8.0 * (double)g.dx);
+    d.c2x = 1.0 / (24.0 * (double)g.dx);
+    d.c1z = 9.0 / (8.0 * (double)g.dz);
+    d.c2z = 1.0 / (24.0 * (double)g.dz);
+    return d;
+}
+
+int auto_chunk(const sp_ctx *c, const std::vector<int> &batch)
+{
+    if (c->chunk > 0) return c->chunk;
+    // enough wavefronts to fill 256 CUs several times over, long chunks otherwise
+    long long waves64 = 0;
+    for (int s : batch) {
+        const Grid &g = c->grids[c->shots[s].grid];
+        waves64 += (long long)((g.nz + 57) / 58) * ((g.nx + 63) / 64);
+    }
+    if (waves64 >= 8192) return 64;
+    if (waves64 >= 2048) return 32;
+    return 16;
+}
+
+// Upload descriptors and the wavefront work list of a batch.
+int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
+{
+    std::vector<ShotDesc> desc;
+    for (int s : batch) desc.push_back(make_desc(c, c->shots[s]));
+    const int chunk = auto_chunk(c, batch);
+    std::vector<int4> items;
+    if (c->order == 1) {
+        // shot-inner: consecutive wavefronts of a block share a tile of the
+        // material and differ in shot (same grid) or are neighbours in z
+        int maxs = 0, maxc = 0;
+        for (int s : batch) {
+            const Grid &g = c->grids[c->shots[s].grid];
+            maxs = std::max(maxs, (g.nz + 57) / 58);
+            maxc = std::max(maxc, (g.nx + chunk - 1) / chunk);
+        }
+        for (int ch = 0; ch < maxc; ch++)
+            for (int sp = 0; sp < maxs; sp++)
+                for (int b = 0; b < (int)batch.size(); b++) {
+                    const Grid &g = c->grids[c->shots[batch[b]].grid];
+                    const int i0 = ch * chunk, j0 = sp * 58;
+                    if (i0 >= g.nx || j0 >= g.nz) continue;
+                    items.push_back(make_int4(b, j0, i0, std::min(i0 + chunk, g.nx)));
+                }
+    } else {
+        for (int b = 0; b < (int)batch.size(); b++) {
+            const Grid &g = c->grids[c->shots[batch[b]].grid];
+            for (int i0 = 0; i0 < g.nx; i0 += chunk)
+                for (int j0 = 0; j0 < g.nz; j0 += 58)
+                    items.push_back(make_int4(b, j0, i0, std::min(i0 + chunk, g.nx)));
+        }
+    }
+    if (ensure(c, c->d_desc, c->d_desc_cap, desc.size())) return 2;
+    if (ensure(c, c->d_items, c->d_items_cap, items.size())) return 2;
+    HIPCHECK(c, hipMemcpyAsync(c->d_desc, desc.data(), desc.size() * sizeof(ShotDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    *nitems_out = (int)items.size();
+    return 0;
+}
+
+int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint)
+{
+    StepArgs a;
+    a.shots = c->d_desc;
+    a.items = c->d_items;
+    a.nitems = nitems;
+    a.it = it;
+    a.nt = c->nt;
+    a.dt = c->dt;
+    a.rd = it & 1;
+    a.snap_idx = snap_idx;
+    a.adjoint = adjoint;
+    dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
+    switch (c->physics) {
+    case SP_SH: hipLaunchKernelGGL(k_step<SP_SH>, grid, block, 0, c->stream, a); break;
+    case SP_PSV: hipLaunchKernelGGL(k_step<SP_PSV>, grid, block, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_step<SP_SPIN>, grid, block, 0, c->stream, a); break;
+    }
+    HIPCHECK(c, hipGetLastError());
+    c->t_launches++;
+    return 0;
+}
+
+double step_alg_bytes(const sp_ctx *c, const std::vector<int> &batch)
+{
+    // 8 B per state field per shot cell, 4 B per material field per grid cell
+    const int nst = n_s(c->physics) + 2 * n_v(c->physics);
+    const int nmat = c->physics == SP_SH ? 2 : (c->physics == SP_PSV ? 3 : 7);
+    std::vector<int> seen(c->grids.size(), 0);
+    double b = 0;
+    for (int s : batch) {
+        const Grid &g = c->grids[c->shots[s].grid];
+        b += 8.0 * nst * (double)g.N;
+        if (!seen[c->shots[s].grid]) {
+            seen[c->shots[s].grid] = 1;
+            b += 4.0 * nmat * (double)g.N;
+        }
+    }
+    return b;
+}
+
+int clear_state(sp_ctx *c, const std::vector<int> &batch)
+{
+    for (int s : batch)
+        if (fill(c, c->shots[s].state, (long long)n_slots(c->physics) * c->grids[c->shots[s].grid].Npad, 0.f)) return 2;
+    return 0;
+}
+
+// forward run of a batch (solver.py:166-242); snapshots when sae > 0
+int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
+{
+    int nitems = 0;
+    if (prepare_batch(c, batch, &nitems)) return 2;
+    if (clear_state(c, batch)) return 2;
+    const int nsa = sae > 0 ? c->nt / sae : 0;
+    for (int it = 0; it < c->nt; it++) {
+        int isa = -1;
+        if (sae > 0 && (it + 1) % sae == 0) isa = nsa - (it + 1) / sae;
+        if (launch_step(c, nitems, it, isa, 0)) return 2;
+    }
+    c->t_bytes += step_alg_bytes(c, batch) * c->nt;
+    return 0;
+}
+
+int run_misfit_batch(sp_ctx *c, const std::vector<int> &batch, double *norms_out)
+{
+    int nitems = 0;
+    if (prepare_batch(c, batch, &nitems)) return 2;
+    const int ncm = n_mis(c->physics);
+    std::vector<int4> trip;
+    std::vector<const float *> data;
+    for (int b = 0; b < (int)batch.size(); b++) {
+        const Shot &s = c->shots[batch[b]];
+        if (!s.data) return fail(c, "sp_set_data has not been called for shot " + std::to_string(batch[b]));
+        data.push_back(s.data);
+        for (int cc = 0; cc < ncm; cc++)
+            for (int r = 0; r < s.nrec; r++) trip.push_back(make_int4(b, cc, r, 0));
+    }
+    if (trip.empty()) {
+        for (int b = 0; b < (int)batch.size(); b++)
+            for (int cc = 0; cc < ncm; cc++) norms_out[(size_t)batch[b] * ncm + cc] = 0.0;
+        return 0;
+    }
+    int4 *d_trip = nullptr;
+    const float **d_data = nullptr;
+    double *d_part = nullptr;
+    HIPCHECK(c, hipMalloc(&d_trip, trip.size() * sizeof(int4)));
+    HIPCHECK(c, hipMalloc(&d_data, data.size() * sizeof(float *)));
+    HIPCHECK(c, hipMalloc(&d_part, trip.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpyAsync(d_trip, trip.data(), trip.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_data, data.data(), data.size() * sizeof(float *), hipMemcpyHostToDevice, c->stream));
+    MisfitArgs a;
+    a.shots = c->d_desc;
+    a.triples = d_trip;
+    a.data = d_data;
+    a.partial = d_part;
+    a.n = (int)trip.size();
+    a.nt = c->nt;
+    a.dt = c->dt;
+    hipLaunchKernelGGL(k_misfit, dim3((unsigned)trip.size()), dim3(256), 0, c->stream, a);
+    HIPCHECK(c, hipGetLastError());
+    std::vector<double> part(trip.size());
+    HIPCHECK(c, hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipFree(d_trip));
+    HIPCHECK(c, hipFree(d_data));
+    HIPCHECK(c, hipFree(d_part));
+    size_t p = 0;
+    for (int b = 0; b < (int)batch.size(); b++) {
+        const Shot &s = c->shots[batch[b]];
+        for (int cc = 0; cc < ncm; cc++) {
+            double ss = 0.0;
+            for (int r = 0; r < s.nrec; r++) ss += part[p++];
+            norms_out[(size_t)batch[b] * ncm + cc] = std::sqrt(ss);
+        }
+    }
+    return 0;
+}
+
+int run_adjoint_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
+{
+    int nitems = 0;
+    if (prepare_batch(c, batch, &nitems)) return 2;
+    if (clear_state(c, batch)) return 2;
+    // shots of the batch grouped by grid for the interaction
+    std::vector<std::vector<int>> per_grid(c->grids.size());
+    for (int b = 0; b < (int)batch.size(); b++) per_grid[c->shots[batch[b]].grid].push_back(b);
+    std::vector<int> flat, off(c->grids.size() + 1, 0);
+    for (size_t g = 0; g < per_grid.size(); g++) {
+        off[g] = (int)flat.size();
+        flat.insert(flat.end(), per_grid[g].begin(), per_grid[g].end());
+    }
+    off[per_grid.size()] = (int)flat.size();
+    if (ensure(c, c->d_list, c->d_list_cap, std::max<size_t>(flat.size(), 1))) return 2;
+    HIPCHECK(c, hipMemcpyAsync(c->d_list, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const double ndt = (double)sae * c->dt;
+    for (int it = 0; it < c->nt; it++) {
+        if (launch_step(c, nitems, it, -1, 1)) return 2;
+        if ((it + sae) % sae == 0) {
+            const int isae = (it + sae) / sae - 1;
+            for (size_t g = 0; g < per_grid.size(); g++) {
+                if (per_grid[g].empty()) continue;
+                const Grid &G = c->grids[g];
+                InteractArgs a;
+                a.shots = c->d_desc;
+                a.list = c->d_list + off[g];
+                a.nlist = (int)per_grid[g].size();
+                a.nx = G.nx;
+                a.nz = G.nz;
+                a.Npad = G.Npad;
+                a.kern = G.kern;
+                a.cur = (it & 1) ^ 1;
+                a.isae = isae;
+                a.ndt = ndt;
+                a.c1x = 9.0 / (8.0 * (double)G.dx);
+                a.c2x = 1.0 / (24.0 * (double)G.dx);
+                a.c1z = 9.0 / (8.0 * (double)G.dz);
+                a.c2z = 1.0 / (24.0 * (double)G.dz);
+                dim3 gr((unsigned)((G.N + 255) / 256)), bl(256);
+                switch (c->physics) {
+                case SP_SH: hipLaunchKernelGGL(k_interact<SP_SH>, gr, bl, 0, c->stream, a); break;
+                case SP_PSV: hipLaunchKernelGGL(k_interact<SP_PSV>, gr, bl, 0, c->stream, a); break;
+                default: hipLaunchKernelGGL(k_interact<SP_SPIN>, gr, bl, 0, c->stream, a); break;
+                }
+                HIPCHECK(c, hipGetLastError());
+            }
+        }
+    }
+    c->t_bytes += step_alg_bytes(c, batch) * c->nt;
+    return 0;
+}
+
+// split all shots into batches whose snapshots fit the pool
+int plan_batches(sp_ctx *c, int sae, std::vector<std::vector<int>> &out)
+{
+    out.clear();
+    const int ns = (int)c->shots.size();
+    if (sae <= 0) {
+        std::vector<int> all(ns);
+        std::iota(all.begin(), all.end(), 0);
+        out.push_back(all);
+        return 0;
+    }
+    size_t budget;
+    if (c->snap_mb > 0) {
+        budget = (size_t)c->snap_mb << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHECK(c, hipMemGetInfo(&fr, &tot));
+        budget = (size_t)((double)(fr + c->snap_pool_bytes) * 0.6);
+    }
+    std::vector<int> cur;
+    size_t used = 0;
+    for (int s = 0; s < ns; s++) {
+        const size_t b = snap_bytes(c, c->shots[s], sae);
+        if (b > budget) return fail(c, "snapshots of one shot exceed the snapshot memory budget");
+        if (used + b > budget) {
+            out.push_back(cur);
+            cur.clear();
+            used = 0;
+        }
+        cur.push_back(s);
+        used += b;
+    }
+    if (!cur.empty()) out.push_back(cur);
+    return 0;
+}
+
+int assign_snapshots(sp_ctx *c, const std::vector<int> &batch, int sae)
+{
+    size_t need = 0;
+    for (int s : batch) need += snap_bytes(c, c->shots[s], sae);
+    if (need > c->snap_pool_bytes) {
+        if (c->snap_pool) HIPCHECK(c, hipFree(c->snap_pool));
+        c->snap_pool = nullptr;
+        c->snap_pool_bytes = 0;
+        HIPCHECK(c, hipMalloc(&c->snap_pool, need));
+        c->snap_pool_bytes = need;
+    }
+    size_t off = 0;
+    for (auto &sh : c->shots) sh.snap = nullptr;
+    for (int s : batch) {
+        c->shots[s].snap = c->snap_pool + off / sizeof(float);
+        off += snap_bytes(c, c->shots[s], sae);
+    }
+    return 0;
+}
+
+int timing_begin(sp_ctx *c)
+{
+    c->t_launches = 0;
+    c->t_bytes = 0;
+    if (!c->ev0) {
+        HIPCHECK(c, hipEventCreate(&c->ev0));
+        HIPCHECK(c, hipEventCreate(&c->ev1));
+        HIPCHECK(c, hipEventCreate(&c->ev2));
+    }
+    HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    return 0;
+}
+
+int timing_end(sp_ctx *c)
+{
+    HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->t_total_ms = ms;
+    c->t_step_ms = ms;
+    return 0;
+}
+
+// separable sponge factors of set_bound (fd2d.py:47-68)
+void sponge_factors(int nx, int nz, int width, double alpha, int left, int right, int bottom, int top,
+                    std::vector<float> &bx, std::vector<double> &bz)
+{
+    bx.assign(nx, 1.0f);
+    bz.assign(2 * (size_t)nz, 1.0);
+    for (int i = 0; i < nx; i++) {
+        float b = 1.0f;
+        if (left && i + 1 < width) {
+            const double aw = alpha * (double)(width - i - 1);
+            b = (float)((double)b * std::exp(-aw * aw));
+        }
+        if (right && i > nx - width) {
+            const double aw = alpha * (double)(width + i - nx);
+            b = (float)((double)b * std::exp(-aw * aw));
+        }
+        bx[i] = b;
+    }
+    for (int j = 0; j < nz; j++) {
+        if (bottom && j > nz - width) {
+            const double aw = alpha * (double)(width + j - nz);
+            bz[j] = std::exp(-aw * aw);
+        }
+        if (top && j + 1 < width) {
+            const double aw = alpha * (double)(width - j - 1);
+            bz[nz + j] = std::exp(-aw * aw);
+        }
+    }
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int sp_create(int device, sp_ctx **out)
+{
+    if (!out) return 3;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 4;
+    if (device < 0 || device >= n) return 5;
+    if (hipSetDevice(device) != hipSuccess) return 6;
+    sp_ctx *c = new sp_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return 7;
+    }
+    c->stream = c->own;
+    *out = c;
+    return 0;
+}
+
+static void free_shot(Shot &s)
+{
+    if (s.d_idx) (void)hipFree(s.d_idx);
+    if (s.stf) (void)hipFree(s.stf);
+    if (s.state) (void)hipFree(s.state);
+    if (s.rec) (void)hipFree(s.rec);
+    if (s.data) (void)hipFree(s.data);
+    if (s.adstf) (void)hipFree(s.adstf);
+    s = Shot();
+}
+
+int sp_destroy(sp_ctx *c)
+{
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &s : c->shots) free_shot(s);
+    for (auto &g : c->grids) {
+        if (g.mat) (void)hipFree(g.mat);
+        if (g.bx) (void)hipFree(g.bx);
+        if (g.bz) (void)hipFree(g.bz);
+        if (g.kern) (void)hipFree(g.kern);
+    }
+    if (c->snap_pool) (void)hipFree(c->snap_pool);
+    if (c->d_desc) (void)hipFree(c->d_desc);
+    if (c->d_items) (void)hipFree(c->d_items);
+    if (c->d_list) (void)hipFree(c->d_list);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+    return 0;
+}
+
+const char *sp_last_error(const sp_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int sp_set_stream(sp_ctx *c, void *stream)
+{
+    if (!c) return 3;
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return 0;
+}
+
+int sp_sync(sp_ctx *c)
+{
+    if (!c) return 3;
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sp_set_option(sp_ctx *c, int key, long long v)
+{
+    if (!c) return 3;
+    switch (key) {
+    case SP_OPT_CHUNK:
+        if (v < 0 || v > 4096) return fail(c, "chunk must be in [0, 4096]");
+        c->chunk = (int)v;
+        return 0;
+    case SP_OPT_ORDER: c->order = v ? 1 : 0; return 0;
+    case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;
+    case SP_OPT_SNAP_MB:
+        if (v < 0) return fail(c, "snapshot budget must be >= 0");
+        c->snap_mb = v;
+        return 0;
+    default: return fail(c, "unknown option " + std::to_string(key));
+    }
+}
+
+int sp_set_physics(sp_ctx *c, int physics, int nt, double dt)
+{
+    if (!c) return 3;
+    if (physics != SP_SH && physics != SP_PSV && physics != SP_SPIN) return fail(c, "physics must be SP_SH, SP_PSV or SP_SPIN");
+    if (nt <= 0) return fail(c, "nt must be positive");
+    if (!(dt > 0)) return fail(c, "dt must be positive");
+    if (!c->shots.empty() && (physics != c->physics || nt != c->nt))
+        return fail(c, "clear the shots before changing physics or nt");
+    c->physics = physics;
+    c->nt = nt;
+    c->dt = dt;
+    return 0;
+}
+
+int sp_add_grid(sp_ctx *c, int nx, int nz, float dx, float dz, int param, const float *const *fields, int *grid_out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (nx < 1 || nz < 1) return fail(c, "grid must have nx >= 1 and nz >= 1");
+    if (!(dx > 0) || !(dz > 0)) return fail(c, "dx and dz must be positive");
+    if (!fields) return fail(c, "fields is null");
+    Grid g;
+    g.nx = nx;
+    g.nz = nz;
+    g.dx = dx;
+    g.dz = dz;
+    g.N = (long long)nx * nz;
+    g.Npad = pad64(g.N);
+    const size_t nb = (size_t)g.N * sizeof(float);
+    HIPCHECK(c, hipMalloc(&g.mat, (size_t)N_MAT * g.Npad * sizeof(float)));
+    HIPCHECK(c, hipMalloc(&g.kern, (size_t)SP_NKERN * g.Npad * sizeof(float)));
+    HIPCHECK(c, hipMalloc(&g.bx, (size_t)nx * sizeof(float)));
+    HIPCHECK(c, hipMalloc(&g.bz, 2 * (size_t)nz * sizeof(double)));
+    c->grids.push_back(g);
+    Grid &G = c->grids.back();
+    if (fill(c, G.mat, (long long)N_MAT * G.Npad, 0.f)) return 2;
+    if (fill(c, G.kern, (long long)SP_NKERN * G.Npad, 0.f)) return 2;
+    float *m = G.mat;
+    auto up = [&](int slot, const float *src) -> int {
+        if (!src) return fail(c, "missing model field");
+        HIPCHECK(c, hipMemcpyAsync(m + (size_t)slot * G.Npad, src, nb, hipMemcpyHostToDevice, c->stream));
+        return 0;
+    };
+    if (param == 0) {  // vp, vs, rho
+        if (up(M_LAM, fields[0]) || up(M_MU, fields[1]) || up(M_RHO, fields[2])) return 2;
+        hipLaunchKernelGGL(k_vps2lm, dim3((unsigned)((G.N + 255) / 256)), dim3(256), 0, c->stream,
+                           m + (size_t)M_LAM * G.Npad, m + (size_t)M_MU * G.Npad, m + (size_t)M_RHO * G.Npad, G.N);
+        HIPCHECK(c, hipGetLastError());
+    } else if (param == 1) {  // lambda, mu, nu, j, lambda_c, mu_c, nu_c, rho
+        const int slots[8] = {M_LAM, M_MU, M_NU, M_J, SP_F_LAMC, SP_F_MUC, SP_F_NUC, M_RHO};
+        for (int f = 0; f < 8; f++)
+            if (up(slots[f], fields[f])) return 2;
+        hipLaunchKernelGGL(k_gamma, dim3((unsigned)((G.N + 255) / 256)), dim3(256), 0, c->stream,
+                           m + (size_t)M_GAM * G.Npad, m + (size_t)SP_F_MUC * G.Npad, m + (size_t)SP_F_NUC * G.Npad, G.N);
+        HIPCHECK(c, hipGetLastError());
+    } else if (param == 2) {  // lambda, mu, rho
+        if (up(M_LAM, fields[0]) || up(M_MU, fields[1]) || up(M_RHO, fields[2])) return 2;
+    } else {
+        return fail(c, "param must be 0 (vp, vs, rho), 1 (spin list) or 2 (lambda, mu, rho)");
+    }
+    // no sponge until sp_set_sponge: all factors 1
+    std::vector<float> bx(nx, 1.0f);
+    std::vector<double> bz(2 * (size_t)nz, 1.0);
+    HIPCHECK(c, hipMemcpyAsync(G.bx, bx.data(), bx.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(G.bz, bz.data(), bz.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (grid_out) *grid_out = (int)c->grids.size() - 1;
+    return 0;
+}
+
+int sp_set_sponge(sp_ctx *c, int grid, int width, double alpha, int left, int right, int bottom, int top)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    Grid &G = c->grids[grid];
+    std::vector<float> bx;
+    std::vector<double> bz;
+    sponge_factors(G.nx, G.nz, width, alpha, left, right, bottom, top, bx, bz);
+    HIPCHECK(c, hipMemcpyAsync(G.bx, bx.data(), bx.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(G.bz, bz.data(), bz.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sp_set_field(sp_ctx *c, int grid, int field, const float *src, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    if (field < 0 || field > SP_F_NUC) return fail(c, "bad field id");
+    Grid &G = c->grids[grid];
+    float *m = G.mat;
+    if (dcopy(c, m + (size_t)field * G.Npad, src, (size_t)G.N * sizeof(float), 1, on_dev)) return 2;
+    if (field == SP_F_MUC || field == SP_F_NUC) {
+        hipLaunchKernelGGL(k_gamma, dim3((unsigned)((G.N + 255) / 256)), dim3(256), 0, c->stream,
+                           m + (size_t)M_GAM * G.Npad, m + (size_t)SP_F_MUC * G.Npad, m + (size_t)SP_F_NUC * G.Npad, G.N);
+        HIPCHECK(c, hipGetLastError());
+    }
+    c->snaps_valid = 0;
+    return 0;
+}
+
+int sp_get_field(sp_ctx *c, int grid, int field, float *dst, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    if (field < 0 || field > SP_F_NUC) return fail(c, "bad field id");
+    Grid &G = c->grids[grid];
+    return dcopy(c, dst, G.mat + (size_t)field * G.Npad, (size_t)G.N * sizeof(float), on_dev, 1);
+}
+
+int sp_add_shot(sp_ctx *c, int grid, int nsrc, const int *src_k, const float *stf, int nrec, const int *rec_k, int *shot_out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    if (nsrc < 0 || nrec < 0) return fail(c, "negative source or receiver count");
+    if ((nsrc && (!src_k || !stf)) || (nrec && !rec_k)) return fail(c, "null source/receiver arrays");
+    const Grid &G = c->grids[grid];
+    for (int s = 0; s < nsrc; s++)
+        if (src_k[s] < 0 || src_k[s] >= G.N) return fail(c, "source index outside the grid");
+    for (int r = 0; r < nrec; r++)
+        if (rec_k[r] < 0 || rec_k[r] >= G.N) return fail(c, "receiver index outside the grid");
+    Shot s;
+    s.grid = grid;
+    s.nsrc = nsrc;
+    s.nrec = nrec;
+    // index lists sorted by grid index, ties in input order
+    std::vector<int> so(nsrc), ro(nrec);
+    std::iota(so.begin(), so.end(), 0);
+    std::iota(ro.begin(), ro.end(), 0);
+    std::stable_sort(so.begin(), so.end(), [&](int a, int b) { return src_k[a] < src_k[b]; });
+    std::stable_sort(ro.begin(), ro.end(), [&](int a, int b) { return rec_k[a] < rec_k[b]; });
+    std::vector<int> idx(2 * (size_t)nsrc + 2 * (size_t)nrec + 1, 0);
+    for (int q = 0; q < nsrc; q++) {
+        idx[q] = src_k[so[q]];
+        idx[nsrc + q] = so[q];
+    }
+    for (int q = 0; q < nrec; q++) {
+        idx[2 * nsrc + q] = rec_k[ro[q]];
+        idx[2 * nsrc + nrec + q] = ro[q];
+    }
+    const int P = c->physics;
+    HIPCHECK(c, hipMalloc(&s.d_idx, idx.size() * sizeof(int)));
+    HIPCHECK(c, hipMemcpyAsync(s.d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const size_t nstf = (size_t)n_inj(P) * nsrc * c->nt;
+    HIPCHECK(c, hipMalloc(&s.stf, std::max<size_t>(nstf, 1) * sizeof(float)));
+    if (nstf) HIPCHECK(c, hipMemcpyAsync(s.stf, stf, nstf * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMalloc(&s.state, (size_t)n_slots(P) * G.Npad * sizeof(float)));
+    const long long nr = std::max(rec_floats(c, s), 1LL), na = std::max(adj_floats(c, s), 1LL);
+    HIPCHECK(c, hipMalloc(&s.rec, (size_t)nr * sizeof(float)));
+    HIPCHECK(c, hipMalloc(&s.adstf, (size_t)na * sizeof(float)));
+    c->shots.push_back(s);
+    Shot &S = c->shots.back();
+    if (fill(c, S.rec, nr, 0.f) || fill(c, S.adstf, na, 0.f)) return 2;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->snaps_valid = 0;
+    if (shot_out) *shot_out = (int)c->shots.size() - 1;
+    return 0;
+}
+
+int sp_clear_shots(sp_ctx *c)
+{
+    if (!c) return 3;
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (auto &s : c->shots) free_shot(s);
+    c->shots.clear();
+    c->snaps_valid = 0;
+    return 0;
+}
+
+int sp_num_components(const sp_ctx *c) { return c ? n_rec(c->physics) : 0; }
+
+int sp_forward(sp_ctx *c, int sae)
+{
+    if (int e = check_ctx(c)) return e;
+    if (sae < 0) return fail(c, "adjoint_interval must be >= 0");
+    if (sae > 0 && c->nt % sae) return fail(c, "nt must be a multiple of adjoint_interval");
+    if (c->shots.empty()) return fail(c, "no shots");
+    std::vector<std::vector<int>> batches;
+    if (plan_batches(c, sae, batches)) return 2;
+    if (batches.size() > 1) return fail(c, "snapshots of all shots exceed the snapshot memory budget; use sp_gradient");
+    c->snaps_valid = 0;
+    if (timing_begin(c)) return 2;
+    if (sae > 0 && assign_snapshots(c, batches[0], sae)) return 2;
+    if (run_forward_batch(c, batches[0], sae)) return 2;
+    if (timing_end(c)) return 2;
+    if (sae > 0) {
+        c->snaps_valid = 1;
+        c->snaps_sae = sae;
+    }
+    return 0;
+}
+
+int sp_get_records(sp_ctx *c, int shot, float *dst, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (shot < 0 || shot >= (int)c->shots.size()) return fail(c, "bad shot id");
+    const Shot &s = c->shots[shot];
+    return dcopy(c, dst, s.rec, (size_t)rec_floats(c, s) * sizeof(float), on_dev, 1);
+}
+
+int sp_set_data(sp_ctx *c, int shot, const float *src, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (shot < 0 || shot >= (int)c->shots.size()) return fail(c, "bad shot id");
+    Shot &s = c->shots[shot];
+    const size_t n = (size_t)std::max(rec_floats(c, s), 1LL);
+    if (!s.data) HIPCHECK(c, hipMalloc(&s.data, n * sizeof(float)));
+    return dcopy(c, s.data, src, (size_t)rec_floats(c, s) * sizeof(float), 1, on_dev);
+}
+
+int sp_misfit(sp_ctx *c, double *norms_out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!norms_out) return fail(c, "norms_out is null");
+    std::vector<int> all(c->shots.size());
+    std::iota(all.begin(), all.end(), 0);
+    return run_misfit_batch(c, all, norms_out);
+}
+
+int sp_adjoint(sp_ctx *c, int sae)
+{
+    if (int e = check_ctx(c)) return e;
+    if (sae <= 0 || c->nt % sae) return fail(c, "nt must be a positive multiple of adjoint_interval");
+    if (!c->snaps_valid || c->snaps_sae != sae) return fail(c, "no snapshots for this adjoint_interval: run sp_forward(ctx, sae) first");
+    for (auto &g : c->grids)
+        if (fill(c, g.kern, (long long)SP_NKERN * g.Npad, 0.f)) return 2;
+    std::vector<int> all(c->shots.size());
+    std::iota(all.begin(), all.end(), 0);
+    if (timing_begin(c)) return 2;
+    if (run_adjoint_batch(c, all, sae)) return 2;
+    return timing_end(c);
+}
+
+int sp_gradient(sp_ctx *c, int sae, double *norms_out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (sae <= 0 || c->nt % sae) return fail(c, "nt must be a positive multiple of adjoint_interval");
+    if (!norms_out) return fail(c, "norms_out is null");
+    if (c->shots.empty()) return fail(c, "no shots");
+    std::vector<std::vector<int>> batches;
+    if (plan_batches(c, sae, batches)) return 2;
+    for (auto &g : c->grids)
+        if (fill(c, g.kern, (long long)SP_NKERN * g.Npad, 0.f)) return 2;
+    c->snaps_valid = 0;
+    if (timing_begin(c)) return 2;
+    double bytes = 0;
+    long long launches = 0;
+    for (auto &b : batches) {
+        if (assign_snapshots(c, b, sae)) return 2;
+        if (run_forward_batch(c, b, sae)) return 2;
+        if (run_misfit_batch(c, b, norms_out)) return 2;
+        if (run_adjoint_batch(c, b, sae)) return 2;
+        bytes += c->t_bytes;
+        launches += c->t_launches;
+        c->t_bytes = 0;
+        c->t_launches = 0;
+    }
+    if (timing_end(c)) return 2;
+    c->t_bytes = bytes;
+    c->t_launches = launches;
+    if (batches.size() == 1) {
+        c->snaps_valid = 1;
+        c->snaps_sae = sae;
+    }
+    return 0;
+}
+
+int sp_get_kernel(sp_ctx *c, int grid, int which, float *dst, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    if (which < 0 || which >= SP_NKERN) return fail(c, "bad kernel id");
+    Grid &G = c->grids[grid];
+    return dcopy(c, dst, G.kern + (size_t)which * G.Npad, (size_t)G.N * sizeof(float), on_dev, 1);
+}
+
+int sp_kernels_vps(sp_ctx *c, int grid)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    Grid &G = c->grids[grid];
+    hipLaunchKernelGGL(k_kern_vps, dim3((unsigned)((G.N + 255) / 256)), dim3(256), 0, c->stream, G.kern, G.mat, G.N, G.Npad);
+    HIPCHECK(c, hipGetLastError());
+    return 0;
+}
+
+static int smooth_dev(sp_ctx *c, float *field, int nx, int nz, int sigma)
+{
+    if (sigma <= 0) return fail(c, "smooth sigma must be positive");
+    const int nm = std::max(nx, nz);
+    const double s = (double)sigma;
+    const double cc = 1.0 / (std::sqrt(2.0 * M_PI) * s);
+    std::vector<double> g(nm), sx(nx), sz(nz);
+    for (int d = 0; d < nm; d++) g[d] = cc * std::exp(-(double)((long long)d * d) / (2.0 * s * s));
+    for (int i = 0; i < nx; i++) {
+        double a = 0.0;
+        for (int n = 0; n < nx; n++) a += g[std::abs(i - n)];
+        sx[i] = a;
+    }
+    for (int j = 0; j < nz; j++) {
+        double a = 0.0;
+        for (int n = 0; n < nz; n++) a += g[std::abs(j - n)];
+        sz[j] = a;
+    }
+    double *dg = nullptr;
+    float *tmp = nullptr;
+    const long long N = (long long)nx * nz;
+    HIPCHECK(c, hipMalloc(&dg, (size_t)(nm + nx + nz) * sizeof(double)));
+    HIPCHECK(c, hipMalloc(&tmp, (size_t)N * sizeof(float)));
+    HIPCHECK(c, hipMemcpyAsync(dg, g.data(), nm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dg + nm, sx.data(), nx * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dg + nm + nx, sz.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const dim3 gr((unsigned)((N + 255) / 256)), bl(256);
+    hipLaunchKernelGGL(k_smooth_x, gr, bl, 0, c->stream, field, tmp, dg, nx, nz);
+    hipLaunchKernelGGL(k_smooth_z, gr, bl, 0, c->stream, field, tmp, dg, dg + nm, dg + nm + nx, nx, nz);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipFree(dg));
+    HIPCHECK(c, hipFree(tmp));
+    return 0;
+}
+
+int sp_smooth_kernel(sp_ctx *c, int grid, int which, int sigma)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    if (which < 0 || which >= SP_NKERN) return fail(c, "bad kernel id");
+    Grid &G = c->grids[grid];
+    return smooth_dev(c, G.kern + (size_t)which * G.Npad, G.nx, G.nz, sigma);
+}
+
+int sp_smooth(sp_ctx *c, float *field, int nx, int nz, int sigma, int on_dev)
+{
+    if (!c) return 3;
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (nx < 1 || nz < 1 || !field) return fail(c, "bad field");
+    if (on_dev) return smooth_dev(c, field, nx, nz, sigma);
+    float *d = nullptr;
+    const size_t nb = (size_t)nx * nz * sizeof(float);
+    HIPCHECK(c, hipMalloc(&d, nb));
+    if (dcopy(c, d, field, nb, 1, 0) || smooth_dev(c, d, nx, nz, sigma) || dcopy(c, field, d, nb, 0, 1)) {
+        (void)hipFree(d);
+        return 2;
+    }
+    HIPCHECK(c, hipFree(d));
+    return 0;
+}
+
+int sp_last_timing(const sp_ctx *c, long long *launches, double *step_ms, double *total_ms, double *bytes)
+{
+    if (!c) return 3;
+    if (launches) *launches = c->t_launches;
+    if (step_ms) *step_ms = c->t_step_ms;
+    if (total_ms) *total_ms = c->t_total_ms;
+    if (bytes) *bytes = c->t_bytes;
+    return 0;
+}
+
+}  // extern "C"

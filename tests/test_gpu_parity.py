@@ -1,0 +1,223 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances: seismograms relative L2 <= 1e-5, kernels <= 1e-4 (the GPU
+applies 1/(8dx), 1/(24dx), 1/rho as multiplications and accumulates the
+kernels of a batch time-major; both move results by float32 rounding
+steps only)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from seispie_amd import _lib, configs
+from tests.helpers import MIS_NAMES, oracle_gradient, oracle_records, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+KERN_TOL = 1e-4
+
+
+def _small(physics, nx=70, nz=61, nt=300, nshots=2, nrec=9, seed=0, sponge=(12, 0.015, 1, 1, 1, 0)):
+    rng = np.random.default_rng(seed)
+    n = nx * nz
+    dx = 5.0
+    if physics == _lib.SPIN:
+        prob = configs.spin_config4(scale=0.1, nt=nt, nshots=nshots)
+        g = prob.grids[0]
+        return prob
+    vp = (3000 * (1 + 0.05 * rng.standard_normal(n))).astype(np.float32)
+    vs = (1730 * (1 + 0.05 * rng.standard_normal(n))).astype(np.float32)
+    rho = (2600 * (1 + 0.05 * rng.standard_normal(n))).astype(np.float32)
+    comps = configs.ricker_stf(nt, 5e-4, 30.0, ang=30.0)
+    stf = configs.shot_stf(physics, comps)
+    rec_k = np.array([configs.gindex(x, 0.0, np.float32(dx), np.float32(dx), nz)
+                      for x in np.linspace(0, (nx - 1) * dx, nrec)], np.int32)
+    shots = [configs.Shot(np.array([configs.gindex((s + 1) * nx * dx / (nshots + 1), 25.0, np.float32(dx),
+                                                   np.float32(dx), nz)], np.int32), stf * (1 + 0.1 * s), rec_k)
+             for s in range(nshots)]
+    g = configs.Grid(nx, nz, dx, dx, [vp, vs, rho], 0, sponge=sponge, shots=shots)
+    return configs.Problem(physics, nt, 5e-4, [g], sae=10)
+
+
+def _run_forward(ctx_factory, prob, **opts):
+    ctx = ctx_factory()
+    for k, v in opts.items():
+        ctx.set_option(getattr(_lib, "OPT_" + k.upper()), v)
+    ids = configs.load_problem(ctx, prob)
+    ctx.forward(0)
+    return ctx, [ctx.records(i) for i in ids]
+
+
+def _check_forward(prob, recs):
+    s = 0
+    worst = 0.0
+    for g in prob.grids:
+        for shot in g.shots:
+            ref, _, _ = oracle_records(prob, g, shot)
+            err = rel_l2(recs[s], ref)
+            worst = max(worst, err)
+            assert err <= SEIS_TOL, (s, err)
+            s += 1
+    print("worst seismogram rel L2 %.3e" % worst)
+    return worst
+
+
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+def test_forward_matches_oracle(ctx_factory, physics):
+    prob = _small(physics)
+    print("case: forward", prob.physics, prob.grids[0].nx, prob.grids[0].nz, prob.nt, flush=True)
+    _, recs = _run_forward(ctx_factory, prob)
+    _check_forward(prob, recs)
+
+
+def test_forward_sh_config1_full_size(ctx_factory):
+    """SH 201^2, dt 1 ms, 2000 steps (the SH benchmark configuration)."""
+    prob = configs.sh_config1()
+    print("case: sh config1", flush=True)
+    _, recs = _run_forward(ctx_factory, prob)
+    _check_forward(prob, recs)
+
+
+@pytest.mark.parametrize("chunk,order", [(16, 0), (64, 1), (5, 1)])
+def test_ragged_batch_matches_single_shot_oracle(ctx_factory, chunk, order):
+    """Shots on three grids of different shapes in one launch per step;
+    chunk/order only change the work decomposition."""
+    probs = [_small(_lib.PSV, 40, 130, 200, 2, seed=1), _small(_lib.PSV, 97, 33, 200, 1, seed=2),
+             _small(_lib.PSV, 64, 64, 200, 3, seed=3)]
+    prob = configs.Problem(_lib.PSV, 200, 5e-4, [p.grids[0] for p in probs], sae=10)
+    print("case: ragged", chunk, order, flush=True)
+    _, recs = _run_forward(ctx_factory, prob, chunk=chunk, order=order)
+    _check_forward(prob, recs)
+
+
+def test_edge_and_coincident_sources_and_receivers(ctx_factory):
+    """Sources and receivers on the outermost rows/columns and on the same
+    cells; receivers on a strip boundary (j = 58) and a shot without
+    sources."""
+    nx, nz, nt, dx = 67, 121, 160, 5.0
+    n = nx * nz
+    prob = _small(_lib.PSV, nx, nz, nt, 1)
+    g = prob.grids[0]
+    comps = configs.ricker_stf(nt, 5e-4, 30.0, ang=45.0)
+    stf = configs.shot_stf(_lib.PSV, comps)
+    src_k = np.array([0, nz - 1, (nx - 1) * nz, n - 1, 30 * nz + 58, 30 * nz + 58, 10 * nz + 57], np.int32)
+    stf_many = np.concatenate([stf * (1 + 0.1 * q) for q in range(src_k.size)], axis=1)
+    rec_k = np.array([0, 1, nz - 1, n - 1, 30 * nz + 58, 30 * nz + 58, 30 * nz + 59, 2 * nz + 57, 66 * nz + 3], np.int32)
+    g.shots = [configs.Shot(src_k, stf_many, rec_k), configs.Shot(np.zeros(0, np.int32), stf[:, :0], rec_k)]
+    print("case: edges", flush=True)
+    _, recs = _run_forward(ctx_factory, prob)
+    assert np.all(recs[1] == 0)
+    ref, _, _ = oracle_records(prob, g, g.shots[0])
+    assert rel_l2(recs[0], ref) <= SEIS_TOL
+    np.testing.assert_array_equal(recs[0][:, 4], recs[0][:, 5])
+
+
+def test_tiny_grids(ctx_factory):
+    """Grids smaller than the stencil bands and than one strip."""
+    for nx, nz in ((5, 4), (3, 70), (9, 9)):
+        prob = _small(_lib.SH, nx, nz, 50, 1, nrec=2, sponge=(3, 0.1, 1, 1, 1, 1))
+        print("case: tiny", nx, nz, flush=True)
+        _, recs = _run_forward(ctx_factory, prob)
+        _check_forward(prob, recs)
+
+
+def _data_for(prob, scale=1.02):
+    """Observed data: oracle records of a perturbed model (mu and rho)."""
+    out = []
+    for g in prob.grids:
+        gt = configs.Grid(g.nx, g.nz, g.dx, g.dz, [f.copy() for f in g.fields], g.param, g.sponge, g.shots)
+        if g.param == 0:
+            gt.fields[1] = (gt.fields[1] * scale).astype(np.float32)
+        else:
+            gt.fields[1] = (gt.fields[1] * scale).astype(np.float32)
+            gt.fields[3] = (gt.fields[3] / scale).astype(np.float32)
+        for shot in g.shots:
+            out.append(oracle_records(prob, gt, shot)[0])
+    return out
+
+
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+def test_misfit_and_gradient_match_oracle(ctx_factory, physics):
+    prob = _small(physics, nt=200)
+    prob.sae = 10
+    data = _data_for(prob)
+    print("case: gradient", physics, flush=True)
+    ctx = ctx_factory()
+    ids = configs.load_problem(ctx, prob)
+    for i, d in zip(ids, data):
+        ctx.set_data(i, d)
+    norms = ctx.gradient(prob.sae)
+    kref, nref = oracle_gradient(prob, data)
+    np.testing.assert_allclose(norms, nref, rtol=1e-6)
+    names = {_lib.SH: [("mu", _lib.K_MU)],
+             _lib.PSV: [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO)],
+             _lib.SPIN: [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO), ("nu", _lib.K_NU),
+                         ("j", _lib.K_J), ("muc", _lib.K_MUC)]}[physics]
+    for nm, w in names:
+        err = rel_l2(ctx.kernel(0, w), kref[0][nm])
+        print(nm, "kernel rel L2 %.3e" % err)
+        assert err <= KERN_TOL, (nm, err)
+    if physics != _lib.SH:
+        # vp/vs/rho kernels: chain rule of vps2lm applied to the oracle kernels
+        ctx.kernels_vps(0)
+        lam, mu, rho = (ctx.get_field(0, f).astype(np.float64) for f in (_lib.F_LAM, _lib.F_MU, _lib.F_RHO))
+        vp, vs = np.sqrt((lam + 2 * mu) / rho), np.sqrt(mu / rho)
+        kl, km, kr = (kref[0][k].astype(np.float64) for k in ("lam", "mu", "rho"))
+        assert rel_l2(ctx.kernel(0, _lib.K_VP), 2 * rho * vp * kl) <= KERN_TOL
+        assert rel_l2(ctx.kernel(0, _lib.K_VS), 2 * rho * vs * (km - 2 * kl)) <= KERN_TOL
+        assert rel_l2(ctx.kernel(0, _lib.K_RHOP), (vp ** 2 - 2 * vs ** 2) * kl + vs ** 2 * km + kr) <= KERN_TOL
+
+
+def test_forward_then_adjoint_equals_gradient_and_sub_batches(ctx_factory):
+    """sp_forward(sae) + sp_misfit + sp_adjoint == sp_gradient, and a
+    snapshot budget that forces one shot per sub-batch gives the same."""
+    prob = _small(_lib.PSV, nt=200, nshots=3)
+    data = _data_for(prob)
+    print("case: sub-batches", flush=True)
+    ks = []
+    for mode in ("split", "one", "tiny"):
+        ctx = ctx_factory()
+        if mode == "tiny":
+            one = (prob.nt // prob.sae) * 4 * ((prob.grids[0].nx * prob.grids[0].nz + 63) // 64 * 64) * 4
+            ctx.set_option(_lib.OPT_SNAP_MB, max(1, (one + (1 << 20) - 1) >> 20))
+        ids = configs.load_problem(ctx, prob)
+        for i, d in zip(ids, data):
+            ctx.set_data(i, d)
+        if mode == "split":
+            ctx.forward(prob.sae)
+            ctx.misfit()
+            ctx.adjoint(prob.sae)
+        else:
+            ctx.gradient(prob.sae)
+        ks.append([ctx.kernel(0, w) for w in (_lib.K_LAM, _lib.K_MU, _lib.K_RHO)])
+    for a, b in zip(ks[0], ks[1]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(ks[1], ks[2]):
+        assert rel_l2(a, b) <= 1e-6
+
+
+def test_smoothing_matches_oracle(ctx_factory):
+    ctx = ctx_factory()
+    ctx.set_physics(_lib.SH, 10, 1e-3)
+    rng = np.random.default_rng(5)
+    for nx, nz, sigma in ((37, 29, 3), (120, 80, 5)):
+        a = rng.standard_normal(nx * nz).astype(np.float32)
+        out = ctx.smooth(a, nx, nz, sigma)
+        ref = O.smooth(a, nx, nz, sigma)
+        assert rel_l2(out, ref) <= 1e-6
+
+
+def test_errors_are_reported_not_raised_through_the_abi(ctx_factory):
+    ctx = ctx_factory()
+    ctx.set_physics(_lib.PSV, 100, 5e-4)
+    with pytest.raises(_lib.SeispieError, match="no shots"):
+        ctx.forward(0)
+    g = ctx.add_grid(10, 10, 5.0, 5.0, [np.full(100, 3000, np.float32), np.full(100, 1700, np.float32),
+                                        np.full(100, 2600, np.float32)], 0)
+    with pytest.raises(_lib.SeispieError, match="outside the grid"):
+        ctx.add_shot(g, [100], np.zeros((2, 1, 100), np.float32), [0])
+    ctx.add_shot(g, [5], np.zeros((2, 1, 100), np.float32), [0])
+    with pytest.raises(_lib.SeispieError, match="multiple"):
+        ctx.gradient(7)
+    with pytest.raises(_lib.SeispieError, match="sp_set_data"):
+        ctx.gradient(10)
