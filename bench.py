@@ -98,7 +98,7 @@ def main():
     ap.add_argument("--config", default="3a")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--order", type=int, default=1)
+    ap.add_argument("--order", type=int, default=0, help="work order when --chunk is given")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -120,6 +120,9 @@ def main():
         ctx.set_data(i, d)
     del data
     sae = init.sae
+    if not args.chunk:
+        tuned = ctx.autotune(20)
+        print("# autotuned chunk=%d order=%d" % tuned, file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         ctx.gradient(sae)

@@ -58,9 +58,11 @@ typedef struct sp_ctx sp_ctx;
 
 /* tuning options (sp_set_option) */
 #define SP_OPT_CHUNK 1      /* rows per wavefront item in the step kernel */
-#define SP_OPT_ORDER 2      /* 0: shot-outer item order, 1: shot-inner */
+#define SP_OPT_ORDER 2      /* work order: 0 shot-outer (default), 1 shot-inner, 2 tile-major, 3 strip-group-major */
 #define SP_OPT_GRAPH 3      /* 1: replay time steps from a captured hipGraph */
 #define SP_OPT_SNAP_MB 4    /* snapshot memory budget in MiB (0 = auto) */
+#define SP_OPT_KERNEL 5     /* step kernel: 1 buffer-addressed unrolled march (default), 0 plain */
+#define SP_OPT_STRIP 6      /* strip geometry (shift << 16) | width: output lanes [shift, shift + width) */
 
 /* Context and device.  Replaces fd2d.setup (fd2d.py:111-120). */
 int sp_create(int device, sp_ctx **out);
@@ -124,6 +126,13 @@ int sp_kernels_vps(sp_ctx *ctx, int grid);
 int sp_smooth_kernel(sp_ctx *ctx, int grid, int which, int sigma);
 /* the same smoothing on a caller buffer of nx*nz floats */
 int sp_smooth(sp_ctx *ctx, float *field, int nx, int nz, int sigma, int on_device);
+
+/* Autotuner: times `steps` forward steps of the current shots for each
+ * rows-per-wavefront chunk and work order, keeps the fastest for later runs
+ * and reports it.  Overwrites wavefields and records (every run clears
+ * them first).  No reference counterpart (the reference launches one
+ * thread per cell with threads_per_block from the config, fd2d.py:185-187). */
+int sp_autotune(sp_ctx *ctx, int steps, int *chunk_out, int *order_out);
 
 /* Timing of the last sp_forward / sp_adjoint / sp_gradient on the
  * context's stream: step-kernel launches and their total milliseconds

@@ -11,14 +11,14 @@ import numpy as np
 SH, PSV, SPIN = 1, 2, 3
 F_RHO, F_LAM, F_MU, F_NU, F_J, F_LAMC, F_MUC, F_NUC = range(8)
 K_LAM, K_MU, K_RHO, K_NU, K_J, K_MUC, K_VP, K_VS, K_RHOP = range(9)
-OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB = 1, 2, 3, 4
+OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP = 1, 2, 3, 4, 5, 6
 
 N_REC = {SH: 1, PSV: 2, SPIN: 3}   # record components: SH y; P-SV x, z; spin x, z, y
 N_INJ = {SH: 1, PSV: 2, SPIN: 2}   # source components: SH y; P-SV/spin x, z
 N_MIS = {SH: 1, PSV: 2, SPIN: 2}   # misfit components: SH y; P-SV/spin x, z
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libseispie_hip.so")
+LIB_PATH = os.environ.get("SEISPIE_HIP_LIB") or os.path.join(_HERE, "lib", "libseispie_hip.so")
 _lib = None
 
 
@@ -62,6 +62,7 @@ def load():
         "sp_smooth_kernel": ([vp, i, i, i], i),
         "sp_smooth": ([vp, vp, i, i, i, i], i),
         "sp_last_timing": ([vp, ctypes.POINTER(ll), ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d)], i),
+        "sp_autotune": ([vp, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -77,7 +78,7 @@ def exported_symbols():
             "sp_set_physics", "sp_add_grid", "sp_set_sponge", "sp_set_field", "sp_get_field",
             "sp_add_shot", "sp_clear_shots", "sp_num_components", "sp_forward", "sp_get_records",
             "sp_set_data", "sp_misfit", "sp_adjoint", "sp_gradient", "sp_get_kernel", "sp_kernels_vps",
-            "sp_smooth_kernel", "sp_smooth", "sp_last_timing"]
+            "sp_smooth_kernel", "sp_smooth", "sp_last_timing", "sp_autotune"]
 
 
 def _f32(a, n=None):
@@ -251,6 +252,13 @@ class Context:
         a = _f32(field, nx * nz).copy()
         self._chk(self.L.sp_smooth(self._p, _ptr(a), int(nx), int(nz), int(sigma), 0))
         return a
+
+    def autotune(self, steps=20):
+        """Pick the fastest rows-per-wavefront chunk and work order for the
+        current shots; returns (chunk, order)."""
+        ch, od = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.L.sp_autotune(self._p, int(steps), ctypes.byref(ch), ctypes.byref(od)))
+        return ch.value, od.value
 
     def timing(self):
         n = ctypes.c_longlong()

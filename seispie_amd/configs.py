@@ -5,8 +5,11 @@ model on 500^2), built with the reference's own conventions:
 
   * grid index k = i*nz + j, dx/dz float32 scalars (fd2d.py:189-196);
   * Ricker sources (source/ricker.py:7-11) with t0 = 1.2/f0, amp = 1e10,
-    sources at z = 10 m, receivers at z = 0 (grid index rounding of
-    fd2d.py:132-134, 157-159);
+    sources at z = 10 m, receivers at z = 2 dz (grid index rounding of
+    fd2d.py:132-134, 157-159).  Receivers sit two cells below the surface
+    because the reference's difference operators are zero in their edge
+    bands (solver.py:22-51): rows j = 0 (all physics) and j = 1 (SH) are
+    decoupled from the interior, so a receiver there records exactly zero;
   * sponge width 20, alpha 0.015 on the left, right and bottom sides, top
     free (examples/forward/config.ini:20-25).
 
@@ -98,12 +101,15 @@ def checkerboard(nx, nz, cell, amp):
     return (amp * (-1.0) ** (i + j)).reshape(-1)
 
 
+REC_DEPTH_CELLS = 2  # shallowest row the stencils couple to the interior
+
+
 def _geometry(physics, nx, nz, dx, dz, nt, dt, nshots, nrec, f0, ang=0.0):
-    """Evenly spaced sources at z = 10 m (one per shot), receivers at z = 0."""
+    """Evenly spaced sources at z = 10 m (one per shot), receivers at z = 2 dz."""
     dx32, dz32 = np.float32(dx), np.float32(dz)
     lx = (nx - 1) * dx
     rec_x = np.linspace(0.0, lx, nrec)
-    rec_k = np.array([gindex(x, 0.0, dx32, dz32, nz) for x in rec_x], dtype=np.int32)
+    rec_k = np.array([gindex(x, REC_DEPTH_CELLS * dz, dx32, dz32, nz) for x in rec_x], dtype=np.int32)
     comps = ricker_stf(nt, dt, f0, ang)
     stf = shot_stf(physics, comps)
     shots = []
@@ -130,7 +136,8 @@ def sh_config1(scale=1.0, nt=None):
     rho = np.full(n * n, 2700.0, np.float32)
     dx32 = np.float32(dx)
     nrec = 101 if n >= 101 else n
-    rec_k = np.array([gindex(xx, 0.0, dx32, dx32, n) for xx in np.linspace(0, (n - 1) * dx, nrec)], np.int32)
+    rec_k = np.array([gindex(xx, REC_DEPTH_CELLS * dx, dx32, dx32, n)
+                      for xx in np.linspace(0, (n - 1) * dx, nrec)], np.int32)
     stf = shot_stf(_lib.SH, ricker_stf(nt, dt, 10.0))
     g = Grid(n, n, dx, dx, [vp, vs, rho], 0, shots=[Shot(np.array([gindex(c, 100.0, dx32, dx32, n)], np.int32), stf, rec_k)])
     return Problem(_lib.SH, nt, dt, [g], sae=50, name="sh_201x201_nt2000")

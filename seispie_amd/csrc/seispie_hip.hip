@@ -81,13 +81,16 @@ struct ShotDesc {
 
 struct StepArgs {
     const ShotDesc *shots;
-    const int4 *items;  // (shot, j0, i0, i1) per wavefront
+    const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-1) nz,
+                        // the first receiver with k >= (i0-1) nz and the first with k >= i0 nz
     int nitems;
     int it, nt;
     double dt;
     int rd;        // stress/velocity set read this step
     int snap_idx;  // -1: no snapshot this step
     int adjoint;   // inject adjoint sources at the receivers, record nothing
+    int shift;     // lanes before the first output cell of a strip (>= 3)
+    int wout;      // output cells per strip (shift + wout <= 61)
 };
 
 // float32 operations the reference performs in float32 (float32 * float32
@@ -122,46 +125,137 @@ __device__ __forceinline__ double dplus(float m1, float c0, float p1, float p2, 
 
 __device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, 64); }
 
-__device__ __forceinline__ int lower_bound_k(const int *a, int n, int key)
+// ------------------------------------------------------------ step kernel
+// Global-address-space pointers: loads through them become global_load with
+// a scalar base and one 32-bit offset per row shared by every field.
+typedef const float __attribute__((address_space(1))) *gcf;
+typedef float __attribute__((address_space(1))) *gf;
+
+// 1/x in float64 to within an ulp: hardware estimate + two Newton steps
+__device__ __forceinline__ double rcp64(double x)
 {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
-// ------------------------------------------------------------ step kernel
+// Everything iteration m of the row march reads from memory, fetched one
+// iteration ahead so that a wavefront's loads overlap its own arithmetic.
+// Row and lane indices are clamped into the grid so every load is valid;
+// values fetched for rows or lanes outside the grid are never used.
 template <int P>
-__global__ __launch_bounds__(256) void k_step(StepArgs a)
+struct RowLoads {
+    float s[n_s(P)];   // old stresses, row m
+    float v[n_v(P)];   // old velocities, row r = m-1
+    float u[n_v(P)];   // displacement, row r
+    float rho, jj, bx; // row r
+    float lam, mu, nu, gam;  // row q = m-3
+};
+
+struct StepPtrs {
+    gcf s_in[6];
+    gcf v_in[3];
+    gcf u_in[3];
+    gcf rho, jj, lam, mu, nu, gam, bx;
+};
+
+template <int P>
+__device__ __forceinline__ void load_row(RowLoads<P> &p, const StepPtrs &q, int m, int nx, int nz, int jc,
+                                         int i0, int i1)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ms = min(max(m, 0), nx - 1);
+    const int mr = min(max(m - 1, 0), nx - 1);
+    const int mq = min(max(m - 3, i0), i1 - 1);
+    const unsigned os = (unsigned)(ms * nz + jc), orr = (unsigned)(mr * nz + jc), oq = (unsigned)(mq * nz + jc);
+#pragma unroll
+    for (int f = 0; f < NS; f++) p.s[f] = q.s_in[f][os];
+#pragma unroll
+    for (int f = 0; f < NV; f++) {
+        p.v[f] = q.v_in[f][orr];
+        p.u[f] = q.u_in[f][orr];
+    }
+    p.rho = q.rho[orr];
+    p.bx = q.bx[mr];
+    p.jj = 1.f;
+    if constexpr (P == SP_SPIN) p.jj = q.jj[orr];
+    p.mu = q.mu[oq];
+    p.lam = p.nu = p.gam = 0.f;
+    if constexpr (P != SP_SH) p.lam = q.lam[oq];
+    if constexpr (P == SP_SPIN) {
+        p.nu = q.nu[oq];
+        p.gam = q.gam[oq];
+    }
+}
+
+#ifndef SP_STEP_WAVES
+#define SP_STEP_WAVES 1
+#endif
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAVES))) void k_step(StepArgs a)
+{
+    constexpr int NS = n_s(P), NV = n_v(P);
+    // Blocks are dispatched round-robin over the 8 XCDs; remap them so that
+    // each XCD (and its L2) walks a contiguous run of the work list.
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q8 = nb >> 3, r8 = nb & 7, bx8 = b & 7, by8 = b >> 3;
+    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
+    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
     if (wave >= a.nitems) return;  // whole wavefront leaves together
     const int L = threadIdx.x & 63;
-    const int4 item = a.items[wave];
+    const int4 item = a.items[2 * wave];
+    if (item.x < 0) return;        // padding entry of the work list
+    const int4 item2 = a.items[2 * wave + 1];
     const ShotDesc &d = a.shots[item.x];
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - 3 + L;
+    const int jc = min(max(j, 0), nz - 1);
     const bool jin = j >= 0 && j < nz;
     const bool outl = jin && L >= 3 && L < 61;
     const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
     const long long NP = d.Npad;
     const double dt = a.dt, dt2 = 2.0 * a.dt;
     const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
-    float *st = d.state;
     const int rd = a.rd, wr = a.rd ^ 1;
-    const float *mat = d.mat;
-    const double bzb = jin ? d.bzb[j] : 1.0, bzt = jin ? d.bzt[j] : 1.0;
+    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
+
+    gf st = (gf)d.state;
+    gcf mat = (gcf)d.mat;
+    StepPtrs q;
+#pragma unroll
+    for (int f = 0; f < NS; f++) q.s_in[f] = st + slot_s(P, rd, f) * NP;
+#pragma unroll
+    for (int f = 0; f < NV; f++) {
+        q.v_in[f] = st + slot_v(P, rd, f) * NP;
+        q.u_in[f] = st + slot_u(P, f) * NP;
+    }
+    q.rho = mat + M_RHO * NP;
+    q.jj = mat + M_J * NP;
+    q.lam = mat + M_LAM * NP;
+    q.mu = mat + M_MU * NP;
+    q.nu = mat + M_NU * NP;
+    q.gam = mat + M_GAM * NP;
+    q.bx = (gcf)d.bx;
+    gf s_out[NS];
+    gf v_out[NV];
+    gf u_out[NV];
+#pragma unroll
+    for (int f = 0; f < NS; f++) s_out[f] = st + slot_s(P, wr, f) * NP;
+#pragma unroll
+    for (int f = 0; f < NV; f++) {
+        v_out[f] = st + slot_v(P, wr, f) * NP;
+        u_out[f] = st + slot_u(P, f) * NP;
+    }
 
     // injection list: sources (forward) or receivers with adjoint sources
     const int *ik = a.adjoint ? d.rec_k : d.src_k;
     const int *irow = a.adjoint ? d.rec_row : d.src_row;
     const int ni = a.adjoint ? d.nrec : d.nsrc;
     const float *ival = a.adjoint ? d.adstf : d.stf;
-    int pi = lower_bound_k(ik, ni, (i0 - 1) * nz);
-    int pr = lower_bound_k(d.rec_k, d.nrec, i0 * nz);
+    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
+    // first list entries at or after the chunk's rows, found on the host
+    int pi = a.adjoint ? item2.y : item2.x;
+    int pr = item2.z;
     const bool record = !a.adjoint;
 
     float S0[NS], S1[NS], S2[NS], S3[NS];
@@ -171,26 +265,30 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 #pragma unroll
     for (int f = 0; f < NV; f++) W0[f] = W1[f] = W2[f] = W3[f] = 0.f;
 
+    // two rows in flight ahead of the one being computed
+    RowLoads<P> n1, n2;
+    load_row<P>(n1, q, i0 - 3, nx, nz, jc, i0, i1);
+    load_row<P>(n2, q, i0 - 2, nx, nz, jc, i0, i1);
     for (int m = i0 - 3; m <= i1 + 2; ++m) {
-        // ---- old stresses of row m into the window
-        const bool mrow = m >= 0 && m < nx;
+        const RowLoads<P> cur = n1;
+        n1 = n2;
+        if (m + 2 <= i1 + 2) load_row<P>(n2, q, m + 2, nx, nz, jc, i0, i1);
 #pragma unroll
         for (int f = 0; f < NS; f++) {
             S0[f] = S1[f];
             S1[f] = S2[f];
             S2[f] = S3[f];
-            S3[f] = (mrow && jin) ? st[slot_s(P, rd, f) * NP + (long long)m * nz + j] : 0.f;
+            S3[f] = cur.s[f];
         }
         if (m < i0) continue;
 
-        // ---- new velocities of row r = m-1 (window rows r-2..r+1)
+        // ---- new velocities of row r = m-1 (stress window rows r-2..r+1)
         const int r = m - 1;
         float vn[NV];
 #pragma unroll
         for (int f = 0; f < NV; f++) vn[f] = 0.f;
         if (r >= 0 && r < nx) {
             const bool xm_ok = r >= 2 && r < nx - 2;
-            const long long k = (long long)r * nz + j;
             float dsv[NV];
             if constexpr (P == SP_SH) {
                 // div_sy (solver.py:54-58)
@@ -227,62 +325,55 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             while (pi < ni && ik[pi] < kend) {
                 if (ik[pi] == kr) {
                     const int row = irow[pi];
-                    if constexpr (P == SP_SH) {
-                        dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + a.it]);
-                    } else {
-                        const long long nr = a.adjoint ? d.nrec : d.nsrc;
-                        dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + a.it]);
-                        dsv[1] = add_rn(dsv[1], ival[(nr + row) * a.nt + a.it]);
-                    }
+                    dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + a.it]);
+                    if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], ival[(nrows + row) * a.nt + a.it]);
                 }
                 ++pi;
             }
-            float un[NV];
+            // add_vy / add_vxz (solver.py:95-109) with the separable sponge of
+            // set_bound (fd2d.py:47-68): f32(f32(bx*bottom)*top)
+            const float b1 = (float)((double)cur.bx * bzb);
+            const double b = (double)(float)((double)b1 * bzt);
+            const double dtr = dt * rcp64((double)cur.rho);
 #pragma unroll
-            for (int f = 0; f < NV; f++) un[f] = 0.f;
-            if (jin) {
-                // add_vy / add_vxz (solver.py:95-109) with the separable sponge
-                // of set_bound (fd2d.py:47-68): f32(f32(bx*bottom)*top)
-                const float b1 = (float)((double)d.bx[r] * bzb);
-                const double b = (double)(float)((double)b1 * bzt);
-                const double dtr = dt / (double)mat[M_RHO * NP + k];
+            for (int f = 0; f < NV; f++) {
+                double dtm = dtr;
+                if constexpr (P == SP_SPIN) {
+                    if (f == 2) dtm = dt * rcp64((double)cur.jj);
+                }
+                vn[f] = jin ? (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm)) : 0.f;
+            }
+            if (outl && r >= i0 && r < i1) {
+                const unsigned k = (unsigned)(r * nz + j);
+                float un[NV];
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
-                    double dtm = dtr;
-                    if constexpr (P == SP_SPIN) {
-                        if (f == 2) dtm = dt / (double)mat[M_J * NP + k];
-                    }
-                    const float vo = st[slot_v(P, rd, f) * NP + k];
-                    vn[f] = (float)(b * ((double)vo + (double)dsv[f] * dtm));
+                    un[f] = (float)((double)cur.u[f] + (double)vn[f] * dt);
+                    u_out[f][k] = un[f];
+                    v_out[f][k] = vn[f];
                 }
-                if (outl && r >= i0 && r < i1) {
+                if (a.snap_idx >= 0) {
+                    gf sp = (gf)d.snap + (long long)a.snap_idx * n_snap(P) * NP;
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
-                        float *up = st + slot_u(P, f) * NP + k;
-                        const float uo = *up;
-                        un[f] = (float)((double)uo + (double)vn[f] * dt);
-                        *up = un[f];
-                        st[slot_v(P, wr, f) * NP + k] = vn[f];
-                        if (a.snap_idx >= 0) {
-                            float *sp = d.snap + (long long)a.snap_idx * n_snap(P) * NP + k;
-                            sp[f * NP] = uo;
-                            sp[(NV + f) * NP] = vn[f];
+                        sp[f * NP + k] = cur.u[f];
+                        sp[(NV + f) * NP + k] = vn[f];
+                    }
+                }
+                // save_obs (solver.py:151-156): displacement after the step
+                if (record) {
+                    for (int q2 = pr; q2 < d.nrec && d.rec_k[q2] < kend; ++q2) {
+                        if (d.rec_k[q2] == kr) {
+                            const int row = d.rec_row[q2];
+#pragma unroll
+                            for (int f = 0; f < NV; f++)
+                                d.rec[((long long)f * d.nrec + row) * a.nt + a.it] = un[f];
                         }
                     }
                 }
             }
-            // save_obs (solver.py:151-156): displacement after the step
-            if (record && r >= i0 && r < i1) {
-                while (pr < d.nrec && d.rec_k[pr] < kend) {
-                    if (outl && d.rec_k[pr] == kr) {
-                        const int row = d.rec_row[pr];
-#pragma unroll
-                        for (int f = 0; f < NV; f++)
-                            d.rec[((long long)f * d.nrec + row) * a.nt + a.it] = un[f];
-                    }
-                    ++pr;
-                }
-            }
+            if (record && r >= i0 && r < i1)
+                while (pr < d.nrec && d.rec_k[pr] < kend) ++pr;
         }
 #pragma unroll
         for (int f = 0; f < NV; f++) {
@@ -292,20 +383,20 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             W3[f] = vn[f];
         }
 
-        // ---- new stresses of row q = m-3 (velocity window rows q-1..q+2)
-        const int q = m - 3;
-        if (q < i0) continue;
-        const bool xp_ok = q >= 1 && q < nx - 2;
-        const long long kq = (long long)q * nz + j;
+        // ---- new stresses of row qr = m-3 (velocity window rows qr-1..qr+2)
+        const int qr = m - 3;
+        if (qr < i0) continue;
+        const bool xp_ok = qr >= 1 && qr < nx - 2;
+        const unsigned kq = (unsigned)(qr * nz + j);
         if constexpr (P == SP_SH) {
             // div_vy + add_sy (solver.py:112-117, 128-133)
             float zm1 = shfl(W1[0], L - 1), zp1 = shfl(W1[0], L + 1), zp2 = shfl(W1[0], L + 2);
             if (outl) {
                 const float dvdx = xp_ok ? (float)dplus(W0[0], W1[0], W2[0], W3[0], c1x, c2x) : 0.f;
                 const float dvdz = zp_ok ? (float)dplus(zm1, W1[0], zp1, zp2, c1z, c2z) : 0.f;
-                const double dtmu = dt * (double)mat[M_MU * NP + kq];
-                st[slot_s(P, wr, 0) * NP + kq] = (float)((double)S0[0] + dtmu * (double)dvdx);
-                st[slot_s(P, wr, 1) * NP + kq] = (float)((double)S0[1] + dtmu * (double)dvdz);
+                const double dtmu = dt * (double)cur.mu;
+                s_out[0][kq] = (float)((double)S0[0] + dtmu * (double)dvdx);
+                s_out[1][kq] = (float)((double)S0[1] + dtmu * (double)dvdz);
             }
         } else {
             // div_vxz + add_sxz (solver.py:119-126, 143-149)
@@ -322,29 +413,335 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
                 const float dvzdx = xp_ok ? (float)dplus(W0[1], W1[1], W2[1], W3[1], c1x, c2x) : 0.f;
                 const float dvxdz = zp_ok ? (float)dplus(xm1, W1[0], xp1, xp2, c1z, c2z) : 0.f;
                 const float dvzdz = zp_ok ? (float)dplus(zm1, W1[1], zp1, zp2, c1z, c2z) : 0.f;
-                const float lam = mat[M_LAM * NP + kq], mu = mat[M_MU * NP + kq];
+                const float lam = cur.lam, mu = cur.mu;
                 const double l2m = (double)lam + 2.0 * (double)mu;
-                st[slot_s(P, wr, 0) * NP + kq] =
-                    (float)((double)S0[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz)));
-                st[slot_s(P, wr, 1) * NP + kq] =
-                    (float)((double)S0[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx)));
-                st[slot_s(P, wr, 2) * NP + kq] =
-                    (float)((double)S0[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+                s_out[0][kq] = (float)((double)S0[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz)));
+                s_out[1][kq] = (float)((double)S0[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx)));
+                s_out[2][kq] = (float)((double)S0[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
                 if constexpr (P == SP_SPIN) {
                     // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
                     const float dcdx = xp_ok ? (float)dplus(W0[2], W1[2], W2[2], W3[2], c1x, c2x) : 0.f;
                     const float dcdz = zp_ok ? (float)dplus(cm1, W1[2], cp1, cp2, c1z, c2z) : 0.f;
-                    const float nu = mat[M_NU * NP + kq];
-                    const double dtg = dt * (double)mat[M_GAM * NP + kq];
-                    st[slot_s(P, wr, 4) * NP + kq] =
-                        (float)((double)S0[4] + dt2 * (double)nu
-                                * (-(double)W1[2] - 0.5 * (double)sub_rn(dvzdx, dvxdz)));
-                    st[slot_s(P, wr, 3) * NP + kq] = (float)((double)S0[3] + dtg * (double)dcdx);
-                    st[slot_s(P, wr, 5) * NP + kq] = (float)((double)S0[5] + dtg * (double)dcdz);
+                    const double dtg = dt * (double)cur.gam;
+                    s_out[4][kq] = (float)((double)S0[4] + dt2 * (double)cur.nu
+                                           * (-(double)W1[2] - 0.5 * (double)sub_rn(dvzdx, dvxdz)));
+                    s_out[3][kq] = (float)((double)S0[3] + dtg * (double)dcdx);
+                    s_out[5][kq] = (float)((double)S0[5] + dtg * (double)dcdz);
                 }
             }
         }
     }
+}
+
+// ------------------------------------------------- step kernel, version 3
+// Same algorithm and rounding as k_step; the difference is memory-pipeline
+// behaviour.  On gfx9 loads and stores share one in-order vmcnt counter, so
+// the compiler can only wait for exactly the prefetched row (and not drain
+// every outstanding load and store) when each row iteration issues the same
+// sequence of vector memory operations.  Hence:
+//   * buffer loads/stores with one 32-bit byte offset per row, the field's
+//     slot offset in an SGPR and the base in a buffer resource;
+//   * loads always issue (rows and lanes clamped into the grid); stores
+//     always issue, with an offset past the end of the buffer for lanes or
+//     rows that must not be written -- the buffer range check drops them;
+//   * snapshots and receiver records are separate kernels (k_snap,
+//     k_record); source injection reads the time function with scalar
+//     loads only;
+//   * the row loop is unrolled by four with windows indexed by row & 3, so
+//     the register windows never move.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr unsigned OOB = 0x40000000u;  // >= every per-shot buffer size (checked on the host)
+// read-only, wave-uniform data through the constant address space: scalar
+// loads, counted by lgkmcnt and never by the vector memory counter
+typedef const int __attribute__((address_space(4))) *cint_p;
+typedef const float __attribute__((address_space(4))) *cflt_p;
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, long long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+}
+__device__ __forceinline__ void bst(rsrc_t r, unsigned voff, int soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, soff, 0);
+}
+
+template <int P>
+struct Row3 {
+    float s[n_s(P)];   // old stresses, row m
+    float v[n_v(P)];   // old velocities, row m-1
+    float u[n_v(P)];   // displacement, row m-1
+    float rho, jj, bx; // row m-1
+    float lam, mu, nu, gam;  // row m-3
+};
+
+template <int P>
+__device__ __forceinline__ void load3(Row3<P> &p, rsrc_t RS, rsrc_t RM, cflt_p bxp, int NP4, int rd, int mp, int nx,
+                                      int nz, int jc, int i0, int i1)
+{
+    constexpr int NS = n_s(P), NV = n_v(P);
+    // each field's row is clamped to the rows the chunk needs, so the extra
+    // prologue/epilogue iterations re-read cached rows instead of new ones:
+    // stresses i0-3..i1+2, velocities i0-1..i1+1, u and material i0..i1-1
+    const int ms = min(max(mp, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
+    const int mr = min(max(mp - 1, max(i0 - 1, 0)), min(i1 + 1, nx - 1));
+    const int mu_ = min(max(mp - 1, i0), i1 - 1);
+    const int mq = min(max(mp - 3, i0), i1 - 1);
+    const unsigned os = (unsigned)(ms * nz + jc) * 4u, orr = (unsigned)(mr * nz + jc) * 4u,
+                   ou = (unsigned)(mu_ * nz + jc) * 4u, oq = (unsigned)(mq * nz + jc) * 4u;
+#pragma unroll
+    for (int f = 0; f < NS; f++) p.s[f] = bld(RS, os, slot_s(P, rd, f) * NP4);
+#pragma unroll
+    for (int f = 0; f < NV; f++) {
+        p.v[f] = bld(RS, orr, slot_v(P, rd, f) * NP4);
+        p.u[f] = bld(RS, ou, slot_u(P, f) * NP4);
+    }
+    p.rho = bld(RM, orr, M_RHO * NP4);
+    p.bx = bxp[mr];
+    p.jj = 1.f;
+    if constexpr (P == SP_SPIN) p.jj = bld(RM, orr, M_J * NP4);
+    p.mu = bld(RM, oq, M_MU * NP4);
+    p.lam = p.nu = p.gam = 0.f;
+    if constexpr (P != SP_SH) p.lam = bld(RM, oq, M_LAM * NP4);
+    if constexpr (P == SP_SPIN) {
+        p.nu = bld(RM, oq, M_NU * NP4);
+        p.gam = bld(RM, oq, M_GAM * NP4);
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_step3(StepArgs a)
+{
+    constexpr int NS = n_s(P), NV = n_v(P);
+    const int nb = gridDim.x, bb = blockIdx.x;
+    const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
+    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
+    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
+    if (wave >= a.nitems) return;
+    const int L = threadIdx.x & 63;
+    const int4 item = a.items[2 * wave];
+    if (item.x < 0) return;
+    const int4 item2 = a.items[2 * wave + 1];
+    const ShotDesc &d = a.shots[item.x];
+    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
+    const int j = item.y - a.shift + L;
+    const int jc = min(max(j, 0), nz - 1);
+    const bool jin = j >= 0 && j < nz;
+    const bool outl = jin && L >= a.shift && L < a.shift + a.wout;
+    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
+    const int NP4 = (int)(d.Npad * 4);  // bytes per field
+    const double dt = a.dt, dt2 = 2.0 * a.dt;
+    const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
+    const int rd = a.rd, wr = a.rd ^ 1;
+    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
+    const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
+    const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
+    const cflt_p bxp = (cflt_p)d.bx;
+
+    // injection list: sources (forward) or receivers with adjoint sources
+    const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
+    const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
+    const int ni = a.adjoint ? d.nrec : d.nsrc;
+    const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
+    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
+    int pi = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
+    int src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+
+    float S[4][NS], W[4][NV];
+    Row3<P> PF[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+#pragma unroll
+        for (int f = 0; f < NS; f++) S[u][f] = 0.f;
+#pragma unroll
+        for (int f = 0; f < NV; f++) W[u][f] = 0.f;
+    }
+
+    // prologue (i0 is a multiple of 4): rows i0-3, i0-2, i0-1 fill stress
+    // slots 1, 2, 3; rows up to i0+1 are prefetched.
+    load3<P>(PF[1], RS, RM, bxp, NP4, rd, i0 - 3, nx, nz, jc, i0, i1);
+    load3<P>(PF[2], RS, RM, bxp, NP4, rd, i0 - 2, nx, nz, jc, i0, i1);
+    load3<P>(PF[3], RS, RM, bxp, NP4, rd, i0 - 1, nx, nz, jc, i0, i1);
+    load3<P>(PF[0], RS, RM, bxp, NP4, rd, i0, nx, nz, jc, i0, i1);
+#pragma unroll
+    for (int f = 0; f < NS; f++) {
+        S[1][f] = PF[1].s[f];
+        S[2][f] = PF[2].s[f];
+        S[3][f] = PF[3].s[f];
+    }
+    load3<P>(PF[1], RS, RM, bxp, NP4, rd, i0 + 1, nx, nz, jc, i0, i1);
+
+    const int mend = i1 + 2;
+    for (int mb = i0; mb <= mend; mb += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int m = mb + u;
+            if (m > mend) break;
+            const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
+            // ---- prefetch row m+2 into slot (u+2)&3 (always issued)
+            load3<P>(PF[B], RS, RM, bxp, NP4, rd, m + 2, nx, nz, jc, i0, i1);
+            const Row3<P> &cur = PF[D];
+#pragma unroll
+            for (int f = 0; f < NS; f++) S[D][f] = cur.s[f];
+            // stress window rows m-3, m-2, m-1, m = slots A, B, C, D
+            float(&Sa)[NS] = S[A];
+            float(&Sb)[NS] = S[B];
+            float(&Sc)[NS] = S[C];
+            float(&Sd)[NS] = S[D];
+
+            // ---- new velocities of row r = m-1
+            const int r = m - 1;
+            const bool rin = r >= 0 && r < nx;
+            const bool xm_ok = r >= 2 && r < nx - 2;
+            float dsv[NV];
+            if constexpr (P == SP_SH) {
+                float zm2 = shfl(Sc[1], L - 2), zm1 = shfl(Sc[1], L - 1), zp1 = shfl(Sc[1], L + 1);
+                double ddx = xm_ok ? dminus(Sa[0], Sb[0], Sc[0], Sd[0], c1x, c2x) : 0.0;
+                double ddz = zm_ok ? dminus(zm2, zm1, Sc[1], zp1, c1z, c2z) : 0.0;
+                dsv[0] = (float)(ddx + ddz);
+            } else {
+                float xzm2 = shfl(Sc[2], L - 2), xzm1 = shfl(Sc[2], L - 1), xzp1 = shfl(Sc[2], L + 1);
+                float zzm2 = shfl(Sc[1], L - 2), zzm1 = shfl(Sc[1], L - 1), zzp1 = shfl(Sc[1], L + 1);
+                double dxx = xm_ok ? dminus(Sa[0], Sb[0], Sc[0], Sd[0], c1x, c2x) : 0.0;
+                double dxxz = xm_ok ? dminus(Sa[2], Sb[2], Sc[2], Sd[2], c1x, c2x) : 0.0;
+                double dzxz = zm_ok ? dminus(xzm2, xzm1, Sc[2], xzp1, c1z, c2z) : 0.0;
+                double dzzz = zm_ok ? dminus(zzm2, zzm1, Sc[1], zzp1, c1z, c2z) : 0.0;
+                dsv[0] = (float)(dxx + dzxz);
+                dsv[1] = (float)(dxxz + dzzz);
+                if constexpr (P == SP_SPIN) {
+                    float yzm2 = shfl(Sc[5], L - 2), yzm1 = shfl(Sc[5], L - 1), yzp1 = shfl(Sc[5], L + 1);
+                    float yym2 = shfl(Sc[4], L - 2), yym1 = shfl(Sc[4], L - 1), yyp1 = shfl(Sc[4], L + 1);
+                    double dxyx = xm_ok ? dminus(Sa[3], Sb[3], Sc[3], Sd[3], c1x, c2x) : 0.0;
+                    double dzyz = zm_ok ? dminus(yzm2, yzm1, Sc[5], yzp1, c1z, c2z) : 0.0;
+                    float dsy = (float)(dxyx + dzyz);
+                    double dzyy = zm_ok ? dminus(yym2, yym1, Sc[4], yyp1, c1z, c2z) : 0.0;
+                    double dxyy = xm_ok ? dminus(Sa[4], Sb[4], Sc[4], Sd[4], c1x, c2x) : 0.0;
+                    dsv[0] = (float)((double)dsv[0] + dzyy);
+                    dsv[1] = (float)((double)dsv[1] - dxyy);
+                    dsv[2] = (float)((double)dsy + 2.0 * (double)Sc[4]);
+                }
+            }
+            if (r == src_row) {
+                // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list
+                // order; every read here is wave-uniform (scalar loads)
+                const int kend = (r + 1) * nz, kr = r * nz + j;
+                while (pi < ni && ik[pi] < kend) {
+                    const int kk = ik[pi];
+                    const int row = irow[pi];
+                    const float s0 = ival[(long long)row * a.nt + a.it];
+                    float s1 = 0.f;
+                    if constexpr (P != SP_SH) s1 = ival[(nrows + row) * a.nt + a.it];
+                    if (kk == kr) {
+                        dsv[0] = add_rn(dsv[0], s0);
+                        if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
+                    }
+                    ++pi;
+                }
+                src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+            }
+            const float b1 = (float)((double)cur.bx * bzb);
+            const double b = (double)(float)((double)b1 * bzt);
+            const double dtr = dt * rcp64((double)cur.rho);
+            float vn[NV], un[NV];
+#pragma unroll
+            for (int f = 0; f < NV; f++) {
+                double dtm = dtr;
+                if constexpr (P == SP_SPIN) {
+                    if (f == 2) dtm = dt * rcp64((double)cur.jj);
+                }
+                const float v = (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm));
+                vn[f] = (rin && jin) ? v : 0.f;
+                un[f] = (float)((double)cur.u[f] + (double)vn[f] * dt);
+            }
+            {
+                const unsigned ov = (outl && r >= i0 && r < i1) ? (unsigned)(r * nz + j) * 4u : OOB;
+#pragma unroll
+                for (int f = 0; f < NV; f++) {
+                    bst(RS, ov, slot_u(P, f) * NP4, un[f]);
+                    bst(RS, ov, slot_v(P, wr, f) * NP4, vn[f]);
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < NV; f++) W[C][f] = vn[f];
+
+            // ---- new stresses of row q = m-3 (velocity rows q-1..q+2 = slots D, A, B, C)
+            const int qr = m - 3;
+            float(&Wa)[NV] = W[D];
+            float(&Wb)[NV] = W[A];
+            float(&Wc)[NV] = W[B];
+            float(&Wd)[NV] = W[C];
+            const bool xp_ok = qr >= 1 && qr < nx - 2;
+            const unsigned oq = (outl && qr >= i0 && qr < i1) ? (unsigned)(qr * nz + j) * 4u : OOB;
+            if constexpr (P == SP_SH) {
+                float zm1 = shfl(Wb[0], L - 1), zp1 = shfl(Wb[0], L + 1), zp2 = shfl(Wb[0], L + 2);
+                const float dvdx = xp_ok ? (float)dplus(Wa[0], Wb[0], Wc[0], Wd[0], c1x, c2x) : 0.f;
+                const float dvdz = zp_ok ? (float)dplus(zm1, Wb[0], zp1, zp2, c1z, c2z) : 0.f;
+                const double dtmu = dt * (double)cur.mu;
+                bst(RS, oq, slot_s(P, wr, 0) * NP4, (float)((double)Sa[0] + dtmu * (double)dvdx));
+                bst(RS, oq, slot_s(P, wr, 1) * NP4, (float)((double)Sa[1] + dtmu * (double)dvdz));
+            } else {
+                float xm1 = shfl(Wb[0], L - 1), xp1 = shfl(Wb[0], L + 1), xp2 = shfl(Wb[0], L + 2);
+                float zm1 = shfl(Wb[1], L - 1), zp1 = shfl(Wb[1], L + 1), zp2 = shfl(Wb[1], L + 2);
+                const float dvxdx = xp_ok ? (float)dplus(Wa[0], Wb[0], Wc[0], Wd[0], c1x, c2x) : 0.f;
+                const float dvzdx = xp_ok ? (float)dplus(Wa[1], Wb[1], Wc[1], Wd[1], c1x, c2x) : 0.f;
+                const float dvxdz = zp_ok ? (float)dplus(xm1, Wb[0], xp1, xp2, c1z, c2z) : 0.f;
+                const float dvzdz = zp_ok ? (float)dplus(zm1, Wb[1], zp1, zp2, c1z, c2z) : 0.f;
+                const float lam = cur.lam, mu = cur.mu;
+                const double l2m = (double)lam + 2.0 * (double)mu;
+                bst(RS, oq, slot_s(P, wr, 0) * NP4,
+                    (float)((double)Sa[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz))));
+                bst(RS, oq, slot_s(P, wr, 1) * NP4,
+                    (float)((double)Sa[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx))));
+                bst(RS, oq, slot_s(P, wr, 2) * NP4,
+                    (float)((double)Sa[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx))));
+                if constexpr (P == SP_SPIN) {
+                    float cm1 = shfl(Wb[2], L - 1), cp1 = shfl(Wb[2], L + 1), cp2 = shfl(Wb[2], L + 2);
+                    const float dcdx = xp_ok ? (float)dplus(Wa[2], Wb[2], Wc[2], Wd[2], c1x, c2x) : 0.f;
+                    const float dcdz = zp_ok ? (float)dplus(cm1, Wb[2], cp1, cp2, c1z, c2z) : 0.f;
+                    const double dtg = dt * (double)cur.gam;
+                    bst(RS, oq, slot_s(P, wr, 4) * NP4,
+                        (float)((double)Sa[4] + dt2 * (double)cur.nu
+                                * (-(double)Wb[2] - 0.5 * (double)sub_rn(dvzdx, dvxdz))));
+                    bst(RS, oq, slot_s(P, wr, 3) * NP4, (float)((double)Sa[3] + dtg * (double)dcdx));
+                    bst(RS, oq, slot_s(P, wr, 5) * NP4, (float)((double)Sa[5] + dtg * (double)dcdz));
+                }
+            }
+        }
+    }
+}
+
+// save_obs (solver.py:151-156) for every receiver of a batch, after a step:
+// one thread per (shot, receiver) of the batch.
+__global__ __launch_bounds__(256) void k_record(const ShotDesc *shots, int nshots, int maxrec, int it, int nt, int P)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int s = t / maxrec, q = t - s * maxrec;
+    if (s >= nshots) return;
+    const ShotDesc &d = shots[s];
+    if (q >= d.nrec) return;
+    const int k = d.rec_k[q], row = d.rec_row[q];
+    const int nv = n_v(P);
+    for (int f = 0; f < nv; f++)
+        d.rec[((long long)f * d.nrec + row) * nt + it] = d.state[(long long)(2 * n_s(P) + 2 * nv + f) * d.Npad + k];
+}
+
+// Adjoint snapshots (solver.py:193-203, 236-242): u before a step (part 0)
+// or the new v after it (part 1) into snapshot isa, for every shot.
+__global__ __launch_bounds__(256) void k_snap(const ShotDesc *shots, int nshots, long long maxN, int isa, int part,
+                                              int vset, int P)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int s = blockIdx.y;
+    if (s >= nshots) return;
+    const ShotDesc &d = shots[s];
+    if (k >= (long long)d.nx * d.nz) return;
+    const int nv = n_v(P), ns = n_s(P);
+    const long long NP = d.Npad;
+    float *dst = d.snap + ((long long)isa * 2 * nv + part * nv) * NP + k;
+    const float *src = d.state + (part == 0 ? (long long)(2 * ns + 2 * nv) : (long long)(2 * ns + vset * nv)) * NP + k;
+    for (int f = 0; f < nv; f++) dst[f * NP] = src[f * NP];
 }
 
 // ------------------------------------------------------ interaction kernel
@@ -571,6 +968,7 @@ struct Grid {
 
 struct Shot {
     int grid = 0, nsrc = 0, nrec = 0;
+    std::vector<int> src_sorted, rec_sorted;  // grid indices in list order
     int *d_idx = nullptr;  // src_k, src_row, rec_k, rec_row
     float *stf = nullptr, *state = nullptr, *rec = nullptr, *data = nullptr, *adstf = nullptr;
     float *snap = nullptr;  // into the context's snapshot pool
@@ -588,7 +986,10 @@ struct sp_ctx {
     std::string err;
     // tuning
     int chunk = 0;         // 0: automatic
-    int order = 1;
+    int order = 0;
+    int kernel = 1;
+    int cur_kernel = 1;        // kernel of the prepared batch
+    int shift = 3, wout = 58;  // strip geometry of the step kernel
     int graph = 0;
     long long snap_mb = 0;
     // snapshot pool
@@ -700,18 +1101,31 @@ ShotDesc make_desc(const sp_ctx *c, const Shot &s)
     return d;
 }
 
+// The buffer-addressed kernel needs every per-shot buffer below the
+// out-of-range store offset; larger grids use the plain kernel.
+int batch_kernel(const sp_ctx *c, const std::vector<int> &batch)
+{
+    if (c->kernel != 1) return 0;
+    for (int s : batch) {
+        const Grid &g = c->grids[c->shots[s].grid];
+        const long long bytes = (long long)std::max(n_slots(c->physics), N_MAT) * g.Npad * 4;
+        if (bytes >= (long long)OOB) return 0;
+    }
+    return 1;
+}
+
 int auto_chunk(const sp_ctx *c, const std::vector<int> &batch)
 {
-    if (c->chunk > 0) return c->chunk;
+    if (c->chunk > 0) return batch_kernel(c, batch) == 1 ? (c->chunk + 3) / 4 * 4 : c->chunk;
     // enough wavefronts to fill 256 CUs several times over, long chunks otherwise
     long long waves64 = 0;
     for (int s : batch) {
         const Grid &g = c->grids[c->shots[s].grid];
         waves64 += (long long)((g.nz + 57) / 58) * ((g.nx + 63) / 64);
     }
-    if (waves64 >= 8192) return 64;
-    if (waves64 >= 2048) return 32;
-    return 16;
+    if (waves64 >= 2048) return 16;
+    if (waves64 >= 256) return 8;
+    return 4;
 }
 
 // Upload descriptors and the wavefront work list of a batch.
@@ -720,30 +1134,79 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     std::vector<ShotDesc> desc;
     for (int s : batch) desc.push_back(make_desc(c, c->shots[s]));
     const int chunk = auto_chunk(c, batch);
+    c->cur_kernel = batch_kernel(c, batch);
+    const int W = c->cur_kernel == 1 ? c->wout : 58;
     std::vector<int4> items;
-    if (c->order == 1) {
+    auto push = [&](int b, int j0, int i0) {
+        const Shot &sh = c->shots[batch[b]];
+        const Grid &g = c->grids[sh.grid];
+        const int i1 = std::min(i0 + chunk, g.nx);
+        auto first = [](const std::vector<int> &v, long long key) {
+            return (int)(std::lower_bound(v.begin(), v.end(), (int)std::max(key, -1LL)) - v.begin());
+        };
+        items.push_back(make_int4(b, j0, i0, i1));
+        items.push_back(make_int4(first(sh.src_sorted, (long long)(i0 - 1) * g.nz),
+                                  first(sh.rec_sorted, (long long)(i0 - 1) * g.nz),
+                                  first(sh.rec_sorted, (long long)i0 * g.nz), 0));
+    };
+    if (c->order == 3) {
+        // strip-group-major: a block's four wavefronts are four z-adjacent
+        // strips of one shot (one ~1 KB contiguous segment per field row);
+        // the next blocks take the same strips of the following shots of
+        // the grid, which reuse the material rows from L2.
+        std::vector<std::vector<int>> per_grid(c->grids.size());
+        for (int b = 0; b < (int)batch.size(); b++) per_grid[c->shots[batch[b]].grid].push_back(b);
+        for (size_t g = 0; g < per_grid.size(); g++) {
+            const Grid &G = c->grids[g];
+            const int nstrip = (G.nz + W - 1) / W;
+            for (int i0 = 0; i0 < G.nx; i0 += chunk)
+                for (int sg = 0; sg < nstrip; sg += 4)
+                    for (int b : per_grid[g]) {
+                        for (int sp = sg; sp < sg + 4; sp++) {
+                            if (sp < nstrip) push(b, sp * W, i0);
+                            else items.push_back(make_int4(-1, 0, 0, 0)), items.push_back(make_int4(0, 0, 0, 0));
+                        }
+                    }
+        }
+    } else if (c->order == 2) {
+        // tile-major: groups of four shots of one grid walk the tiles
+        // (chunk-major, strips inner); a block's four wavefronts are the
+        // four shots of one tile, so they share its material rows, and
+        // consecutive blocks are neighbouring strips, so they share halos.
+        std::vector<std::vector<int>> per_grid(c->grids.size());
+        for (int b = 0; b < (int)batch.size(); b++) per_grid[c->shots[batch[b]].grid].push_back(b);
+        for (size_t g = 0; g < per_grid.size(); g++) {
+            const auto &list = per_grid[g];
+            const Grid &G = c->grids[g];
+            for (size_t s0 = 0; s0 < list.size(); s0 += 4) {
+                const size_t s1 = std::min(list.size(), s0 + 4);
+                for (int i0 = 0; i0 < G.nx; i0 += chunk)
+                    for (int j0 = 0; j0 < G.nz; j0 += W)
+                        for (size_t q = s0; q < s1; q++) push(list[q], j0, i0);
+            }
+        }
+    } else if (c->order == 1) {
         // shot-inner: consecutive wavefronts of a block share a tile of the
         // material and differ in shot (same grid) or are neighbours in z
         int maxs = 0, maxc = 0;
         for (int s : batch) {
             const Grid &g = c->grids[c->shots[s].grid];
-            maxs = std::max(maxs, (g.nz + 57) / 58);
+            maxs = std::max(maxs, (g.nz + W - 1) / W);
             maxc = std::max(maxc, (g.nx + chunk - 1) / chunk);
         }
         for (int ch = 0; ch < maxc; ch++)
             for (int sp = 0; sp < maxs; sp++)
                 for (int b = 0; b < (int)batch.size(); b++) {
                     const Grid &g = c->grids[c->shots[batch[b]].grid];
-                    const int i0 = ch * chunk, j0 = sp * 58;
+                    const int i0 = ch * chunk, j0 = sp * W;
                     if (i0 >= g.nx || j0 >= g.nz) continue;
-                    items.push_back(make_int4(b, j0, i0, std::min(i0 + chunk, g.nx)));
+                    push(b, j0, i0);
                 }
     } else {
         for (int b = 0; b < (int)batch.size(); b++) {
             const Grid &g = c->grids[c->shots[batch[b]].grid];
             for (int i0 = 0; i0 < g.nx; i0 += chunk)
-                for (int j0 = 0; j0 < g.nz; j0 += 58)
-                    items.push_back(make_int4(b, j0, i0, std::min(i0 + chunk, g.nx)));
+                for (int j0 = 0; j0 < g.nz; j0 += W) push(b, j0, i0);
         }
     }
     if (ensure(c, c->d_desc, c->d_desc_cap, desc.size())) return 2;
@@ -751,7 +1214,7 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     HIPCHECK(c, hipMemcpyAsync(c->d_desc, desc.data(), desc.size() * sizeof(ShotDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    *nitems_out = (int)items.size();
+    *nitems_out = (int)items.size() / 2;
     return 0;
 }
 
@@ -767,11 +1230,21 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint)
     a.rd = it & 1;
     a.snap_idx = snap_idx;
     a.adjoint = adjoint;
+    a.shift = c->cur_kernel == 1 ? c->shift : 3;
+    a.wout = c->cur_kernel == 1 ? c->wout : 58;
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
-    switch (c->physics) {
-    case SP_SH: hipLaunchKernelGGL(k_step<SP_SH>, grid, block, 0, c->stream, a); break;
-    case SP_PSV: hipLaunchKernelGGL(k_step<SP_PSV>, grid, block, 0, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_step<SP_SPIN>, grid, block, 0, c->stream, a); break;
+    if (c->cur_kernel == 1) {
+        switch (c->physics) {
+        case SP_SH: hipLaunchKernelGGL(k_step3<SP_SH>, grid, block, 0, c->stream, a); break;
+        case SP_PSV: hipLaunchKernelGGL(k_step3<SP_PSV>, grid, block, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_step3<SP_SPIN>, grid, block, 0, c->stream, a); break;
+        }
+    } else {
+        switch (c->physics) {
+        case SP_SH: hipLaunchKernelGGL(k_step<SP_SH>, grid, block, 0, c->stream, a); break;
+        case SP_PSV: hipLaunchKernelGGL(k_step<SP_PSV>, grid, block, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_step<SP_SPIN>, grid, block, 0, c->stream, a); break;
+        }
     }
     HIPCHECK(c, hipGetLastError());
     c->t_launches++;
@@ -810,10 +1283,28 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     if (prepare_batch(c, batch, &nitems)) return 2;
     if (clear_state(c, batch)) return 2;
     const int nsa = sae > 0 ? c->nt / sae : 0;
+    const int nb = (int)batch.size();
+    int maxrec = 1;
+    long long maxN = 1;
+    for (int s : batch) {
+        maxrec = std::max(maxrec, c->shots[s].nrec);
+        maxN = std::max(maxN, c->grids[c->shots[s].grid].N);
+    }
+    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256)), sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
+    const bool split = c->cur_kernel == 1;  // snapshots and records as kernels of their own
     for (int it = 0; it < c->nt; it++) {
         int isa = -1;
         if (sae > 0 && (it + 1) % sae == 0) isa = nsa - (it + 1) / sae;
-        if (launch_step(c, nitems, it, isa, 0)) return 2;
+        if (split && isa >= 0)
+            hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 0, 0, c->physics);
+        if (launch_step(c, nitems, it, split ? -1 : isa, 0)) return 2;
+        if (split) {
+            if (isa >= 0)
+                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 1, (it & 1) ^ 1,
+                                   c->physics);
+            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics);
+            HIPCHECK(c, hipGetLastError());
+        }
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
     return 0;
@@ -1121,8 +1612,19 @@ int sp_set_option(sp_ctx *c, int key, long long v)
         if (v < 0 || v > 4096) return fail(c, "chunk must be in [0, 4096]");
         c->chunk = (int)v;
         return 0;
-    case SP_OPT_ORDER: c->order = v ? 1 : 0; return 0;
+    case SP_OPT_ORDER:
+        if (v < 0 || v > 3) return fail(c, "order must be 0, 1, 2 or 3");
+        c->order = (int)v;
+        return 0;
     case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;
+    case SP_OPT_KERNEL: c->kernel = v ? 1 : 0; return 0;
+    case SP_OPT_STRIP: {
+        const int sh = (int)(v >> 16), wo = (int)(v & 0xffff);
+        if (sh < 3 || wo < 1 || sh + wo > 61) return fail(c, "strip geometry needs shift >= 3 and shift + width <= 61");
+        c->shift = sh;
+        c->wout = wo;
+        return 0;
+    }
     case SP_OPT_SNAP_MB:
         if (v < 0) return fail(c, "snapshot budget must be >= 0");
         c->snap_mb = v;
@@ -1270,6 +1772,8 @@ int sp_add_shot(sp_ctx *c, int grid, int nsrc, const int *src_k, const float *st
         idx[2 * nsrc + q] = rec_k[ro[q]];
         idx[2 * nsrc + nrec + q] = ro[q];
     }
+    s.src_sorted.assign(idx.begin(), idx.begin() + nsrc);
+    s.rec_sorted.assign(idx.begin() + 2 * nsrc, idx.begin() + 2 * nsrc + nrec);
     const int P = c->physics;
     HIPCHECK(c, hipMalloc(&s.d_idx, idx.size() * sizeof(int)));
     HIPCHECK(c, hipMemcpyAsync(s.d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -1476,6 +1980,50 @@ int sp_smooth(sp_ctx *c, float *field, int nx, int nz, int sigma, int on_dev)
         return 2;
     }
     HIPCHECK(c, hipFree(d));
+    return 0;
+}
+
+int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (c->shots.empty()) return fail(c, "no shots");
+    if (steps < 1) return fail(c, "steps must be >= 1");
+    steps = std::min(steps, c->nt);
+    std::vector<int> all(c->shots.size());
+    std::iota(all.begin(), all.end(), 0);
+    const int chunks[] = {4, 8, 16, 32, 64};
+    const int keep_chunk = c->chunk, keep_order = c->order;
+    float best = 1e30f;
+    int bc = 16, bo = 0;
+    if (timing_begin(c)) return 2;
+    for (int order = 0; order < 4; order++) {
+        for (int ch : chunks) {
+            c->chunk = ch;
+            c->order = order;
+            int nitems = 0;
+            if (prepare_batch(c, all, &nitems) || clear_state(c, all)) return 2;
+            if (launch_step(c, nitems, 0, -1, 0)) return 2;
+            HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+            for (int it = 0; it < steps; it++)
+                if (launch_step(c, nitems, it, -1, 0)) return 2;
+            HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHECK(c, hipEventSynchronize(c->ev1));
+            float ms = 0;
+            HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (ms < best) {
+                best = ms;
+                bc = ch;
+                bo = order;
+            }
+        }
+    }
+    (void)keep_chunk;
+    (void)keep_order;
+    c->chunk = bc;
+    c->order = bo;
+    c->snaps_valid = 0;
+    if (chunk_out) *chunk_out = bc;
+    if (order_out) *order_out = bo;
     return 0;
 }
 

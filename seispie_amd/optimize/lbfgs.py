@@ -1,0 +1,70 @@
+import numpy as np
+
+from seispie_amd.optimize.cg import cg
+from seispie_amd.optimize.line_search.backtrack import backtrack
+
+
+class lbfgs(cg):
+    """L-BFGS two-loop recursion with backtracking
+    (seispie/optimize/lbfgs.py:5-59).
+
+    self.precond, when set, replaces the scalar initial inverse Hessian
+    (y.s / y.y) of the two-loop recursion with the callable g -> H0 g; if the
+    resulting direction is not a descent direction the unconditioned
+    L-BFGS direction is used, and if that is not one either the search
+    restarts from steepest descent, as the reference does.
+    """
+
+    def setup(self, workflow):
+        self.solver = workflow.solver
+        self.bracket = backtrack(workflow.solver, self.config)
+        self.bracket.head = 0 if (getattr(self, 'mpi', None) and self.mpi.rank()) else 1
+        self.lbfgs_s = []
+        self.lbfgs_y = []
+        self.lbfgs_mem = int(self.config['lbfgs_mem'])
+
+    def restart_search(self):
+        super().restart_search()
+        self.lbfgs_s = []
+        self.lbfgs_y = []
+
+    def _two_loop(self, g, h0):
+        nused = len(self.lbfgs_s)
+        rh = np.zeros(nused)
+        al = np.zeros(nused)
+        q = np.copy(g)
+        for i in range(nused):
+            rh[i] = 1 / np.dot(self.lbfgs_y[i], self.lbfgs_s[i])
+            al[i] = rh[i] * np.dot(self.lbfgs_s[i], q)
+            q -= al[i] * self.lbfgs_y[i]
+        q = h0(q)
+        for i in range(nused - 1, -1, -1):
+            be = rh[i] * np.dot(self.lbfgs_y[i], q)
+            q += self.lbfgs_s[i] * (al[i] - be)
+        return q
+
+    def compute_direction(self):
+        g_new = self.g_new
+        if not hasattr(self, 'g_old'):
+            return -self._conditioned(g_new)
+
+        self.lbfgs_s.insert(0, self.m_new - self.m_old)
+        self.lbfgs_y.insert(0, self.g_new - self.g_old)
+        if len(self.lbfgs_s) > self.lbfgs_mem:
+            self.lbfgs_s.pop()
+            self.lbfgs_y.pop()
+
+        y0, s0 = self.lbfgs_y[0], self.lbfgs_s[0]
+        scalar = lambda q: q * (np.dot(y0, s0) / np.dot(y0, y0))
+        q = None
+        if self.precond is not None:
+            q = self._two_loop(g_new, self.precond)
+            if self.bracket.angle(g_new, q) >= np.pi / 2:
+                q = None
+        if q is None:
+            q = self._two_loop(g_new, scalar)
+        if self.bracket.angle(g_new, q) >= np.pi / 2:
+            print('  restarting LBFGS: not descent')
+            self.restart_search()
+            return -g_new
+        return -q
