@@ -99,6 +99,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--order", type=int, default=0, help="work order when --chunk is given")
+    ap.add_argument("--nt", type=int, default=0, help="override the number of time steps")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="time forward passes (the step kernel) instead of gradients; for counter runs")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 256 MiB elementwise kernel whose HBM bytes are known, to scale "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -106,7 +112,21 @@ def main():
         return 2
 
     from seispie_amd import _lib, configs
+    if args.calibrate:
+        import torch
+        a = torch.full((64 << 20,), 1.0, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        del a, b
     true, init = build_problem(args.config)
+    if args.nt:
+        for p in (true, init):
+            p.nt = args.nt
+            for g in p.grids:
+                for s in g.shots:
+                    s.stf = np.ascontiguousarray(s.stf[..., :args.nt])
     t0 = time.time()
     data = observed_data(true)
     print("# observed data for %s in %.1fs" % (true.name, time.time() - t0), file=sys.stderr, flush=True)
@@ -120,6 +140,18 @@ def main():
         ctx.set_data(i, d)
     del data
     sae = init.sae
+    if args.forward_only:
+        if args.chunk:
+            ctx.set_option(_lib.OPT_CHUNK, args.chunk)
+            ctx.set_option(_lib.OPT_ORDER, args.order)
+        for _ in range(args.warmup + args.steps):
+            ctx.forward(0)
+        ctx.sync()
+        t = ctx.timing()
+        print(json.dumps({"forward_step_ms": t["total_ms"] / t["step_launches"],
+                          "alg_bytes_per_launch": t["alg_bytes"] / t["step_launches"],
+                          "workload": init.name}))
+        return 0
     if not args.chunk:
         tuned = ctx.autotune(20)
         print("# autotuned chunk=%d order=%d" % tuned, file=sys.stderr, flush=True)
@@ -150,7 +182,7 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tf):
-        t = json.load(open(tf)).get(init.name)
+        t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0])
         if t:
             traffic = t["hbm_bytes_per_launch"]
 

@@ -1,0 +1,141 @@
+"""The INI runner and the plugin surface: config inheritance on the CPU; an
+end-to-end forward + adjoint workflow through the fd2d plugin on the GPU,
+checked against the oracle.  Example inputs are written in the reference's
+formats (proc000000_<name>.bin = int32 npt + float32, sources.dat,
+stations.dat) into a temporary directory."""
+import os
+
+import numpy as np
+import pytest
+
+from seispie_amd import sprun
+
+FORWARD_INI = """[workflow]
+mode = forward
+mpi = no
+
+[solver]
+method = fd2d
+sh = yes
+psv = no
+nt = {nt}
+dt = {dt}
+abs_top = no
+abs_bottom = yes
+abs_left = yes
+abs_right = yes
+abs_width = 12
+abs_alpha = 0.015
+save_coordinates = yes
+save_snapshot = 0
+threads_per_block = 512
+combine_sources = yes
+
+[path]
+output = output
+model_init = model_init
+model_true = model_true
+sources = sources.dat
+stations = stations.dat
+"""
+
+ADJOINT_INI = """[workflow]
+mode = adjoint
+inherit = forward
+mpi = no
+
+[solver]
+combine_sources = no
+save_snapshot = 0
+adjoint_interval = 10
+smooth = 3
+
+[postprocess]
+method = default
+
+[path]
+output = output
+"""
+
+
+def _write_model(d, nx, nz, h, vp, vs, rho):
+    os.makedirs(d, exist_ok=True)
+    x = np.repeat(np.arange(nx) * h, nz).astype(np.float32)
+    z = np.tile(np.arange(nz) * h, nx).astype(np.float32)
+    npt = np.array([nx * nz], np.int32)
+    for name, v in (("x", x), ("z", z), ("vp", vp), ("vs", vs), ("rho", rho)):
+        with open(os.path.join(d, "proc000000_%s.bin" % name), "wb") as f:
+            npt.tofile(f)
+            np.asarray(v, np.float32).tofile(f)
+
+
+def make_examples(root, nx=60, nz=50, h=10.0, nt=300, dt=1e-3):
+    fw = os.path.join(root, "forward")
+    ad = os.path.join(root, "adjoint")
+    os.makedirs(fw)
+    os.makedirs(ad)
+    rng = np.random.default_rng(2)
+    n = nx * nz
+    vs_t = (1800 * (1 + 0.05 * rng.standard_normal(n))).astype(np.float32)
+    _write_model(os.path.join(fw, "model_true"), nx, nz, h, vs_t * 1.7, vs_t, np.full(n, 2500, np.float32))
+    vs_i = np.full(n, 1800, np.float32)
+    _write_model(os.path.join(fw, "model_init"), nx, nz, h, vs_i * 1.7, vs_i, np.full(n, 2500, np.float32))
+    with open(os.path.join(fw, "sources.dat"), "w") as f:
+        f.write("%g %g 0 25.0 0.05 0.0 1.0e10\n" % (20 * h, 20 * h))
+        f.write("%g %g 0 25.0 0.05 0.0 1.0e10\n" % (40 * h, 22 * h))
+    with open(os.path.join(fw, "stations.dat"), "w") as f:
+        for i in range(14, 46, 4):
+            f.write("%g %g\n" % (i * h, 6 * h))
+    open(os.path.join(fw, "config.ini"), "w").write(FORWARD_INI.format(nt=nt, dt=dt))
+    open(os.path.join(ad, "config.ini"), "w").write(ADJOINT_INI)
+    return fw, ad
+
+
+def test_inherit_merges_parent_sections_and_rewrites_paths(tmp_path):
+    fw, ad = make_examples(str(tmp_path))
+    config, _ = sprun.parse_config(ad, "config.ini")
+    assert config["workflow"]["mode"] == "adjoint"
+    assert config["solver"]["combine_sources"] == "no"          # child wins
+    assert config["solver"]["sh"] == "yes"                      # inherited
+    assert config["path"]["output"] == "output"                 # child's own path kept
+    assert config["path"]["sources"] == "../forward/sources.dat"  # parent path rewritten
+
+
+@pytest.mark.gpu
+def test_forward_and_adjoint_workflows_match_the_oracle(tmp_path):
+    from oracle import oracle as O
+    fw, ad = make_examples(str(tmp_path))
+    print("case: sprun forward/adjoint", flush=True)
+    config, wd = sprun.parse_config(fw, "config.ini")
+    wf = sprun.setup(config, wd)
+    s = wf.solver
+    # the reference forward run: combine_sources = yes fires both sources
+    lam, mu = O.vps2lm(*[np.fromfile(os.path.join(fw, "model_true", "proc000000_%s.bin" % k), np.float32)[1:]
+                         for k in ("vp", "vs")], np.fromfile(os.path.join(fw, "model_true", "proc000000_rho.bin"), np.float32)[1:])
+    rho = np.fromfile(os.path.join(fw, "model_true", "proc000000_rho.bin"), np.float32)[1:]
+    bound = O.set_bound(s.nx, s.nz, 12, 0.015, 1, 1, 1, 0)
+    sim = O.Sim(O.SH, s.nx, s.nz, s.dx, s.dz, s.dt, s.nt, {"rho": rho, "lam": lam, "mu": mu}, bound,
+                s.src_id, s.rec_id, -1)
+    stf = O.make_stf(np.loadtxt(os.path.join(fw, "sources.dat"), ndmin=2), s.nt, s.dt)
+    ref = sim.forward(stf)[0]["y"]
+    got = s.obs_y.reshape(s.nrec, s.nt)
+    assert np.abs(ref).max() > 0
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    assert os.path.exists(os.path.join(fw, "output", "proc000000_x.bin"))
+
+    config, wd = sprun.parse_config(ad, "config.ini")
+    wf = sprun.setup(config, wd)
+    kmu = np.fromfile(os.path.join(ad, "output", "proc000000_kmu.bin"), np.float32)[1:]
+    # oracle: traces of the true model per shot, gradient at the initial model
+    data = []
+    for i in range(2):
+        sim.isrc = i
+        data.append({"y": sim.forward(stf)[0]["y"]})
+    vp0, vs0, rho0 = (np.fromfile(os.path.join(fw, "model_init", "proc000000_%s.bin" % k), np.float32)[1:]
+                      for k in ("vp", "vs", "rho"))
+    lam0, mu0 = O.vps2lm(vp0, vs0, rho0)
+    sim0 = O.Sim(O.SH, s.nx, s.nz, s.dx, s.dz, s.dt, s.nt, {"rho": rho0, "lam": lam0, "mu": mu0}, bound,
+                 s.src_id, s.rec_id, 0)
+    mis, K = O.run_kernel(sim0, stf, data, 10, adj=True, sigma=3)
+    assert np.abs(K["mu"]).max() > 0
+    assert np.linalg.norm(kmu - K["mu"]) <= 1e-4 * np.linalg.norm(K["mu"])
