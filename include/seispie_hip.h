@@ -54,7 +54,8 @@ typedef struct sp_ctx sp_ctx;
 #define SP_K_VP 6
 #define SP_K_VS 7
 #define SP_K_RHOP 8
-#define SP_NKERN 9
+#define SP_K_HESS 9   /* pseudo-Hessian diagonal: sum over shots of the forward velocity energy */
+#define SP_NKERN 10
 
 /* tuning options (sp_set_option) */
 #define SP_OPT_CHUNK 1      /* rows per wavefront item in the step kernel */
@@ -68,7 +69,9 @@ typedef struct sp_ctx sp_ctx;
 int sp_create(int device, sp_ctx **out);
 int sp_destroy(sp_ctx *ctx);
 const char *sp_last_error(const sp_ctx *ctx);
-int sp_set_stream(sp_ctx *ctx, void *hip_stream); /* NULL: the context's own stream */
+/* Work of the context goes to its own non-blocking stream until
+ * sp_set_stream names another hipStream_t (NULL: the null stream). */
+int sp_set_stream(sp_ctx *ctx, void *hip_stream);
 int sp_sync(sp_ctx *ctx);
 int sp_set_option(sp_ctx *ctx, int key, long long value);
 
@@ -118,6 +121,22 @@ int sp_adjoint(sp_ctx *ctx, int sae);
  * norms_out as sp_misfit.  compute_misfit (adj=0) is sp_forward(ctx, 0)
  * followed by sp_misfit. */
 int sp_gradient(sp_ctx *ctx, int sae, double *norms_out);
+/* Checkpointed forward and adjoint (no reference counterpart: the
+ * reference keeps every adjoint snapshot in host memory, solver.py:193-203).
+ * sp_forward_ckpt runs every shot, records, and keeps the full state every
+ * C steps (C even).  sp_adjoint_ckpt runs the adjoint with the adjoint
+ * sources currently set (sp_misfit or sp_set_adjoint_source) and
+ * recomputes each C-step segment of the forward run from its checkpoint,
+ * last segment first, for the snapshots of that segment: memory is
+ * nt/C states plus C/sae snapshots per shot instead of nt/sae snapshots. */
+int sp_forward_ckpt(sp_ctx *ctx, int C);
+int sp_adjoint_ckpt(sp_ctx *ctx, int sae);
+/* adjoint sources of a shot, [ncomp_misfit][nrec][nt], already time-reversed
+ * (the layout sp_misfit writes, waveform.py:10,25) */
+int sp_set_adjoint_source(sp_ctx *ctx, int shot, const float *src, int src_on_device);
+/* a new model in vp/vs/rho, converted by vps2lm on the device (fd2d.py:21-33) */
+int sp_set_model_vps(sp_ctx *ctx, int grid, const float *vp, const float *vs, const float *rho, int on_device);
+
 /* kernels (SP_K_*) of a grid, [N] */
 int sp_get_kernel(sp_ctx *ctx, int grid, int which, float *dst, int dst_on_device);
 /* vp/vs/rho kernels from lam/mu/rho kernels (chain rule of vps2lm) */

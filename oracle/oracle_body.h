@@ -373,9 +373,12 @@ static void FN(interact)(FN(state) *s, const orc_model *m, const T *snap, double
         FN(div_vy)(s->dvydx, s->dvydz, s->uy, dx, dz, nx, nz);
         FN(div_vy)(s->ax, s->az, ufw, dx, dz, nx, nz);
 #pragma omp parallel for schedule(static)
-        for (k = 0; k < n; k++)
+        for (k = 0; k < n; k++) {
             kmu[k] = (T)((double)kmu[k]
                          + (double)(T)((T)(s->dvydx[k] * s->ax[k]) + (T)(s->dvydz[k] * s->az[k])) * ndt);
+            /* density: adjoint velocity times forward velocity, as P-SV below */
+            krho[k] = (T)((double)krho[k] + (double)s->vy[k] * (double)ufw[n + k] * ndt);
+        }
         return;
     }
     {

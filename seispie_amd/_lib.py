@@ -10,7 +10,7 @@ import numpy as np
 
 SH, PSV, SPIN = 1, 2, 3
 F_RHO, F_LAM, F_MU, F_NU, F_J, F_LAMC, F_MUC, F_NUC = range(8)
-K_LAM, K_MU, K_RHO, K_NU, K_J, K_MUC, K_VP, K_VS, K_RHOP = range(9)
+K_LAM, K_MU, K_RHO, K_NU, K_J, K_MUC, K_VP, K_VS, K_RHOP, K_HESS = range(10)
 OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP = 1, 2, 3, 4, 5, 6
 
 N_REC = {SH: 1, PSV: 2, SPIN: 3}   # record components: SH y; P-SV x, z; spin x, z, y
@@ -26,6 +26,27 @@ class SeispieError(RuntimeError):
     pass
 
 
+def _share_torch_runtime():
+    """One HIP runtime per process.  PyTorch-ROCm ships its own
+    libamdhip64 (same soname, libamdhip64.so.7, as /opt/rocm's), which its
+    libraries find by RUNPATH.  If this library were loaded first, its
+    dependency would resolve to /opt/rocm's copy and a later `import torch`
+    would map torch's copy as a second runtime, which then finds no GPU, and
+    device pointers and streams could not pass between the two.  Mapping
+    torch's copy first (without importing torch) makes this library's
+    dependency resolve to it by soname and torch's to it by file identity."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
 def load():
     """Load the HIP library (raises ImportError when it has not been built)."""
     global _lib
@@ -34,6 +55,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libseispie_hip.so is not built (%s); run __graft_entry__.build() or "
                           "python -m seispie_amd.build" % LIB_PATH)
+    _share_torch_runtime()
     L = ctypes.CDLL(LIB_PATH)
     vp, i, d, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
     sig = {
@@ -63,6 +85,10 @@ def load():
         "sp_smooth": ([vp, vp, i, i, i, i], i),
         "sp_last_timing": ([vp, ctypes.POINTER(ll), ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d)], i),
         "sp_autotune": ([vp, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "sp_forward_ckpt": ([vp, i], i),
+        "sp_adjoint_ckpt": ([vp, i], i),
+        "sp_set_adjoint_source": ([vp, i, vp, i], i),
+        "sp_set_model_vps": ([vp, i, vp, vp, vp, i], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -78,7 +104,8 @@ def exported_symbols():
             "sp_set_physics", "sp_add_grid", "sp_set_sponge", "sp_set_field", "sp_get_field",
             "sp_add_shot", "sp_clear_shots", "sp_num_components", "sp_forward", "sp_get_records",
             "sp_set_data", "sp_misfit", "sp_adjoint", "sp_gradient", "sp_get_kernel", "sp_kernels_vps",
-            "sp_smooth_kernel", "sp_smooth", "sp_last_timing", "sp_autotune"]
+            "sp_smooth_kernel", "sp_smooth", "sp_last_timing", "sp_autotune", "sp_forward_ckpt",
+            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps"]
 
 
 def _f32(a, n=None):
@@ -137,6 +164,8 @@ class Context:
 
     # ------------------------------------------------------------ setup
     def set_stream(self, stream_handle):
+        """Order the context's work on a hipStream_t (an int handle, e.g.
+        torch.cuda.current_stream().cuda_stream; 0 is the null stream)."""
         self._chk(self.L.sp_set_stream(self._p, ctypes.c_void_p(stream_handle)))
 
     def sync(self):
@@ -219,6 +248,31 @@ class Context:
         out = np.zeros(len(self.shots) * N_MIS[self.physics], dtype=np.float64)
         self._chk(self.L.sp_misfit(self._p, _ptr(out)))
         return out.reshape(len(self.shots), N_MIS[self.physics])
+
+    def forward_ckpt(self, interval):
+        """Forward run of every shot keeping a full state every `interval` steps."""
+        self._chk(self.L.sp_forward_ckpt(self._p, int(interval)))
+
+    def adjoint_ckpt(self, sae):
+        """Adjoint run recomputing the forward snapshots from the checkpoints."""
+        self._chk(self.L.sp_adjoint_ckpt(self._p, int(sae)))
+
+    def set_adjoint_source(self, shot, src):
+        """[n_mis, nrec, nt], time-reversed; numpy or a device tensor."""
+        nrec = self.shots[shot][2]
+        if _is_tensor(src):
+            self._chk(self.L.sp_set_adjoint_source(self._p, shot, _devptr(src), 1))
+        else:
+            a = _f32(src, N_MIS[self.physics] * nrec * self.nt)
+            self._chk(self.L.sp_set_adjoint_source(self._p, shot, _ptr(a), 0))
+
+    def set_model_vps(self, grid, vp, vs, rho):
+        if _is_tensor(vp):
+            self._chk(self.L.sp_set_model_vps(self._p, grid, _devptr(vp), _devptr(vs), _devptr(rho), 1))
+        else:
+            nx, nz = self.grids[grid]
+            a = [_f32(x, nx * nz) for x in (vp, vs, rho)]
+            self._chk(self.L.sp_set_model_vps(self._p, grid, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), 0))
 
     def adjoint(self, sae):
         self._chk(self.L.sp_adjoint(self._p, int(sae)))

@@ -1,0 +1,187 @@
+"""Differentiable forward modelling: torch.autograd over the HIP solver.
+
+    sim = Simulation(PSV, nx, nz, dx, dz, nt, dt, shots, sponge=(20, 0.015, 1, 1, 1, 0))
+    rec = sim(vp, vs, rho)            # [nshot, ncomp, nrec, nt] on the GPU
+    loss = some_function(rec)
+    loss.backward()                   # vp.grad, vs.grad, rho.grad
+
+The forward pass is the library's batched time loop over every shot
+(sp_forward_ckpt), which keeps the full wavefield state every `interval`
+steps.  The backward pass is the hand-written adjoint (sp_adjoint_ckpt): the
+adjoint sources are -dL/d(records), time-reversed, injected at the receivers
+as the reference injects its waveform adjoint sources (solver.py:290-321,
+misfit/waveform.py:10,25); the forward wavefield it correlates with is
+recomputed segment by segment from the checkpoints, last segment first, so
+device memory holds nt/interval states and interval/sae snapshots per shot
+instead of the nt/sae snapshots of the reference's stored-wavefield adjoint
+(solver.py:193-203).
+
+The accumulated kernels satisfy K = dt * dL/dm (lam, mu, rho), as the
+reference's k_mu does for its waveform misfit, so the gradient is K / dt,
+chained to vp, vs, rho through vps2lm (fd2d.py:21-33):
+    dL/dvp  = 2 rho vp K_lam
+    dL/dvs  = 2 rho vs (K_mu - 2 K_lam)
+    dL/drho = (vp^2 - 2 vs^2) K_lam + vs^2 K_mu + K_rho.
+Sampling the correlation every `sae` steps (the reference's
+adjoint_interval) approximates the time integral; sae = 1 is the default.
+Inside sponge layers the velocities are damped every step, so there the
+density kernel (adjoint velocity times forward velocity) is approximate;
+lam and mu are not affected.
+
+SH uses mu and rho only (its vp gradient is zero); P-SV uses all three.
+The micropolar-spin physics has further parameters and is driven through
+fd2d / the C ABI instead.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def default_interval(nt):
+    """Even checkpoint interval near sqrt(nt): memory ~ 2 sqrt(nt) fields."""
+    c = max(2, int(round(math.sqrt(nt))))
+    return c + (c & 1)
+
+
+class Simulation:
+    """A grid with a fixed set of shots whose records are differentiable in
+    the model.  shots: list of (src_k [nsrc], stf [n_inj, nsrc, nt], rec_k
+    [nrec]) with grid indices k = i*nz + j; every shot has the same nrec."""
+
+    def __init__(self, physics, nx, nz, dx, dz, nt, dt, shots, sponge=None, interval=None, sae=1, device=0,
+                 init_model=None):
+        if physics not in (_lib.SH, _lib.PSV):
+            raise ValueError("autograd covers SH and P-SV physics")
+        if nt % sae:
+            raise ValueError("nt must be a multiple of sae")
+        nrec = {len(np.asarray(s[2]).reshape(-1)) for s in shots}
+        if len(nrec) != 1:
+            raise ValueError("every shot needs the same number of receivers")
+        self.physics, self.nx, self.nz, self.nt, self.dt = physics, nx, nz, nt, float(dt)
+        self.sae = int(sae)
+        self.interval = int(interval or default_interval(nt))
+        if self.interval < 2 or self.interval % 2:
+            raise ValueError("interval must be even and >= 2")
+        self.nrec = nrec.pop()
+        self.device = torch.device("cuda", device)
+        self.ctx = _lib.Context(device)
+        self.ctx.set_physics(physics, nt, dt)
+        if init_model is None:
+            init_model = (np.full(nx * nz, 3000, "f4"), np.full(nx * nz, 1700, "f4"), np.full(nx * nz, 2500, "f4"))
+        self.grid = self.ctx.add_grid(nx, nz, dx, dz, init_model, 0)
+        if sponge is not None:
+            self.ctx.set_sponge(self.grid, *sponge)
+        self.nshot = len(shots)
+        for src_k, stf, rec_k in shots:
+            self.ctx.add_shot(self.grid, src_k, stf, rec_k)
+        self.ncomp = _lib.N_REC[physics]
+        self.nmis = _lib.N_MIS[physics]
+        self._gen = 0      # forward runs so far: backward re-runs the forward if another came between
+        self._last = None  # model of the last forward run
+
+    def __call__(self, vp, vs, rho):
+        return SeismicRecords.apply(vp, vs, rho, self)
+
+    # ------------------------------------------------------------ plumbing
+    def _bind_stream(self):
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _set_model(self, vp, vs, rho):
+        self._bind_stream()
+        flat = [t.detach().to(self.device, torch.float32).contiguous().reshape(-1) for t in (vp, vs, rho)]
+        for t in flat:
+            if t.numel() != self.nx * self.nz:
+                raise ValueError("model fields must have nx*nz = %d cells" % (self.nx * self.nz))
+        self.ctx.set_model_vps(self.grid, *flat)
+        return flat
+
+    def run_forward(self, vp, vs, rho):
+        flat = self._set_model(vp, vs, rho)
+        self.ctx.forward_ckpt(self.interval)
+        self._gen += 1
+        self._last = flat
+        out = torch.empty((self.nshot, self.ncomp, self.nrec, self.nt), device=self.device, dtype=torch.float32)
+        for i in range(self.nshot):
+            self.ctx.records(i, out[i])
+        return out, self._gen
+
+    def run_adjoint(self, grad_records):
+        """Kernels (lam, mu, rho) of L for dL/d(records) = grad_records."""
+        self._bind_stream()
+        ad = (-grad_records[:, :self.nmis]).flip(-1).to(torch.float32).contiguous()
+        for i in range(self.nshot):
+            self.ctx.set_adjoint_source(i, ad[i])
+        self.ctx.adjoint_ckpt(self.sae)
+        n = self.nx * self.nz
+        ks = [torch.empty(n, device=self.device, dtype=torch.float32) for _ in range(3)]
+        for t, which in zip(ks, (_lib.K_LAM, _lib.K_MU, _lib.K_RHO)):
+            self.ctx.kernel(self.grid, which, t)
+        return ks
+
+    def hessian(self):
+        """Pseudo-Hessian diagonal of the last backward pass: dt * sum over
+        shots and time of the forward particle-velocity energy, [nx*nz]."""
+        t = torch.empty(self.nx * self.nz, device=self.device, dtype=torch.float32)
+        self._bind_stream()
+        self.ctx.kernel(self.grid, _lib.K_HESS, t)
+        return t
+
+    def close(self):
+        self.ctx.close()
+
+
+class SeismicRecords(torch.autograd.Function):
+    """records = F(vp, vs, rho); backward = the hand-written adjoint."""
+
+    @staticmethod
+    def forward(ctx, vp, vs, rho, sim):
+        out, gen = sim.run_forward(vp, vs, rho)
+        ctx.sim = sim
+        ctx.gen = gen
+        ctx.shape = vp.shape
+        ctx.save_for_backward(*sim._last)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        sim = ctx.sim
+        vp, vs, rho = ctx.saved_tensors
+        if sim._gen != ctx.gen:
+            # another forward run replaced the checkpoints: rebuild them
+            sim._bind_stream()
+            sim.ctx.set_model_vps(sim.grid, vp, vs, rho)
+            sim.ctx.forward_ckpt(sim.interval)
+            sim._gen += 1
+            ctx.gen = sim._gen
+        kl, km, kr = sim.run_adjoint(grad.contiguous())
+        inv = 1.0 / sim.dt
+        vp64, vs64, rho64 = (t.double() for t in (vp, vs, rho))
+        kl, km, kr = (k.double() * inv for k in (kl, km, kr))
+        live = vp64 > vs64  # vps2lm sets lambda to 0 where vp <= vs
+        kl_eff = torch.where(live, kl, torch.zeros_like(kl))
+        gvp = 2.0 * rho64 * vp64 * kl_eff
+        gvs = 2.0 * rho64 * vs64 * (km - 2.0 * kl_eff)
+        grho = torch.where(live, vp64 * vp64 - 2.0 * vs64 * vs64, torch.zeros_like(vp64)) * kl + vs64 * vs64 * km + kr
+        shape = ctx.shape
+        return (gvp.float().reshape(shape), gvs.float().reshape(shape), grho.float().reshape(shape), None)
+
+
+def waveform_loss(records, data, dt):
+    """The reference's waveform misfit (misfit/waveform.py:5-34) in torch:
+    sum over shots, misfit components and receivers of the squared tapered
+    residual; records/data [nshot, ncomp, nrec, nt].  Its gradient drives
+    the same adjoint sources as sp_misfit."""
+    nt = records.shape[-1]
+    t = torch.arange(nt, device=records.device, dtype=torch.float64) * dt
+    t_end = (nt - 1) * dt
+    tw = t_end / 10.0
+    t_min, t_max = tw, t_end - tw
+    taper = torch.ones_like(t)
+    lo, hi = t <= t_min, t >= t_max
+    taper[lo] = 0.5 + 0.5 * torch.cos(math.pi * (t_min - t[lo]) / tw)
+    taper[hi] = 0.5 + 0.5 * torch.cos(math.pi * (t_max - t[hi]) / tw)
+    res = (data - records).double() * taper
+    return (res * res).sum()

@@ -788,6 +788,7 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
     float *K = a.kern;
     if constexpr (P == SP_SH) {
         float km = K[SP_K_MU * NP + k];
+        double kh = K[SP_K_HESS * NP + k], kr = K[SP_K_RHO * NP + k];
         for (int s = 0; s < a.nlist; s++) {
             const ShotDesc &d = a.shots[a.list[s]];
             const float *ua = d.state + slot_u(P, 0) * NP;
@@ -795,11 +796,17 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
             const float ax = dpx(ua, i, k, nx, nz, a.c1x, a.c2x), fx = dpx(uf, i, k, nx, nz, a.c1x, a.c2x);
             const float az = dpz(ua, j, k, nz, a.c1z, a.c2z), fz = dpz(uf, j, k, nz, a.c1z, a.c2z);
             km = (float)((double)km + (double)add_rn(mul_rn(ax, fx), mul_rn(az, fz)) * a.ndt);
+            const double vf = uf[NP + k], va = d.state[slot_v(P, a.cur, 0) * NP + k];
+            kr = (double)(float)(kr + va * vf * a.ndt);
+            kh = (double)(float)(kh + vf * vf * a.ndt);
         }
         K[SP_K_MU * NP + k] = km;
+        K[SP_K_RHO * NP + k] = (float)kr;
+        K[SP_K_HESS * NP + k] = (float)kh;
         return;
     }
     double kl = K[SP_K_LAM * NP + k], kmu = K[SP_K_MU * NP + k], kr = K[SP_K_RHO * NP + k];
+    double kh = K[SP_K_HESS * NP + k];
     double knu = 0, kj = 0, kc = 0;
     if constexpr (P == SP_SPIN) {
         knu = K[SP_K_NU * NP + k];
@@ -822,6 +829,7 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
         kl = (double)(float)(kl + diva * divf * a.ndt);
         kmu = (double)(float)(kmu + (2.0 * axx * fxx + 2.0 * azz * fzz + sha * shf) * a.ndt);
         kr = (double)(float)(kr + (vxa * vxf + vza * vzf) * a.ndt);
+        kh = (double)(float)(kh + (vxf * vxf + vzf * vzf) * a.ndt);
         if constexpr (P == SP_SPIN) {
             const float *uca = d.state + slot_u(P, 2) * NP, *ucf = sn + 2 * NP;
             const double ea = -(double)uca[k] - 0.5 * (azx - axz);
@@ -837,6 +845,7 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
     K[SP_K_LAM * NP + k] = (float)kl;
     K[SP_K_MU * NP + k] = (float)kmu;
     K[SP_K_RHO * NP + k] = (float)kr;
+    K[SP_K_HESS * NP + k] = (float)kh;
     if constexpr (P == SP_SPIN) {
         K[SP_K_NU * NP + k] = (float)knu;
         K[SP_K_J * NP + k] = (float)kj;
@@ -972,6 +981,9 @@ struct Shot {
     int *d_idx = nullptr;  // src_k, src_row, rec_k, rec_row
     float *stf = nullptr, *state = nullptr, *rec = nullptr, *data = nullptr, *adstf = nullptr;
     float *snap = nullptr;  // into the context's snapshot pool
+    float *state2 = nullptr;  // second state block: forward recomputation from checkpoints
+    float *ckpt = nullptr;    // [nckpt][n_s + 2 n_v][Npad] full forward states
+    size_t ckpt_bytes = 0;
 };
 
 }  // namespace
@@ -999,6 +1011,11 @@ struct sp_ctx {
     // batch descriptors
     ShotDesc *d_desc = nullptr;
     size_t d_desc_cap = 0;
+    ShotDesc *d_desc2 = nullptr;  // the same shots on their second state block
+    size_t d_desc2_cap = 0;
+    float *seg_pool = nullptr;    // snapshots of one recomputed segment
+    size_t seg_pool_bytes = 0;
+    int ckpt_C = 0, ckpt_valid = 0;
     int4 *d_items = nullptr;
     size_t d_items_cap = 0;
     int *d_list = nullptr;
@@ -1218,10 +1235,10 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     return 0;
 }
 
-int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint)
+int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, const ShotDesc *desc = nullptr)
 {
     StepArgs a;
-    a.shots = c->d_desc;
+    a.shots = desc ? desc : c->d_desc;
     a.items = c->d_items;
     a.nitems = nitems;
     a.it = it;
@@ -1365,56 +1382,209 @@ int run_misfit_batch(sp_ctx *c, const std::vector<int> &batch, double *norms_out
     return 0;
 }
 
+// interaction launches of one adjoint step for the shots of a batch,
+// grouped by grid (lists prepared by interact_lists)
+struct InteractPlan {
+    std::vector<std::vector<int>> per_grid;
+    std::vector<int> off;
+};
+
+int interact_lists(sp_ctx *c, const std::vector<int> &batch, InteractPlan &pl)
+{
+    pl.per_grid.assign(c->grids.size(), {});
+    for (int b = 0; b < (int)batch.size(); b++) pl.per_grid[c->shots[batch[b]].grid].push_back(b);
+    std::vector<int> flat;
+    pl.off.assign(c->grids.size() + 1, 0);
+    for (size_t g = 0; g < pl.per_grid.size(); g++) {
+        pl.off[g] = (int)flat.size();
+        flat.insert(flat.end(), pl.per_grid[g].begin(), pl.per_grid[g].end());
+    }
+    pl.off[pl.per_grid.size()] = (int)flat.size();
+    if (ensure(c, c->d_list, c->d_list_cap, std::max<size_t>(flat.size(), 1))) return 2;
+    HIPCHECK(c, hipMemcpyAsync(c->d_list, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+int launch_interact(sp_ctx *c, const InteractPlan &pl, int cur, int isae, double ndt)
+{
+    for (size_t g = 0; g < pl.per_grid.size(); g++) {
+        if (pl.per_grid[g].empty()) continue;
+        const Grid &G = c->grids[g];
+        InteractArgs a;
+        a.shots = c->d_desc;
+        a.list = c->d_list + pl.off[g];
+        a.nlist = (int)pl.per_grid[g].size();
+        a.nx = G.nx;
+        a.nz = G.nz;
+        a.Npad = G.Npad;
+        a.kern = G.kern;
+        a.cur = cur;
+        a.isae = isae;
+        a.ndt = ndt;
+        a.c1x = 9.0 / (8.0 * (double)G.dx);
+        a.c2x = 1.0 / (24.0 * (double)G.dx);
+        a.c1z = 9.0 / (8.0 * (double)G.dz);
+        a.c2z = 1.0 / (24.0 * (double)G.dz);
+        dim3 gr((unsigned)((G.N + 255) / 256)), bl(256);
+        switch (c->physics) {
+        case SP_SH: hipLaunchKernelGGL(k_interact<SP_SH>, gr, bl, 0, c->stream, a); break;
+        case SP_PSV: hipLaunchKernelGGL(k_interact<SP_PSV>, gr, bl, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_interact<SP_SPIN>, gr, bl, 0, c->stream, a); break;
+        }
+        HIPCHECK(c, hipGetLastError());
+    }
+    return 0;
+}
+
 int run_adjoint_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
 {
     int nitems = 0;
     if (prepare_batch(c, batch, &nitems)) return 2;
     if (clear_state(c, batch)) return 2;
-    // shots of the batch grouped by grid for the interaction
-    std::vector<std::vector<int>> per_grid(c->grids.size());
-    for (int b = 0; b < (int)batch.size(); b++) per_grid[c->shots[batch[b]].grid].push_back(b);
-    std::vector<int> flat, off(c->grids.size() + 1, 0);
-    for (size_t g = 0; g < per_grid.size(); g++) {
-        off[g] = (int)flat.size();
-        flat.insert(flat.end(), per_grid[g].begin(), per_grid[g].end());
-    }
-    off[per_grid.size()] = (int)flat.size();
-    if (ensure(c, c->d_list, c->d_list_cap, std::max<size_t>(flat.size(), 1))) return 2;
-    HIPCHECK(c, hipMemcpyAsync(c->d_list, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    InteractPlan pl;
+    if (interact_lists(c, batch, pl)) return 2;
     const double ndt = (double)sae * c->dt;
     for (int it = 0; it < c->nt; it++) {
         if (launch_step(c, nitems, it, -1, 1)) return 2;
-        if ((it + sae) % sae == 0) {
-            const int isae = (it + sae) / sae - 1;
-            for (size_t g = 0; g < per_grid.size(); g++) {
-                if (per_grid[g].empty()) continue;
-                const Grid &G = c->grids[g];
-                InteractArgs a;
-                a.shots = c->d_desc;
-                a.list = c->d_list + off[g];
-                a.nlist = (int)per_grid[g].size();
-                a.nx = G.nx;
-                a.nz = G.nz;
-                a.Npad = G.Npad;
-                a.kern = G.kern;
-                a.cur = (it & 1) ^ 1;
-                a.isae = isae;
-                a.ndt = ndt;
-                a.c1x = 9.0 / (8.0 * (double)G.dx);
-                a.c2x = 1.0 / (24.0 * (double)G.dx);
-                a.c1z = 9.0 / (8.0 * (double)G.dz);
-                a.c2z = 1.0 / (24.0 * (double)G.dz);
-                dim3 gr((unsigned)((G.N + 255) / 256)), bl(256);
-                switch (c->physics) {
-                case SP_SH: hipLaunchKernelGGL(k_interact<SP_SH>, gr, bl, 0, c->stream, a); break;
-                case SP_PSV: hipLaunchKernelGGL(k_interact<SP_PSV>, gr, bl, 0, c->stream, a); break;
-                default: hipLaunchKernelGGL(k_interact<SP_SPIN>, gr, bl, 0, c->stream, a); break;
-                }
-                HIPCHECK(c, hipGetLastError());
-            }
+        if ((it + sae) % sae == 0)
+            if (launch_interact(c, pl, (it & 1) ^ 1, (it + sae) / sae - 1, ndt)) return 2;
+    }
+    c->t_bytes += step_alg_bytes(c, batch) * c->nt;
+    return 0;
+}
+
+// ---- checkpointed forward and adjoint ----------------------------------
+// A checkpoint is the full state before step k*C (C even, so the read set
+// is set 0): the set-0 stresses, the set-0 velocities and the displacement.
+int ckpt_fields(int P) { return n_s(P) + 2 * n_v(P); }
+
+int ckpt_copy(sp_ctx *c, Shot &S, int k, bool save)
+{
+    const int P = c->physics;
+    const Grid &G = c->grids[S.grid];
+    const size_t fb = (size_t)G.Npad * sizeof(float);
+    float *ck = S.ckpt + (size_t)k * ckpt_fields(P) * G.Npad;
+    float *st = save ? S.state : S.state2;
+    const size_t parts[3][3] = {{0, (size_t)slot_s(P, 0, 0), (size_t)n_s(P)},
+                                {(size_t)n_s(P), (size_t)slot_v(P, 0, 0), (size_t)n_v(P)},
+                                {(size_t)(n_s(P) + n_v(P)), (size_t)slot_u(P, 0), (size_t)n_v(P)}};
+    for (auto &q : parts) {
+        float *a = ck + q[0] * G.Npad, *b = st + q[1] * G.Npad;
+        HIPCHECK(c, hipMemcpyAsync(save ? a : b, save ? b : a, q[2] * fb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return 0;
+}
+
+// forward run of a batch that keeps checkpoints every C steps (records too)
+int run_forward_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
+{
+    const int P = c->physics;
+    const int K = (c->nt + C - 1) / C;
+    for (int s : batch) {
+        Shot &S = c->shots[s];
+        const size_t need = (size_t)K * ckpt_fields(P) * c->grids[S.grid].Npad * sizeof(float);
+        if (S.ckpt_bytes != need) {
+            if (S.ckpt) HIPCHECK(c, hipFree(S.ckpt));
+            S.ckpt = nullptr;
+            S.ckpt_bytes = 0;
+            HIPCHECK(c, hipMalloc(&S.ckpt, need));
+            S.ckpt_bytes = need;
+        }
+    }
+    int nitems = 0;
+    if (prepare_batch(c, batch, &nitems)) return 2;
+    if (clear_state(c, batch)) return 2;
+    const int nb = (int)batch.size();
+    int maxrec = 1;
+    for (int s : batch) maxrec = std::max(maxrec, c->shots[s].nrec);
+    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
+    for (int it = 0; it < c->nt; it++) {
+        if (it % C == 0)
+            for (int s : batch)
+                if (ckpt_copy(c, c->shots[s], it / C, true)) return 2;
+        if (launch_step(c, nitems, it, -1, 0)) return 2;
+        if (c->cur_kernel == 1) {
+            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics);
+            HIPCHECK(c, hipGetLastError());
         }
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
+    return 0;
+}
+
+// adjoint run of a batch; the forward snapshots it needs are recomputed
+// segment by segment (last segment first) from the checkpoints, on each
+// shot's second state block.
+int run_adjoint_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int sae, int C)
+{
+    const int P = c->physics, nt = c->nt, nsa = nt / sae;
+    const int cap = C / sae + 2;  // snapshots per segment, at most
+    size_t need = 0;
+    for (int s : batch) need += (size_t)cap * n_snap(P) * c->grids[c->shots[s].grid].Npad * sizeof(float);
+    if (need > c->seg_pool_bytes) {
+        if (c->seg_pool) HIPCHECK(c, hipFree(c->seg_pool));
+        c->seg_pool = nullptr;
+        c->seg_pool_bytes = 0;
+        HIPCHECK(c, hipMalloc(&c->seg_pool, need));
+        c->seg_pool_bytes = need;
+    }
+    size_t off = 0;
+    for (auto &sh : c->shots) sh.snap = nullptr;
+    for (int s : batch) {
+        Shot &S = c->shots[s];
+        S.snap = c->seg_pool + off / sizeof(float);
+        off += (size_t)cap * n_snap(P) * c->grids[S.grid].Npad * sizeof(float);
+        if (!S.state2) {
+            const long long nf = (long long)n_slots(P) * c->grids[S.grid].Npad;
+            HIPCHECK(c, hipMalloc(&S.state2, (size_t)nf * sizeof(float)));
+            if (fill(c, S.state2, nf, 0.f)) return 2;
+        }
+    }
+    int nitems = 0;
+    if (prepare_batch(c, batch, &nitems)) return 2;
+    // descriptors of the same shots on their second state block
+    std::vector<ShotDesc> d2;
+    for (int s : batch) {
+        ShotDesc d = make_desc(c, c->shots[s]);
+        d.state = c->shots[s].state2;
+        d2.push_back(d);
+    }
+    if (ensure(c, c->d_desc2, c->d_desc2_cap, d2.size())) return 2;
+    HIPCHECK(c, hipMemcpyAsync(c->d_desc2, d2.data(), d2.size() * sizeof(ShotDesc), hipMemcpyHostToDevice, c->stream));
+    if (clear_state(c, batch)) return 2;
+    InteractPlan pl;
+    if (interact_lists(c, batch, pl)) return 2;
+    const int nb = (int)batch.size();
+    long long maxN = 1;
+    for (int s : batch) maxN = std::max(maxN, c->grids[c->shots[s].grid].N);
+    const dim3 sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
+    const double ndt = (double)sae * c->dt;
+    int seg_loaded = -1, isa_base = 0;
+    for (int a = 0; a < nt; a++) {
+        if (launch_step(c, nitems, a, -1, 1)) return 2;
+        if (a % sae) continue;
+        const int isae = a / sae, seg = (nt - a - 1) / C;
+        if (seg != seg_loaded) {
+            const int lo = seg * C, hi = std::min(nt, lo + C);
+            isa_base = nsa - hi / sae;
+            for (int s : batch)
+                if (ckpt_copy(c, c->shots[s], seg, false)) return 2;
+            for (int it = lo; it < hi; it++) {
+                const int isa = (it + 1) % sae == 0 ? nsa - (it + 1) / sae : -1;
+                if (isa >= 0)
+                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa - isa_base, 0, 0,
+                                       c->physics);
+                if (launch_step(c, nitems, it, -1, 0, c->d_desc2)) return 2;
+                if (isa >= 0)
+                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa - isa_base, 1,
+                                       (it & 1) ^ 1, c->physics);
+            }
+            HIPCHECK(c, hipGetLastError());
+            seg_loaded = seg;
+        }
+        if (launch_interact(c, pl, (a & 1) ^ 1, isae - isa_base, ndt)) return 2;
+    }
+    c->t_bytes += step_alg_bytes(c, batch) * nt;
     return 0;
 }
 
@@ -1554,6 +1724,8 @@ int sp_create(int device, sp_ctx **out)
 
 static void free_shot(Shot &s)
 {
+    if (s.state2) (void)hipFree(s.state2);
+    if (s.ckpt) (void)hipFree(s.ckpt);
     if (s.d_idx) (void)hipFree(s.d_idx);
     if (s.stf) (void)hipFree(s.stf);
     if (s.state) (void)hipFree(s.state);
@@ -1577,6 +1749,8 @@ int sp_destroy(sp_ctx *c)
     }
     if (c->snap_pool) (void)hipFree(c->snap_pool);
     if (c->d_desc) (void)hipFree(c->d_desc);
+    if (c->d_desc2) (void)hipFree(c->d_desc2);
+    if (c->seg_pool) (void)hipFree(c->seg_pool);
     if (c->d_items) (void)hipFree(c->d_items);
     if (c->d_list) (void)hipFree(c->d_list);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1592,7 +1766,7 @@ const char *sp_last_error(const sp_ctx *c) { return c ? c->err.c_str() : "null c
 int sp_set_stream(sp_ctx *c, void *stream)
 {
     if (!c) return 3;
-    c->stream = stream ? (hipStream_t)stream : c->own;
+    c->stream = (hipStream_t)stream;  // NULL is the device's null stream, as in HIP
     return 0;
 }
 
@@ -1730,6 +1904,7 @@ int sp_set_field(sp_ctx *c, int grid, int field, const float *src, int on_dev)
         HIPCHECK(c, hipGetLastError());
     }
     c->snaps_valid = 0;
+    c->ckpt_valid = 0;
     return 0;
 }
 
@@ -1801,6 +1976,7 @@ int sp_clear_shots(sp_ctx *c)
     for (auto &s : c->shots) free_shot(s);
     c->shots.clear();
     c->snaps_valid = 0;
+    c->ckpt_valid = 0;
     return 0;
 }
 
@@ -1899,6 +2075,64 @@ int sp_gradient(sp_ctx *c, int sae, double *norms_out)
         c->snaps_valid = 1;
         c->snaps_sae = sae;
     }
+    return 0;
+}
+
+int sp_forward_ckpt(sp_ctx *c, int C)
+{
+    if (int e = check_ctx(c)) return e;
+    if (c->shots.empty()) return fail(c, "no shots");
+    if (C < 2 || C % 2) return fail(c, "checkpoint interval must be even and >= 2");
+    std::vector<int> all(c->shots.size());
+    std::iota(all.begin(), all.end(), 0);
+    c->ckpt_valid = 0;
+    c->snaps_valid = 0;
+    if (timing_begin(c)) return 2;
+    if (run_forward_ckpt_batch(c, all, C)) return 2;
+    if (timing_end(c)) return 2;
+    c->ckpt_C = C;
+    c->ckpt_valid = 1;
+    return 0;
+}
+
+int sp_set_adjoint_source(sp_ctx *c, int shot, const float *src, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (shot < 0 || shot >= (int)c->shots.size()) return fail(c, "bad shot id");
+    Shot &s = c->shots[shot];
+    return dcopy(c, s.adstf, src, (size_t)adj_floats(c, s) * sizeof(float), 1, on_dev);
+}
+
+int sp_adjoint_ckpt(sp_ctx *c, int sae)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!c->ckpt_valid) return fail(c, "no checkpoints: run sp_forward_ckpt first");
+    if (sae <= 0 || c->nt % sae) return fail(c, "nt must be a positive multiple of adjoint_interval");
+    for (auto &g : c->grids)
+        if (fill(c, g.kern, (long long)SP_NKERN * g.Npad, 0.f)) return 2;
+    std::vector<int> all(c->shots.size());
+    std::iota(all.begin(), all.end(), 0);
+    if (timing_begin(c)) return 2;
+    if (run_adjoint_ckpt_batch(c, all, sae, c->ckpt_C)) return 2;
+    c->snaps_valid = 0;
+    return timing_end(c);
+}
+
+int sp_set_model_vps(sp_ctx *c, int grid, const float *vp, const float *vs, const float *rho, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
+    Grid &G = c->grids[grid];
+    float *m = G.mat;
+    const size_t nb = (size_t)G.N * sizeof(float);
+    if (dcopy(c, m + (size_t)M_LAM * G.Npad, vp, nb, 1, on_dev) || dcopy(c, m + (size_t)M_MU * G.Npad, vs, nb, 1, on_dev) ||
+        dcopy(c, m + (size_t)M_RHO * G.Npad, rho, nb, 1, on_dev))
+        return 2;
+    hipLaunchKernelGGL(k_vps2lm, dim3((unsigned)((G.N + 255) / 256)), dim3(256), 0, c->stream,
+                       m + (size_t)M_LAM * G.Npad, m + (size_t)M_MU * G.Npad, m + (size_t)M_RHO * G.Npad, G.N);
+    HIPCHECK(c, hipGetLastError());
+    c->snaps_valid = 0;
+    c->ckpt_valid = 0;
     return 0;
 }
 

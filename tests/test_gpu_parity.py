@@ -158,7 +158,7 @@ def test_misfit_and_gradient_match_oracle(ctx_factory, physics):
     # by the seismogram tolerance times the records' norm
     dnorm = np.array([[np.linalg.norm(d[c]) for c in range(len(MIS_NAMES[physics]))] for d in data])
     assert np.all(np.abs(norms - nref) <= SEIS_TOL * dnorm), (norms, nref)
-    names = {_lib.SH: [("mu", _lib.K_MU)],
+    names = {_lib.SH: [("mu", _lib.K_MU), ("rho", _lib.K_RHO)],
              _lib.PSV: [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO)],
              _lib.SPIN: [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO), ("nu", _lib.K_NU),
                          ("j", _lib.K_J), ("muc", _lib.K_MUC)]}[physics]

@@ -257,7 +257,7 @@ def _fd_problem(physics, nx=100, nz=90, nt=400, dt=5e-4, h=5.0):
     return m, bound, sid, rid, stf, data, sr
 
 
-@pytest.mark.parametrize("physics,keys", [(O.SH, ["mu"]), (O.PSV, ["lam", "mu", "rho"]),
+@pytest.mark.parametrize("physics,keys", [(O.SH, ["mu", "rho"]), (O.PSV, ["lam", "mu", "rho"]),
                                           (O.SPIN, ["lam", "mu", "rho", "nu", "j", "muc"])])
 def test_adjoint_kernels_match_finite_differences(physics, keys):
     """k ~= dt * dchi/dm with chi = sum_t,r w^2 (u - d)^2 (sae = 1, edges
