@@ -10,11 +10,14 @@ cell-updates / wall time (forward and adjoint both counted, nothing
 recomputed).
 
 The JSON line adds
-  roofline:     the dominant kernel (the fused step kernel) against the
-                8 TB/s HBM peak: algorithmic bytes per launch (8 B per state
-                field per shot cell + 4 B per material field per grid cell)
-                over its mean launch time from HIP events on the library's
-                stream, in a forward pass timed on its own;
+  roofline:     the dominant kernel (the step kernel; for SH and P-SV it
+                advances two time steps per launch) against the 8 TB/s HBM
+                peak: algorithmic bytes per launch (per time step, 8 B per
+                state field per shot cell + 4 B per material field per grid
+                cell, times the steps per launch) over its mean launch time
+                from HIP events on the library's stream, in a forward pass
+                timed on its own; `traffic` is the measured HBM bytes per
+                launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/);
   cpu_baseline: the oracle's unfused reference kernel sequence (one shot,
                 a fixed window of steps on the same grid) on this host's cores.
 """
@@ -102,6 +105,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="override the number of time steps")
     ap.add_argument("--forward-only", action="store_true",
                     help="time forward passes (the step kernel) instead of gradients; for counter runs")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="one time step per launch (the single-step kernel) instead of two where allowed")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB elementwise kernel whose HBM bytes are known, to scale "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -132,6 +137,8 @@ def main():
     print("# observed data for %s in %.1fs" % (true.name, time.time() - t0), file=sys.stderr, flush=True)
 
     ctx = _lib.Context(0)
+    if args.no_fuse:
+        ctx.set_option(_lib.OPT_FUSE, 0)
     if args.chunk:
         ctx.set_option(_lib.OPT_CHUNK, args.chunk)
     ctx.set_option(_lib.OPT_ORDER, args.order)
@@ -148,8 +155,9 @@ def main():
             ctx.forward(0)
         ctx.sync()
         t = ctx.timing()
-        print(json.dumps({"forward_step_ms": t["total_ms"] / t["step_launches"],
+        print(json.dumps({"forward_launch_ms": t["total_ms"] / t["step_launches"],
                           "alg_bytes_per_launch": t["alg_bytes"] / t["step_launches"],
+                          "steps_per_launch": round(t["alg_bytes"] / t["step_launches"] / init.alg_bytes_per_step(), 3),
                           "workload": init.name}))
         return 0
     if not args.chunk:
@@ -176,13 +184,16 @@ def main():
     launch_ms = ft["total_ms"] / ft["step_launches"]
     bytes_per_launch = ft["alg_bytes"] / ft["step_launches"]
     achieved = bytes_per_launch / (launch_ms * 1e-3)
+    steps_per_launch = bytes_per_launch / init.alg_bytes_per_step()
+    kname = "k_step2<%s> (two time steps per launch)" if steps_per_launch > 1.5 else "k_step3<%s>"
+    kname = kname % {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]
 
     cells = init.cell_updates(passes=2)
     value = cells * args.steps / wall / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tf):
-        t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0])
+        t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0] + ("_fused" if steps_per_launch > 1.5 else ""))
         if t:
             traffic = t["hbm_bytes_per_launch"]
 
@@ -215,8 +226,8 @@ def main():
                                                             / wall / HBM_PEAK, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel": "k_step<P-SV>", "launch_ms": round(launch_ms, 4),
-                     "alg_bytes_per_launch": bytes_per_launch},
+                     "kernel": kname, "launch_ms": round(launch_ms, 4),
+                     "alg_bytes_per_launch": bytes_per_launch, "steps_per_launch": round(steps_per_launch, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -64,6 +64,8 @@ typedef struct sp_ctx sp_ctx;
 #define SP_OPT_SNAP_MB 4    /* snapshot memory budget in MiB (0 = auto) */
 #define SP_OPT_KERNEL 5     /* step kernel: 1 buffer-addressed unrolled march (default), 0 plain */
 #define SP_OPT_STRIP 6      /* strip geometry (shift << 16) | width: output lanes [shift, shift + width) */
+#define SP_OPT_FUSE 7       /* 1 (default): SH and P-SV advance two steps per launch wherever no snapshot,
+                               kernel interaction or checkpoint needs the state between them; 0: one step */
 
 /* Context and device.  Replaces fd2d.setup (fd2d.py:111-120). */
 int sp_create(int device, sp_ctx **out);

@@ -11,7 +11,7 @@ import numpy as np
 SH, PSV, SPIN = 1, 2, 3
 F_RHO, F_LAM, F_MU, F_NU, F_J, F_LAMC, F_MUC, F_NUC = range(8)
 K_LAM, K_MU, K_RHO, K_NU, K_J, K_MUC, K_VP, K_VS, K_RHOP, K_HESS = range(10)
-OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP = 1, 2, 3, 4, 5, 6
+OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP, OPT_FUSE = 1, 2, 3, 4, 5, 6, 7
 
 N_REC = {SH: 1, PSV: 2, SPIN: 3}   # record components: SH y; P-SV x, z; spin x, z, y
 N_INJ = {SH: 1, PSV: 2, SPIN: 2}   # source components: SH y; P-SV/spin x, z
@@ -34,8 +34,12 @@ def _share_torch_runtime():
     would map torch's copy as a second runtime, which then finds no GPU, and
     device pointers and streams could not pass between the two.  Mapping
     torch's copy first (without importing torch) makes this library's
-    dependency resolve to it by soname and torch's to it by file identity."""
+    dependency resolve to it by soname and torch's to it by file identity.
+    SEISPIE_HIP_SYSTEM_RUNTIME=1 keeps /opt/rocm's runtime (for processes
+    that never import torch)."""
     import importlib.util
+    if os.environ.get("SEISPIE_HIP_SYSTEM_RUNTIME") == "1":
+        return
     try:
         spec = importlib.util.find_spec("torch")
     except (ImportError, ValueError):

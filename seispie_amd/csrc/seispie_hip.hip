@@ -81,8 +81,8 @@ struct ShotDesc {
 
 struct StepArgs {
     const ShotDesc *shots;
-    const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-1) nz,
-                        // the first receiver with k >= (i0-1) nz and the first with k >= i0 nz
+    const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-4) nz,
+                        // the first receiver with k >= (i0-4) nz and the first with k >= i0 nz
     int nitems;
     int it, nt;
     double dt;
@@ -547,6 +547,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
     const long long nrows = a.adjoint ? d.nrec : d.nsrc;
     int pi = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
+    while (pi < ni && ik[pi] < (i0 - 1) * nz) ++pi;  // the list pointer starts at row i0-4
     int src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
 
     float S[4][NS], W[4][NV];
@@ -707,6 +708,345 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
                     bst(RS, oq, slot_s(P, wr, 3) * NP4, (float)((double)Sa[3] + dtg * (double)dcdx));
                     bst(RS, oq, slot_s(P, wr, 5) * NP4, (float)((double)Sa[5] + dtg * (double)dcdz));
                 }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------- two steps per launch
+// k_step2 advances a strip by two time steps (it and it+1) in one row march,
+// keeping the intermediate state in registers: the state is read once and
+// written once per two steps.  The intermediate stresses and velocities of a
+// chunk's halo rows are recomputed by each chunk that needs them; the strip
+// keeps 6 lanes of halo on each side (lanes [6, 58) are output), and the
+// chunk reads 6 rows before and 5 after its own.  Every value is computed
+// with the same operations, in the same order, as two launches of k_step3,
+// so results are identical bit for bit.  The result goes to the other
+// stress/velocity set (rd ^ 1), like one step, so no wavefront overwrites
+// rows another one still reads.
+//
+// Iteration m of the march (rows relative to m):
+//   A  v(n+1)  at row m-1 from s(n) rows m-3..m   (source at step it)
+//   B  s(n+1)  at row m-3 from v(n+1) rows m-4..m-1
+//   C  v(n+2)  at row m-4 from s(n+1) rows m-6..m-3 (source at step it+1);
+//      u(n+1), u(n+2) and the receiver records of both steps at row m-4
+//   D  s(n+2)  at row m-6 from v(n+2) rows m-7..m-4
+// Windows of four rows are indexed by row & 3 (loop unrolled by four).
+
+template <int P>
+struct Row2 {
+    float s[n_s(P)];     // s(n), row m
+    float v[n_v(P)];     // v(n), row m-1
+    float u[n_v(P)];     // u(n), row m-4
+    float rhoA, bxA;     // row m-1
+    float rhoC, bxC;     // row m-4
+    float lamB, muB;     // row m-3
+    float lamD, muD;     // row m-6
+};
+
+// The rows are wave-uniform, so each load's row offset goes into the scalar
+// offset and the per-lane offset vj = 4 jc is the same for every load (no
+// vector address arithmetic per row).  Every row is clamped into the grid,
+// so every load stays inside its buffer.
+template <int P>
+__device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p bxp, int NP4, int rd, int mp, int nx,
+                                      int nz, unsigned vj, int i0, int i1)
+{
+    constexpr int NS = n_s(P), NV = n_v(P);
+    // rows clamped to what the chunk needs (and to the grid)
+    const int ms = min(max(mp, max(i0 - 6, 0)), min(i1 + 5, nx - 1));
+    const int ma = min(max(mp - 1, max(i0 - 4, 0)), min(i1 + 4, nx - 1));
+    const int mb = min(max(mp - 3, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
+    const int mc = min(max(mp - 4, max(i0 - 1, 0)), min(i1 + 1, nx - 1));
+    const int mu_ = min(max(mp - 4, i0), i1 - 1);
+    const int md = min(max(mp - 6, i0), i1 - 1);
+    const int rb = nz * 4;
+    const int os = ms * rb, oa = ma * rb, ob = mb * rb, oc = mc * rb, ou = mu_ * rb, od = md * rb;
+#pragma unroll
+    for (int f = 0; f < NS; f++) p.s[f] = bld(RS, vj, slot_s(P, rd, f) * NP4 + os);
+#pragma unroll
+    for (int f = 0; f < NV; f++) {
+        p.v[f] = bld(RS, vj, slot_v(P, rd, f) * NP4 + oa);
+        p.u[f] = bld(RS, vj, slot_u(P, f) * NP4 + ou);
+    }
+    p.rhoA = bld(RM, vj, M_RHO * NP4 + oa);
+    p.rhoC = bld(RM, vj, M_RHO * NP4 + oc);
+    p.bxA = bxp[ma];
+    p.bxC = bxp[mc];
+    p.muB = bld(RM, vj, M_MU * NP4 + ob);
+    p.muD = bld(RM, vj, M_MU * NP4 + od);
+    p.lamB = p.lamD = 0.f;
+    if constexpr (P != SP_SH) {
+        p.lamB = bld(RM, vj, M_LAM * NP4 + ob);
+        p.lamD = bld(RM, vj, M_LAM * NP4 + od);
+    }
+}
+
+// stress divergence of one velocity row (div_sy / div_sxz, solver.py:54-66)
+// from the stress window rows r-2..r+1 (sa..sd)
+template <int P>
+__device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[n_s(P)], const float (&sb)[n_s(P)],
+                                        const float (&sc)[n_s(P)], const float (&sd)[n_s(P)], bool xm_ok, bool zm_ok,
+                                        int L, double c1x, double c2x, double c1z, double c2z)
+{
+    if constexpr (P == SP_SH) {
+        float zm2 = shfl(sc[1], L - 2), zm1 = shfl(sc[1], L - 1), zp1 = shfl(sc[1], L + 1);
+        double ddx = xm_ok ? dminus(sa[0], sb[0], sc[0], sd[0], c1x, c2x) : 0.0;
+        double ddz = zm_ok ? dminus(zm2, zm1, sc[1], zp1, c1z, c2z) : 0.0;
+        dsv[0] = (float)(ddx + ddz);
+    } else {
+        float xzm2 = shfl(sc[2], L - 2), xzm1 = shfl(sc[2], L - 1), xzp1 = shfl(sc[2], L + 1);
+        float zzm2 = shfl(sc[1], L - 2), zzm1 = shfl(sc[1], L - 1), zzp1 = shfl(sc[1], L + 1);
+        double dxx = xm_ok ? dminus(sa[0], sb[0], sc[0], sd[0], c1x, c2x) : 0.0;
+        double dxxz = xm_ok ? dminus(sa[2], sb[2], sc[2], sd[2], c1x, c2x) : 0.0;
+        double dzxz = zm_ok ? dminus(xzm2, xzm1, sc[2], xzp1, c1z, c2z) : 0.0;
+        double dzzz = zm_ok ? dminus(zzm2, zzm1, sc[1], zzp1, c1z, c2z) : 0.0;
+        dsv[0] = (float)(dxx + dzxz);
+        dsv[1] = (float)(dxxz + dzzz);
+    }
+}
+
+// velocity update (add_vy / add_vxz, solver.py:96-117) from the divergence
+// plus sources, the old v, 1/rho and the sponge factors
+// `zint`: every lane of the strip lies outside the top/bottom sponges; with
+// bx = 1 the sponge factor is exactly 1 and its arithmetic is skipped
+// (the same float64 result, bit for bit).
+template <int P>
+__device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&dsv)[n_v(P)], const float (&vo)[n_v(P)],
+                                           float rho, float bxr, double bzb, double bzt, bool zint, bool rin, bool jin,
+                                           double dt)
+{
+    const double dtr = dt * rcp64((double)rho);
+    if (zint && bxr == 1.f) {
+#pragma unroll
+        for (int f = 0; f < n_v(P); f++) {
+            const float v = (float)((double)vo[f] + (double)dsv[f] * dtr);
+            vn[f] = (rin && jin) ? v : 0.f;
+        }
+        return;
+    }
+    const float b1 = (float)((double)bxr * bzb);
+    const double b = (double)(float)((double)b1 * bzt);
+#pragma unroll
+    for (int f = 0; f < n_v(P); f++) {
+        const float v = (float)(b * ((double)vo[f] + (double)dsv[f] * dtr));
+        vn[f] = (rin && jin) ? v : 0.f;
+    }
+}
+
+// stress update of one row (add_sy / add_sxz, solver.py:120-148) from the
+// velocity window rows q-1..q+2 (wa..wd) and the old stresses
+template <int P>
+__device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n_v(P)], const float (&wb)[n_v(P)],
+                                        const float (&wc)[n_v(P)], const float (&wd)[n_v(P)], const float (&so)[n_s(P)],
+                                        float lam, float mu, bool xp_ok, bool zp_ok, int L, double c1x, double c2x,
+                                        double c1z, double c2z, double dt)
+{
+    if constexpr (P == SP_SH) {
+        float zm1 = shfl(wb[0], L - 1), zp1 = shfl(wb[0], L + 1), zp2 = shfl(wb[0], L + 2);
+        const float dvdx = xp_ok ? (float)dplus(wa[0], wb[0], wc[0], wd[0], c1x, c2x) : 0.f;
+        const float dvdz = zp_ok ? (float)dplus(zm1, wb[0], zp1, zp2, c1z, c2z) : 0.f;
+        const double dtmu = dt * (double)mu;
+        sn[0] = (float)((double)so[0] + dtmu * (double)dvdx);
+        sn[1] = (float)((double)so[1] + dtmu * (double)dvdz);
+    } else {
+        float xm1 = shfl(wb[0], L - 1), xp1 = shfl(wb[0], L + 1), xp2 = shfl(wb[0], L + 2);
+        float zm1 = shfl(wb[1], L - 1), zp1 = shfl(wb[1], L + 1), zp2 = shfl(wb[1], L + 2);
+        const float dvxdx = xp_ok ? (float)dplus(wa[0], wb[0], wc[0], wd[0], c1x, c2x) : 0.f;
+        const float dvzdx = xp_ok ? (float)dplus(wa[1], wb[1], wc[1], wd[1], c1x, c2x) : 0.f;
+        const float dvxdz = zp_ok ? (float)dplus(xm1, wb[0], xp1, xp2, c1z, c2z) : 0.f;
+        const float dvzdz = zp_ok ? (float)dplus(zm1, wb[1], zp1, zp2, c1z, c2z) : 0.f;
+        const double l2m = (double)lam + 2.0 * (double)mu;
+        sn[0] = (float)((double)so[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz)));
+        sn[1] = (float)((double)so[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx)));
+        sn[2] = (float)((double)so[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+    }
+}
+
+// sources (forward) or adjoint sources (adjoint) of velocity row r at step
+// `it`: float32 adds in list order (stf_dsy / stf_dsxz, solver.py:78-93)
+template <int P>
+__device__ __forceinline__ void inject(float (&dsv)[n_v(P)], int r, int j, int nz, int &pi, int &src_row, int ni,
+                                       cint_p ik, cint_p irow, cflt_p ival, long long nrows, int nt, int it)
+{
+    if (r != src_row) return;
+    const int kend = (r + 1) * nz, kr = r * nz + j;
+    while (pi < ni && ik[pi] < kend) {
+        const int kk = ik[pi];
+        const int row = irow[pi];
+        const float s0 = ival[(long long)row * nt + it];
+        float s1 = 0.f;
+        if constexpr (P != SP_SH) s1 = ival[(nrows + row) * nt + it];
+        if (kk == kr) {
+            dsv[0] = add_rn(dsv[0], s0);
+            if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
+        }
+        ++pi;
+    }
+    src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_step2(StepArgs a)
+{
+    static_assert(P == SP_SH || P == SP_PSV, "two-step kernel: SH and P-SV");
+    constexpr int NS = n_s(P), NV = n_v(P);
+    const int nb = gridDim.x, bb = blockIdx.x;
+    const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
+    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
+    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
+    if (wave >= a.nitems) return;
+    const int L = threadIdx.x & 63;
+    const int4 item = a.items[2 * wave];
+    if (item.x < 0) return;
+    const int4 item2 = a.items[2 * wave + 1];
+    const ShotDesc &d = a.shots[item.x];
+    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
+    const int j = item.y - a.shift + L;
+    const int jc = min(max(j, 0), nz - 1);
+    const bool jin = j >= 0 && j < nz;
+    const bool outl = jin && L >= a.shift && L < a.shift + a.wout;
+    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
+    const int NP4 = (int)(d.Npad * 4);
+    const double dt = a.dt;
+    const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
+    const int rd = a.rd, wr = a.rd ^ 1;
+    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
+    const bool zint = __all(bzb == 1.0 && bzt == 1.0);
+    const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
+    const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
+    const cflt_p bxp = (cflt_p)d.bx;
+    const unsigned vj = (unsigned)jc * 4u;            // per-lane load offset
+    const unsigned vo = outl ? (unsigned)j * 4u : OOB;  // per-lane store offset (OOB: not written)
+
+    const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
+    const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
+    const int ni = a.adjoint ? d.nrec : d.nsrc;
+    const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
+    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
+    // injection pointers: step it from row i0-4, step it+1 from row i0-1
+    int p1 = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
+    int p2 = p1;
+    while (p2 < ni && ik[p2] < (i0 - 1) * nz) ++p2;
+    int row1 = p1 < ni ? ik[p1] / nz : 0x7fffffff;
+    int row2 = p2 < ni ? ik[p2] / nz : 0x7fffffff;
+    // receivers recorded by this chunk (forward only): rows i0..i1-1
+    const cint_p rk = (cint_p)d.rec_k;
+    const cint_p rrow = (cint_p)d.rec_row;
+    int pr = __builtin_amdgcn_readfirstlane(item2.z);
+    int rec_row = (!a.adjoint && pr < d.nrec) ? rk[pr] / nz : 0x7fffffff;
+    const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
+
+    float S0[4][NS], W1[4][NV], S1[4][NS], W2[4][NV];
+    Row2<P> PF[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+#pragma unroll
+        for (int f = 0; f < NS; f++) S0[u][f] = S1[u][f] = 0.f;
+#pragma unroll
+        for (int f = 0; f < NV; f++) W1[u][f] = W2[u][f] = 0.f;
+    }
+
+    // the march starts at m0 = i0 - 3: s(n) rows m0-3..m0-1 fill window slots 1..3
+    const int m0 = i0 - 3, mend = i1 + 5;
+    load2<P>(PF[1], RS, RM, bxp, NP4, rd, m0 - 3, nx, nz, vj, i0, i1);
+    load2<P>(PF[2], RS, RM, bxp, NP4, rd, m0 - 2, nx, nz, vj, i0, i1);
+    load2<P>(PF[3], RS, RM, bxp, NP4, rd, m0 - 1, nx, nz, vj, i0, i1);
+    load2<P>(PF[0], RS, RM, bxp, NP4, rd, m0, nx, nz, vj, i0, i1);
+#pragma unroll
+    for (int f = 0; f < NS; f++) {
+        S0[1][f] = PF[1].s[f];
+        S0[2][f] = PF[2].s[f];
+        S0[3][f] = PF[3].s[f];
+    }
+    load2<P>(PF[1], RS, RM, bxp, NP4, rd, m0 + 1, nx, nz, vj, i0, i1);
+
+    for (int mb = m0; mb <= mend; mb += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int m = mb + u;
+            if (m > mend) break;
+            const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
+            load2<P>(PF[B], RS, RM, bxp, NP4, rd, m + 2, nx, nz, vj, i0, i1);
+            const Row2<P> &cur = PF[D];
+#pragma unroll
+            for (int f = 0; f < NS; f++) S0[D][f] = cur.s[f];
+
+            // ---- A: v(n+1) at row m-1; s(n) rows m-3..m = slots A, B, C, D
+            {
+                const int r = m - 1;
+                const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
+                float dsv[NV], vn[NV];
+                vel_row<P>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
+                inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, a.it);
+                vel_finish<P>(vn, dsv, cur.v, cur.rhoA, cur.bxA, bzb, bzt, zint, rin, jin, dt);
+#pragma unroll
+                for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
+            }
+            // ---- B: s(n+1) at row m-3; v(n+1) rows m-4..m-1 = slots D, A, B, C
+            {
+                const int q = m - 3;
+                const bool xp_ok = q >= 1 && q < nx - 2;
+                float sn[NS];
+                str_row<P>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, xp_ok, zp_ok, L, c1x, c2x, c1z,
+                           c2z, dt);
+#pragma unroll
+                for (int f = 0; f < NS; f++) S1[A][f] = sn[f];
+            }
+            // ---- C: v(n+2) at row m-4; s(n+1) rows m-6..m-3 = slots B, C, D, A
+            {
+                const int r = m - 4;
+                const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
+                float dsv[NV], vn[NV];
+                vel_row<P>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
+                inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, a.it + 1);
+                vel_finish<P>(vn, dsv, W1[D], cur.rhoC, cur.bxC, bzb, bzt, zint, rin, jin, dt);
+                float u1[NV], u2[NV];
+#pragma unroll
+                for (int f = 0; f < NV; f++) {
+                    u1[f] = (float)((double)cur.u[f] + (double)W1[D][f] * dt);
+                    u2[f] = (float)((double)u1[f] + (double)vn[f] * dt);
+                    W2[D][f] = vn[f];
+                }
+                // rows outside the chunk: every lane's offset is out of range
+                const bool rok = r >= i0 && r < i1;
+                const unsigned ov = rok ? vo : OOB;
+                const int orow = rok ? r * nz * 4 : 0;
+#pragma unroll
+                for (int f = 0; f < NV; f++) {
+                    bst(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
+                    bst(RS, ov, slot_v(P, wr, f) * NP4 + orow, vn[f]);
+                }
+                // save_obs of both steps (solver.py:151-156) for receivers in
+                // this row; rows past the chunk belong to the next chunk
+                if (r == rec_row && r < i1) {
+                    const int kend = (r + 1) * nz, kr = r * nz + j;
+                    while (pr < d.nrec && rk[pr] < kend) {
+                        const int row = rrow[pr];
+                        const unsigned orc = (outl && rk[pr] == kr) ? (unsigned)(row * a.nt + a.it) * 4u : OOB;
+                        const int cs = d.nrec * a.nt * 4;
+#pragma unroll
+                        for (int f = 0; f < NV; f++) {
+                            bst(RR, orc, f * cs, u1[f]);
+                            bst(RR, orc, f * cs + 4, u2[f]);
+                        }
+                        ++pr;
+                    }
+                    rec_row = pr < d.nrec ? rk[pr] / nz : 0x7fffffff;
+                }
+            }
+            // ---- D: s(n+2) at row m-6; v(n+2) rows m-7..m-4 = slots A, B, C, D
+            {
+                const int q = m - 6;
+                const bool xp_ok = q >= 1 && q < nx - 2;
+                float sn[NS];
+                str_row<P>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], cur.lamD, cur.muD, xp_ok, zp_ok, L, c1x, c2x, c1z,
+                           c2z, dt);
+                const bool qok = q >= i0 && q < i1;
+                const unsigned oq = qok ? vo : OOB;
+                const int orow = qok ? q * nz * 4 : 0;
+#pragma unroll
+                for (int f = 0; f < NS; f++) bst(RS, oq, slot_s(P, wr, f) * NP4 + orow, sn[f]);
             }
         }
     }
@@ -1002,6 +1342,8 @@ struct sp_ctx {
     int kernel = 1;
     int cur_kernel = 1;        // kernel of the prepared batch
     int shift = 3, wout = 58;  // strip geometry of the step kernel
+    int fuse = 1;              // two steps per launch where nothing observes the state between them
+    int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
     int graph = 0;
     long long snap_mb = 0;
     // snapshot pool
@@ -1131,6 +1473,30 @@ int batch_kernel(const sp_ctx *c, const std::vector<int> &batch)
     return 1;
 }
 
+// strip geometry of the prepared batch: the two-step kernel needs 6 halo
+// lanes (and the single-step kernel runs on the same strips in that mode)
+int strip_shift(const sp_ctx *c) { return c->cur_fuse ? 6 : (c->cur_kernel == 1 ? c->shift : 3); }
+int strip_width(const sp_ctx *c) { return c->cur_fuse ? 52 : (c->cur_kernel == 1 ? c->wout : 58); }
+
+int fuse_ok(const sp_ctx *c, const std::vector<int> &batch)
+{
+    return c->fuse && c->physics != SP_SPIN && batch_kernel(c, batch) == 1;
+}
+
+// Steps the launch at step `it` may take: two when fusion is on and the
+// state after step it is observed by nothing -- no snapshot on step it or
+// it+1 (each needs the state before and after its step), no interaction
+// after step it, no checkpoint before step it+1 -- and both steps are
+// before `end`.
+int steps_next(const sp_ctx *c, int it, int end, int snap_sae, int inter_sae, int ckpt_C)
+{
+    if (!c->cur_fuse || it + 2 > end) return 1;
+    if (snap_sae > 0 && ((it + 1) % snap_sae == 0 || (it + 2) % snap_sae == 0)) return 1;
+    if (inter_sae > 0 && it % inter_sae == 0) return 1;
+    if (ckpt_C > 0 && (it + 1) % ckpt_C == 0) return 1;
+    return 2;
+}
+
 int auto_chunk(const sp_ctx *c, const std::vector<int> &batch)
 {
     if (c->chunk > 0) return batch_kernel(c, batch) == 1 ? (c->chunk + 3) / 4 * 4 : c->chunk;
@@ -1152,7 +1518,8 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     for (int s : batch) desc.push_back(make_desc(c, c->shots[s]));
     const int chunk = auto_chunk(c, batch);
     c->cur_kernel = batch_kernel(c, batch);
-    const int W = c->cur_kernel == 1 ? c->wout : 58;
+    c->cur_fuse = fuse_ok(c, batch);
+    const int W = strip_width(c);
     std::vector<int4> items;
     auto push = [&](int b, int j0, int i0) {
         const Shot &sh = c->shots[batch[b]];
@@ -1162,8 +1529,10 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
             return (int)(std::lower_bound(v.begin(), v.end(), (int)std::max(key, -1LL)) - v.begin());
         };
         items.push_back(make_int4(b, j0, i0, i1));
-        items.push_back(make_int4(first(sh.src_sorted, (long long)(i0 - 1) * g.nz),
-                                  first(sh.rec_sorted, (long long)(i0 - 1) * g.nz),
+        // list pointers from row i0-4 (the two-step kernel's first velocity
+        // row; the single-step kernels skip forward to row i0-1)
+        items.push_back(make_int4(first(sh.src_sorted, (long long)(i0 - 4) * g.nz),
+                                  first(sh.rec_sorted, (long long)(i0 - 4) * g.nz),
                                   first(sh.rec_sorted, (long long)i0 * g.nz), 0));
     };
     if (c->order == 3) {
@@ -1235,7 +1604,10 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     return 0;
 }
 
-int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, const ShotDesc *desc = nullptr)
+// One launch from step `it`: `nsteps` (1 or 2) steps of every shot of the
+// prepared batch, reading stress/velocity set `rd` and writing set rd ^ 1.
+int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd, int nsteps = 1,
+                const ShotDesc *desc = nullptr)
 {
     StepArgs a;
     a.shots = desc ? desc : c->d_desc;
@@ -1244,13 +1616,19 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, const 
     a.it = it;
     a.nt = c->nt;
     a.dt = c->dt;
-    a.rd = it & 1;
+    a.rd = rd;
     a.snap_idx = snap_idx;
     a.adjoint = adjoint;
-    a.shift = c->cur_kernel == 1 ? c->shift : 3;
-    a.wout = c->cur_kernel == 1 ? c->wout : 58;
+    a.shift = strip_shift(c);
+    a.wout = strip_width(c);
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
-    if (c->cur_kernel == 1) {
+    if (nsteps == 2) {
+        if (!c->cur_fuse) return fail(c, "internal: two-step launch without the two-step geometry");
+        switch (c->physics) {
+        case SP_SH: hipLaunchKernelGGL(k_step2<SP_SH>, grid, block, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_step2<SP_PSV>, grid, block, 0, c->stream, a); break;
+        }
+    } else if (c->cur_kernel == 1) {
         switch (c->physics) {
         case SP_SH: hipLaunchKernelGGL(k_step3<SP_SH>, grid, block, 0, c->stream, a); break;
         case SP_PSV: hipLaunchKernelGGL(k_step3<SP_PSV>, grid, block, 0, c->stream, a); break;
@@ -1309,19 +1687,25 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     }
     const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256)), sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
     const bool split = c->cur_kernel == 1;  // snapshots and records as kernels of their own
-    for (int it = 0; it < c->nt; it++) {
+    int set = 0;
+    for (int it = 0; it < c->nt;) {
+        const int ns = steps_next(c, it, c->nt, sae, 0, 0);
         int isa = -1;
-        if (sae > 0 && (it + 1) % sae == 0) isa = nsa - (it + 1) / sae;
+        if (sae > 0 && (it + 1) % sae == 0) isa = nsa - (it + 1) / sae;  // single steps only
         if (split && isa >= 0)
             hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 0, 0, c->physics);
-        if (launch_step(c, nitems, it, split ? -1 : isa, 0)) return 2;
+        if (launch_step(c, nitems, it, split ? -1 : isa, 0, set, ns)) return 2;
+        set ^= 1;
         if (split) {
             if (isa >= 0)
-                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 1, (it & 1) ^ 1,
+                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 1, set, c->physics);
+            // the two-step kernel records both of its steps itself
+            if (ns == 1)
+                hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt,
                                    c->physics);
-            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics);
             HIPCHECK(c, hipGetLastError());
         }
+        it += ns;
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
     return 0;
@@ -1444,29 +1828,36 @@ int run_adjoint_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     InteractPlan pl;
     if (interact_lists(c, batch, pl)) return 2;
     const double ndt = (double)sae * c->dt;
-    for (int it = 0; it < c->nt; it++) {
-        if (launch_step(c, nitems, it, -1, 1)) return 2;
-        if ((it + sae) % sae == 0)
-            if (launch_interact(c, pl, (it & 1) ^ 1, (it + sae) / sae - 1, ndt)) return 2;
+    int set = 0;
+    for (int it = 0; it < c->nt;) {
+        const int ns = steps_next(c, it, c->nt, 0, sae, 0);
+        if (launch_step(c, nitems, it, -1, 1, set, ns)) return 2;
+        set ^= 1;
+        const int last = it + ns - 1;
+        if (last % sae == 0)
+            if (launch_interact(c, pl, set, last / sae, ndt)) return 2;
+        it += ns;
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
     return 0;
 }
 
 // ---- checkpointed forward and adjoint ----------------------------------
-// A checkpoint is the full state before step k*C (C even, so the read set
-// is set 0): the set-0 stresses, the set-0 velocities and the displacement.
+// A checkpoint is the full state before step k*C: the stresses and
+// velocities of the set holding it, and the displacement.
 int ckpt_fields(int P) { return n_s(P) + 2 * n_v(P); }
 
-int ckpt_copy(sp_ctx *c, Shot &S, int k, bool save)
+// save: set `set` of the state -> checkpoint k; restore: checkpoint k -> set
+// `set` of the second state block
+int ckpt_copy(sp_ctx *c, Shot &S, int k, bool save, int set)
 {
     const int P = c->physics;
     const Grid &G = c->grids[S.grid];
     const size_t fb = (size_t)G.Npad * sizeof(float);
     float *ck = S.ckpt + (size_t)k * ckpt_fields(P) * G.Npad;
     float *st = save ? S.state : S.state2;
-    const size_t parts[3][3] = {{0, (size_t)slot_s(P, 0, 0), (size_t)n_s(P)},
-                                {(size_t)n_s(P), (size_t)slot_v(P, 0, 0), (size_t)n_v(P)},
+    const size_t parts[3][3] = {{0, (size_t)slot_s(P, set, 0), (size_t)n_s(P)},
+                                {(size_t)n_s(P), (size_t)slot_v(P, set, 0), (size_t)n_v(P)},
                                 {(size_t)(n_s(P) + n_v(P)), (size_t)slot_u(P, 0), (size_t)n_v(P)}};
     for (auto &q : parts) {
         float *a = ck + q[0] * G.Npad, *b = st + q[1] * G.Npad;
@@ -1498,15 +1889,19 @@ int run_forward_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
     int maxrec = 1;
     for (int s : batch) maxrec = std::max(maxrec, c->shots[s].nrec);
     const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
-    for (int it = 0; it < c->nt; it++) {
+    int set = 0;
+    for (int it = 0; it < c->nt;) {
         if (it % C == 0)
             for (int s : batch)
-                if (ckpt_copy(c, c->shots[s], it / C, true)) return 2;
-        if (launch_step(c, nitems, it, -1, 0)) return 2;
-        if (c->cur_kernel == 1) {
+                if (ckpt_copy(c, c->shots[s], it / C, true, set)) return 2;
+        const int ns = steps_next(c, it, c->nt, 0, 0, C);
+        if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
+        set ^= 1;
+        if (c->cur_kernel == 1 && ns == 1) {
             hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics);
             HIPCHECK(c, hipGetLastError());
         }
+        it += ns;
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
     return 0;
@@ -1559,30 +1954,38 @@ int run_adjoint_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int sae, in
     for (int s : batch) maxN = std::max(maxN, c->grids[c->shots[s].grid].N);
     const dim3 sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
     const double ndt = (double)sae * c->dt;
-    int seg_loaded = -1, isa_base = 0;
-    for (int a = 0; a < nt; a++) {
-        if (launch_step(c, nitems, a, -1, 1)) return 2;
-        if (a % sae) continue;
-        const int isae = a / sae, seg = (nt - a - 1) / C;
+    int seg_loaded = -1, isa_base = 0, set = 0;
+    for (int a = 0; a < nt;) {
+        const int ns = steps_next(c, a, nt, 0, sae, 0);
+        if (launch_step(c, nitems, a, -1, 1, set, ns)) return 2;
+        set ^= 1;
+        const int last = a + ns - 1;
+        a += ns;
+        if (last % sae) continue;
+        const int isae = last / sae, seg = (nt - last - 1) / C;
         if (seg != seg_loaded) {
             const int lo = seg * C, hi = std::min(nt, lo + C);
             isa_base = nsa - hi / sae;
             for (int s : batch)
-                if (ckpt_copy(c, c->shots[s], seg, false)) return 2;
-            for (int it = lo; it < hi; it++) {
-                const int isa = (it + 1) % sae == 0 ? nsa - (it + 1) / sae : -1;
+                if (ckpt_copy(c, c->shots[s], seg, false, 0)) return 2;
+            int set2 = 0;
+            for (int it = lo; it < hi;) {
+                const int ns2 = steps_next(c, it, hi, sae, 0, 0);
+                const int isa = (it + 1) % sae == 0 ? nsa - (it + 1) / sae : -1;  // single steps only
                 if (isa >= 0)
                     hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa - isa_base, 0, 0,
                                        c->physics);
-                if (launch_step(c, nitems, it, -1, 0, c->d_desc2)) return 2;
+                if (launch_step(c, nitems, it, -1, 0, set2, ns2, c->d_desc2)) return 2;
+                set2 ^= 1;
                 if (isa >= 0)
                     hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa - isa_base, 1,
-                                       (it & 1) ^ 1, c->physics);
+                                       set2, c->physics);
+                it += ns2;
             }
             HIPCHECK(c, hipGetLastError());
             seg_loaded = seg;
         }
-        if (launch_interact(c, pl, (a & 1) ^ 1, isae - isa_base, ndt)) return 2;
+        if (launch_interact(c, pl, set, isae - isa_base, ndt)) return 2;
     }
     c->t_bytes += step_alg_bytes(c, batch) * nt;
     return 0;
@@ -1792,6 +2195,7 @@ int sp_set_option(sp_ctx *c, int key, long long v)
         return 0;
     case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;
     case SP_OPT_KERNEL: c->kernel = v ? 1 : 0; return 0;
+    case SP_OPT_FUSE: c->fuse = v ? 1 : 0; return 0;
     case SP_OPT_STRIP: {
         const int sh = (int)(v >> 16), wo = (int)(v & 0xffff);
         if (sh < 3 || wo < 1 || sh + wo > 61) return fail(c, "strip geometry needs shift >= 3 and shift + width <= 61");
@@ -2236,10 +2640,15 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
             c->order = order;
             int nitems = 0;
             if (prepare_batch(c, all, &nitems) || clear_state(c, all)) return 2;
-            if (launch_step(c, nitems, 0, -1, 0)) return 2;
+            if (launch_step(c, nitems, 0, -1, 0, 0)) return 2;
             HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-            for (int it = 0; it < steps; it++)
-                if (launch_step(c, nitems, it, -1, 0)) return 2;
+            int set = 1;
+            for (int it = 0; it < steps;) {
+                const int ns = steps_next(c, it, steps, 0, 0, 0);
+                if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
+                set ^= 1;
+                it += ns;
+            }
             HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
             HIPCHECK(c, hipEventSynchronize(c->ev1));
             float ms = 0;

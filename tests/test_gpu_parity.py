@@ -235,3 +235,53 @@ def test_errors_are_reported_not_raised_through_the_abi(ctx_factory):
         ctx.gradient(7)
     with pytest.raises(_lib.SeispieError, match="sp_set_data"):
         ctx.gradient(10)
+
+
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV])
+@pytest.mark.parametrize("chunk,order", [(16, 0), (4, 1), (32, 2)])
+def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order):
+    """Two steps per launch (k_step2) give the records, misfit and kernels
+    of one step per launch bit for bit: ragged grids, receivers on every
+    row (chunk boundaries included) and on shared cells, sources in the
+    rows a chunk recomputes for its neighbours and on the grid edges."""
+    nt = 120
+    probs = [_small(physics, 67, 121, nt, 2, seed=4), _small(physics, 40, 59, nt, 1, seed=5)]
+    for p in probs:
+        g = p.grids[0]
+        nz = g.nz
+        rec = [i * nz + 3 + (i % 5) for i in range(g.nx)] + [16 * nz + 20, 16 * nz + 20, 31 * nz + 7]
+        src = [15 * nz + 30, 16 * nz + 60, 0, g.nx * nz - 1, 3 * nz + 55]
+        src_k = np.array(sorted(src), np.int32)
+        comps = configs.ricker_stf(nt, 5e-4, 30.0, ang=30.0)
+        stf = configs.shot_stf(physics, comps)
+        stf_many = np.concatenate([stf * (1 + 0.1 * q) for q in range(src_k.size)], axis=1)
+        g.shots = [configs.Shot(src_k, stf_many, np.array(rec, np.int32)) for _ in g.shots]
+    prob = configs.Problem(physics, nt, 5e-4, [p.grids[0] for p in probs], sae=10)
+    data = [d * 1.01 for d in _data_for(prob)]
+    print("case: fuse", physics, chunk, order, flush=True)
+    out = []
+    for fuse in (0, 1):
+        ctx = ctx_factory()
+        ctx.set_option(_lib.OPT_FUSE, fuse)
+        ctx.set_option(_lib.OPT_CHUNK, chunk)
+        ctx.set_option(_lib.OPT_ORDER, order)
+        ids = configs.load_problem(ctx, prob)
+        ctx.forward(0)
+        recs = [ctx.records(i) for i in ids]
+        for i, d in zip(ids, data):
+            ctx.set_data(i, d)
+        norms = ctx.gradient(prob.sae)
+        kern = [[ctx.kernel(g, w) for w in (_lib.K_LAM, _lib.K_MU, _lib.K_RHO, _lib.K_HESS)]
+                for g in range(len(prob.grids))]
+        t = ctx.timing()
+        out.append((recs, norms, kern, t["step_launches"]))
+    (r0, n0, k0, l0), (r1, n1, k1, l1) = out
+    assert l1 < l0, "fused run took as many launches as the single-step run"
+    for a, b in zip(r0, r1):
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(n0, n1)
+    for ga, gb in zip(k0, k1):
+        for a, b in zip(ga, gb):
+            np.testing.assert_array_equal(a, b)
+    _check_forward(prob, r1)
