@@ -7,7 +7,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "seispie_hip.hip")
+SRC = [os.path.join(HERE, "csrc", "seispie_hip.hip"), os.path.join(HERE, "csrc", "cnn_hip.hip")]
 OUT = os.path.join(HERE, "lib", "libseispie_hip.so")
 
 # -packed-fp32-ops: no v_pk_*_f32 instructions in any kernel.
@@ -20,13 +20,13 @@ def build(verbose=False):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc] + FLAGS + ["-o", tmp, SRC]
+    cmd = [hipcc] + FLAGS + ["-o", tmp] + SRC
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building %s" % SRC)
+        raise RuntimeError("hipcc failed building %s" % " ".join(SRC))
     os.replace(tmp, OUT)
     return OUT
 
