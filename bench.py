@@ -19,7 +19,14 @@ The JSON line adds
                 timed on its own; `traffic` is the measured HBM bytes per
                 launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/);
   cpu_baseline: the oracle's unfused reference kernel sequence (one shot,
-                a fixed window of steps on the same grid) on this host's cores.
+                a fixed window of steps on the same grid) on this host's cores;
+  time_to_misfit: the second half of the metric, on the learned-conditioner
+                FWI workload (P-SV 500 x 500, 64 shots, nt 3000, inverting
+                vp): plain L-BFGS for 20 iterations, then L-BFGS with the
+                learned step conditioner (trained inside its run on the
+                first gradient pass); the wall time the conditioned run needs
+                to reach the plain run's final misfit over the plain run's
+                wall time (--no-fwi skips it).
 """
 import argparse
 import json
@@ -93,6 +100,32 @@ def cpu_baseline(prob, warm=20, nsteps=200):
     return ncores, out
 
 
+def fwi_time_to_misfit(niter):
+    """Time-to-misfit of the learned conditioner against plain L-BFGS; a
+    failure is reported in the line instead of ending the bench."""
+    from seispie_amd import configs, fwi
+    t0 = time.time()
+    try:
+        true = configs.fwi_config5(true_model=True)
+        init = configs.fwi_config5(true_model=False)
+        r = fwi.time_to_misfit(true, init, niter=niter,
+                               log=lambda s: print("# " + s, file=sys.stderr, flush=True))
+    except Exception as e:  # noqa: BLE001 -- reported, not hidden
+        return {"error": "%s: %s" % (type(e).__name__, e)}
+    out = {"workload": init.name + " vp inversion, L-BFGS (lbfgs_mem 5, backtracking)", "iterations": niter,
+           "value": None if r["time_to_misfit"] is None else round(r["time_to_misfit"], 4),
+           "unit": "fraction of plain L-BFGS wall time, conditioner training included", "target": 0.6,
+           "without_training": (None if r["time_to_misfit_without_training"] is None
+                                else round(r["time_to_misfit_without_training"], 4)),
+           "plain_seconds": round(r["plain_seconds"], 2), "plain_final_misfit": r["plain_final_misfit"],
+           "conditioned_seconds": round(r["learned_seconds"], 2),
+           "conditioned_final_misfit": r["learned_final_misfit"],
+           "conditioner_prep_seconds": round(r["prep_seconds"], 3),
+           "conditioner_train_seconds": round(r["train_seconds"], 3),
+           "wall_seconds": round(time.time() - t0, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +138,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="override the number of time steps")
     ap.add_argument("--forward-only", action="store_true",
                     help="time forward passes (the step kernel) instead of gradients; for counter runs")
+    ap.add_argument("--no-fwi", action="store_true", help="skip the time-to-misfit measurement")
+    ap.add_argument("--fwi-iters", type=int, default=20)
     ap.add_argument("--no-fuse", action="store_true",
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
     ap.add_argument("--calibrate", action="store_true",
@@ -206,6 +241,10 @@ def main():
                          "one shot, %d forward steps after 20 warm-up; 1 core: %.4f Gcell-upd/s"
                          % (g.nx, g.nz, cb[ncores][1], cb[1][0])}
 
+    ttm = None
+    if not args.no_fwi and not args.forward_only:
+        ttm = fwi_time_to_misfit(args.fwi_iters)
+
     line = {
         "metric": "Gcell-upd/s (% HBM peak); time-to-misfit, learned vs plain precond",
         "value": round(value, 3),
@@ -229,6 +268,7 @@ def main():
                      "kernel": kname, "launch_ms": round(launch_ms, 4),
                      "alg_bytes_per_launch": bytes_per_launch, "steps_per_launch": round(steps_per_launch, 3)},
         "cpu_baseline": cpu,
+        "time_to_misfit": ttm,
     }
     print(json.dumps(line))
     ctx.close()

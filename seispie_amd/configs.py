@@ -104,18 +104,20 @@ def checkerboard(nx, nz, cell, amp):
 REC_DEPTH_CELLS = 2  # shallowest row the stencils couple to the interior
 
 
-def _geometry(physics, nx, nz, dx, dz, nt, dt, nshots, nrec, f0, ang=0.0):
-    """Evenly spaced sources at z = 10 m (one per shot), receivers at z = 2 dz."""
+def _geometry(physics, nx, nz, dx, dz, nt, dt, nshots, nrec, f0, ang=0.0, src_z=10.0, rec_z=None):
+    """Evenly spaced sources at depth src_z (one per shot), receivers at
+    rec_z (default 2 dz)."""
     dx32, dz32 = np.float32(dx), np.float32(dz)
     lx = (nx - 1) * dx
     rec_x = np.linspace(0.0, lx, nrec)
-    rec_k = np.array([gindex(x, REC_DEPTH_CELLS * dz, dx32, dz32, nz) for x in rec_x], dtype=np.int32)
+    rec_z = REC_DEPTH_CELLS * dz if rec_z is None else rec_z
+    rec_k = np.array([gindex(x, rec_z, dx32, dz32, nz) for x in rec_x], dtype=np.int32)
     comps = ricker_stf(nt, dt, f0, ang)
     stf = shot_stf(physics, comps)
     shots = []
     for s in range(nshots):
         x = lx * (s + 0.5) / nshots
-        shots.append(Shot(np.array([gindex(x, 10.0, dx32, dz32, nz)], np.int32), stf, rec_k))
+        shots.append(Shot(np.array([gindex(x, src_z, dx32, dz32, nz)], np.int32), stf, rec_k))
     return shots
 
 
@@ -230,9 +232,16 @@ def fwi_config5_model(n=500, dx=5.0, true_model=True):
 
 
 def fwi_config5(n=500, nt=3000, nshots=64, true_model=True):
+    """Learned-conditioner FWI: 500^2, dx 5 m, dt 0.5 ms, nt 3000, 64 shots of
+    a 15 Hz Ricker, 200 receivers.  Sponges on all four sides, sources at
+    125 m and receivers at 130 m depth, just below the 20-cell sponge: the
+    reference-style adjoint is the gradient only away from the stencils'
+    edge bands and the sponges (with sources and receivers at 10 m beside a
+    free surface the finite-difference slope of the misfit and the adjoint
+    kernels disagree even in sign), and an inversion needs the gradient."""
     dt = 5e-4
-    shots = _geometry(_lib.PSV, n, n, 5.0, 5.0, nt, dt, nshots, min(200, n), 15.0, ang=30.0)
-    g = Grid(n, n, 5.0, 5.0, fwi_config5_model(n, 5.0, true_model), 0, shots=shots)
+    shots = _geometry(_lib.PSV, n, n, 5.0, 5.0, nt, dt, nshots, min(200, n), 15.0, ang=30.0, src_z=125.0, rec_z=130.0)
+    g = Grid(n, n, 5.0, 5.0, fwi_config5_model(n, 5.0, true_model), 0, sponge=(20, 0.015, 1, 1, 1, 1), shots=shots)
     return Problem(_lib.PSV, nt, dt, [g], sae=50, name="fwi_%dx%d_%dshots_nt%d" % (n, n, nshots, nt))
 
 

@@ -30,10 +30,11 @@ class base:
         misfits = []
         self.history = []
         head = 0 if (getattr(self, 'mpi', None) and self.mpi.rank()) else 1
+        say = print if head and not getattr(self, 'quiet', False) else (lambda *a: None)
+        report = getattr(self, 'on_iteration', None)
 
         for i in range(niter):
-            if head:
-                print('Iteration %d' % (i + 1))
+            say('Iteration %d' % (i + 1))
             self.g_new, misfit, self.m_new = solver.compute_gradient()
             if head:
                 if i > 0:
@@ -42,6 +43,8 @@ class base:
                     ref = misfit
                 misfits.append(misfit / ref)
             self.history.append((time() - start, float(misfit)))
+            if report:
+                report(i, self.history[-1][0], float(misfit))
 
             self.p_new = self.compute_direction()
             self.m_old = self.m_new
@@ -50,22 +53,23 @@ class base:
 
             if self.line_search(misfit) < 0:
                 if head:
-                    print('Line search failed')
+                    say('Line search failed')
                     break
-            if head:
-                print('')
+            say('')
 
         misfit = solver.compute_misfit()
         self.history.append((time() - start, float(misfit)))
+        if report:
+            report(niter, self.history[-1][0], float(misfit))
         if head:
             misfits.append(misfit / ref)
             m_new = np.zeros(solver.nx * solver.nz, dtype='float32')
             solver.mu.copy_to_host(m_new, solver.stream)
             solver.stream.synchronize()
             solver.export_field(m_new, 'mu', niter)
-            print('Misfits:')
+            say('Misfits:')
             for m in misfits:
-                print('  %.4f' % m)
-            print('')
-            print('Elapsed time: %.2fs' % (time() - start))
+                say('  %.4f' % m)
+            say('')
+            say('Elapsed time: %.2fs' % (time() - start))
         return misfits

@@ -37,10 +37,12 @@ class cg(base):
         if self.precond is None:
             return g
         pg = self.precond(g)
-        # the unconditioned step is the fallback when P g is not a descent step
-        return pg if np.dot(pg, g) > 0 else g
+        # the unconditioned step is the fallback when P g is not a finite descent step
+        return pg if np.all(np.isfinite(pg)) and np.dot(pg, g) > 0 else g
 
     def compute_direction(self):
+        if self.precond is not None and hasattr(self.precond, 'refresh'):
+            self.precond.refresh(self.g_new)
         g_new = self._conditioned(self.g_new)
         if not hasattr(self, 'g_old'):
             return -g_new

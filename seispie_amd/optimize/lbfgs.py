@@ -9,10 +9,12 @@ class lbfgs(cg):
     (seispie/optimize/lbfgs.py:5-59).
 
     self.precond, when set, replaces the scalar initial inverse Hessian
-    (y.s / y.y) of the two-loop recursion with the callable g -> H0 g; if the
-    resulting direction is not a descent direction the unconditioned
-    L-BFGS direction is used, and if that is not one either the search
-    restarts from steepest descent, as the reference does.
+    (y.s / y.y) of the two-loop recursion with the callable q -> P q, scaled
+    by y.s / y.(P y) so that unit steps stay well sized; its refresh(g), if
+    it has one, is called first with the new gradient.  If the resulting
+    direction is not a descent direction the unconditioned L-BFGS direction
+    is used, and if that is not one either the search restarts from
+    steepest descent, as the reference does.
     """
 
     def setup(self, workflow):
@@ -45,6 +47,8 @@ class lbfgs(cg):
 
     def compute_direction(self):
         g_new = self.g_new
+        if self.precond is not None and hasattr(self.precond, 'refresh'):
+            self.precond.refresh(g_new)
         if not hasattr(self, 'g_old'):
             return -self._conditioned(g_new)
 
@@ -58,9 +62,13 @@ class lbfgs(cg):
         scalar = lambda q: q * (np.dot(y0, s0) / np.dot(y0, y0))
         q = None
         if self.precond is not None:
-            q = self._two_loop(g_new, self.precond)
-            if self.bracket.angle(g_new, q) >= np.pi / 2:
-                q = None
+            py = self.precond(y0)
+            yPy = np.dot(y0, py)
+            if yPy > 0 and np.isfinite(yPy):
+                gamma = np.dot(y0, s0) / yPy
+                q = self._two_loop(g_new, lambda v: gamma * self.precond(v))
+                if not np.all(np.isfinite(q)) or self.bracket.angle(g_new, q) >= np.pi / 2:
+                    q = None
         if q is None:
             q = self._two_loop(g_new, scalar)
         if self.bracket.angle(g_new, q) >= np.pi / 2:

@@ -42,6 +42,11 @@ class bracket:
         solver = self.solver
         norm_m = np.amax(abs(m_new))
         norm_p = np.amax(abs(p_new))
+        if not (np.isfinite(norm_p) and norm_p > 0):
+            # a vanished (exact minimum) or non-finite direction: the
+            # reference's step_max is then inf and its backtracking loops
+            # for ever comparing the trial count with it
+            return -1
         gtg = np.dot(g_new, g_new)
         gtp = np.dot(g_new, p_new)
         step_max = self.ls_step_max * norm_m / norm_p
