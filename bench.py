@@ -140,6 +140,8 @@ def main():
                     help="time forward passes (the step kernel) instead of gradients; for counter runs")
     ap.add_argument("--no-fwi", action="store_true", help="skip the time-to-misfit measurement")
     ap.add_argument("--fwi-iters", type=int, default=20)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay forward runs without snapshots from a captured hipGraph")
     ap.add_argument("--no-fuse", action="store_true",
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
     ap.add_argument("--calibrate", action="store_true",
@@ -174,6 +176,8 @@ def main():
     ctx = _lib.Context(0)
     if args.no_fuse:
         ctx.set_option(_lib.OPT_FUSE, 0)
+    if args.graph:
+        ctx.set_option(_lib.OPT_GRAPH, 1)
     if args.chunk:
         ctx.set_option(_lib.OPT_CHUNK, args.chunk)
     ctx.set_option(_lib.OPT_ORDER, args.order)

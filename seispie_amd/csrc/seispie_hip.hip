@@ -79,12 +79,18 @@ struct ShotDesc {
     double c1x, c2x, c1z, c2z;  // 9/(8h), 1/(24h)
 };
 
+// read-only, wave-uniform data through the constant address space: scalar
+// loads, counted by lgkmcnt and never by the vector memory counter
+typedef const int __attribute__((address_space(4))) *cint_p;
+typedef const float __attribute__((address_space(4))) *cflt_p;
+
 struct StepArgs {
     const ShotDesc *shots;
     const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-4) nz,
                         // the first receiver with k >= (i0-4) nz and the first with k >= i0 nz
     int nitems;
-    int it, nt;
+    int it, nt;         // step of this launch (plus *itp when itp is set)
+    const int *itp;     // device step counter of a replayed graph, or null
     double dt;
     int rd;        // stress/velocity set read this step
     int snap_idx;  // -1: no snapshot this step
@@ -207,6 +213,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
     if (item.x < 0) return;        // padding entry of the work list
     const int4 item2 = a.items[2 * wave + 1];
     const ShotDesc &d = a.shots[item.x];
+    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - 3 + L;
     const int jc = min(max(j, 0), nz - 1);
@@ -325,8 +332,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
             while (pi < ni && ik[pi] < kend) {
                 if (ik[pi] == kr) {
                     const int row = irow[pi];
-                    dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + a.it]);
-                    if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], ival[(nrows + row) * a.nt + a.it]);
+                    dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + itx]);
+                    if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], ival[(nrows + row) * a.nt + itx]);
                 }
                 ++pi;
             }
@@ -367,7 +374,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
                             const int row = d.rec_row[q2];
 #pragma unroll
                             for (int f = 0; f < NV; f++)
-                                d.rec[((long long)f * d.nrec + row) * a.nt + a.it] = un[f];
+                                d.rec[((long long)f * d.nrec + row) * a.nt + itx] = un[f];
                         }
                     }
                 }
@@ -451,10 +458,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
 //     the register windows never move.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 constexpr unsigned OOB = 0x40000000u;  // >= every per-shot buffer size (checked on the host)
-// read-only, wave-uniform data through the constant address space: scalar
-// loads, counted by lgkmcnt and never by the vector memory counter
-typedef const int __attribute__((address_space(4))) *cint_p;
-typedef const float __attribute__((address_space(4))) *cflt_p;
 __device__ __forceinline__ rsrc_t make_rsrc(const void *p, long long bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
@@ -525,6 +528,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     if (item.x < 0) return;
     const int4 item2 = a.items[2 * wave + 1];
     const ShotDesc &d = a.shots[item.x];
+    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - a.shift + L;
     const int jc = min(max(j, 0), nz - 1);
@@ -631,9 +635,9 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
                 while (pi < ni && ik[pi] < kend) {
                     const int kk = ik[pi];
                     const int row = irow[pi];
-                    const float s0 = ival[(long long)row * a.nt + a.it];
+                    const float s0 = ival[(long long)row * a.nt + itx];
                     float s1 = 0.f;
-                    if constexpr (P != SP_SH) s1 = ival[(nrows + row) * a.nt + a.it];
+                    if constexpr (P != SP_SH) s1 = ival[(nrows + row) * a.nt + itx];
                     if (kk == kr) {
                         dsv[0] = add_rn(dsv[0], s0);
                         if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
@@ -901,6 +905,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     if (item.x < 0) return;
     const int4 item2 = a.items[2 * wave + 1];
     const ShotDesc &d = a.shots[item.x];
+    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - a.shift + L;
     const int jc = min(max(j, 0), nz - 1);
@@ -978,7 +983,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV], vn[NV];
                 vel_row<P>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
-                inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, a.it);
+                inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
                 vel_finish<P>(vn, dsv, cur.v, cur.rhoA, cur.bxA, bzb, bzt, zint, rin, jin, dt);
 #pragma unroll
                 for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
@@ -999,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV], vn[NV];
                 vel_row<P>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
-                inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, a.it + 1);
+                inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
                 vel_finish<P>(vn, dsv, W1[D], cur.rhoC, cur.bxC, bzb, bzt, zint, rin, jin, dt);
                 float u1[NV], u2[NV];
 #pragma unroll
@@ -1023,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                     const int kend = (r + 1) * nz, kr = r * nz + j;
                     while (pr < d.nrec && rk[pr] < kend) {
                         const int row = rrow[pr];
-                        const unsigned orc = (outl && rk[pr] == kr) ? (unsigned)(row * a.nt + a.it) * 4u : OOB;
+                        const unsigned orc = (outl && rk[pr] == kr) ? (unsigned)(row * a.nt + itx) * 4u : OOB;
                         const int cs = d.nrec * a.nt * 4;
 #pragma unroll
                         for (int f = 0; f < NV; f++) {
@@ -1054,8 +1059,10 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 
 // save_obs (solver.py:151-156) for every receiver of a batch, after a step:
 // one thread per (shot, receiver) of the batch.
-__global__ __launch_bounds__(256) void k_record(const ShotDesc *shots, int nshots, int maxrec, int it, int nt, int P)
+__global__ __launch_bounds__(256) void k_record(const ShotDesc *shots, int nshots, int maxrec, int it, int nt, int P,
+                                                const int *itp)
 {
+    if (itp) it += *itp;
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int s = t / maxrec, q = t - s * maxrec;
     if (s >= nshots) return;
@@ -1065,6 +1072,12 @@ __global__ __launch_bounds__(256) void k_record(const ShotDesc *shots, int nshot
     const int nv = n_v(P);
     for (int f = 0; f < nv; f++)
         d.rec[((long long)f * d.nrec + row) * nt + it] = d.state[(long long)(2 * n_s(P) + 2 * nv + f) * d.Npad + k];
+}
+
+// advances the step counter of a replayed graph (one thread, vector store)
+__global__ void k_advance(int *counter, int steps)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += steps;
 }
 
 // Adjoint snapshots (solver.py:193-203, 236-242): u before a step (part 0)
@@ -1343,6 +1356,8 @@ struct sp_ctx {
     int cur_kernel = 1;        // kernel of the prepared batch
     int shift = 3, wout = 58;  // strip geometry of the step kernel
     int fuse = 1;              // two steps per launch where nothing observes the state between them
+    int *d_counter = nullptr;  // step counter read by the kernels of a replayed graph
+    const int *itp = nullptr;  // set while a graph is captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
     int graph = 0;
     long long snap_mb = 0;
@@ -1614,6 +1629,7 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     a.items = c->d_items;
     a.nitems = nitems;
     a.it = it;
+    a.itp = c->itp;
     a.nt = c->nt;
     a.dt = c->dt;
     a.rd = rd;
@@ -1671,6 +1687,83 @@ int clear_state(sp_ctx *c, const std::vector<int> &batch)
     return 0;
 }
 
+// Forward steps [0, nt) of a prepared batch without snapshots, as replays of
+// one captured graph: a block of `block` steps (their step launches and
+// receiver records, step numbers relative to a device counter) and a
+// one-thread kernel that advances the counter; the steps past the last
+// whole block run eagerly.  The block returns the stress/velocity set to
+// where it started (an even number of launches), so every replay starts
+// from the same set.  Needs a stream other than the null stream.
+int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
+{
+    const int nt = c->nt;
+    int block = 64;
+    while (block > 4 && block > nt) block /= 2;
+    auto launches_in = [&](int steps) {
+        int n = 0;
+        for (int it = 0; it < steps;) {
+            it += steps_next(c, it, steps, 0, 0, 0);
+            n++;
+        }
+        return n;
+    };
+    if (launches_in(block) % 2) block -= 2;  // keep the set parity of a block even
+    const int nblk = nt / block;
+    if (nblk < 2) return 1;  // not worth a graph: the caller runs eagerly
+    if (!c->d_counter) HIPCHECK(c, hipMalloc(&c->d_counter, sizeof(int)));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->d_counter, 1LL, 0.f);  // 0.f = int 0
+    HIPCHECK(c, hipGetLastError());
+    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->itp = c->d_counter;
+    int s = set, rc = 0, nl = 0;
+    for (int it = 0; it < block && !rc;) {
+        const int ns = steps_next(c, it, block, 0, 0, 0);
+        rc = launch_step(c, nitems, it, -1, 0, s, ns);
+        s ^= 1;
+        nl++;
+        if (!rc && ns == 1)
+            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
+                               c->itp);
+        it += ns;
+    }
+    if (!rc) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_counter, block);
+    c->itp = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    HIPCHECK(c, ec);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(c, ei);
+    for (int b = 0; b < nblk; b++) {
+        const hipError_t el = hipGraphLaunch(exec, c->stream);
+        if (el != hipSuccess) {
+            (void)hipGraphExecDestroy(exec);
+            HIPCHECK(c, el);
+        }
+    }
+    HIPCHECK(c, hipGraphExecDestroy(exec));
+    c->t_launches += (long long)nl * (nblk - 1);  // the capture counted one block
+    // the remaining steps, eagerly
+    for (int it = nblk * block; it < nt;) {
+        const int ns = steps_next(c, it, nt, 0, 0, 0);
+        if (launch_step(c, nitems, it, -1, 0, s, ns)) return 2;
+        s ^= 1;
+        if (ns == 1)
+            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
+                               nullptr);
+        HIPCHECK(c, hipGetLastError());
+        it += ns;
+    }
+    set = s;
+    return 0;
+}
+
 // forward run of a batch (solver.py:166-242); snapshots when sae > 0
 int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
 {
@@ -1688,6 +1781,14 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256)), sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
     const bool split = c->cur_kernel == 1;  // snapshots and records as kernels of their own
     int set = 0;
+    if (c->graph && sae == 0 && split && c->stream) {
+        const int rc = run_forward_graph(c, nitems, nb, maxrec, set);
+        if (rc == 0) {
+            c->t_bytes += step_alg_bytes(c, batch) * c->nt;
+            return 0;
+        }
+        if (rc != 1) return rc;
+    }
     for (int it = 0; it < c->nt;) {
         const int ns = steps_next(c, it, c->nt, sae, 0, 0);
         int isa = -1;
@@ -1702,7 +1803,7 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
             // the two-step kernel records both of its steps itself
             if (ns == 1)
                 hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt,
-                                   c->physics);
+                                   c->physics, c->itp);
             HIPCHECK(c, hipGetLastError());
         }
         it += ns;
@@ -1898,7 +1999,8 @@ int run_forward_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
         if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
         set ^= 1;
         if (c->cur_kernel == 1 && ns == 1) {
-            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics);
+            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
+                               nullptr);
             HIPCHECK(c, hipGetLastError());
         }
         it += ns;
@@ -2153,6 +2255,7 @@ int sp_destroy(sp_ctx *c)
     if (c->snap_pool) (void)hipFree(c->snap_pool);
     if (c->d_desc) (void)hipFree(c->d_desc);
     if (c->d_desc2) (void)hipFree(c->d_desc2);
+    if (c->d_counter) (void)hipFree(c->d_counter);
     if (c->seg_pool) (void)hipFree(c->seg_pool);
     if (c->d_items) (void)hipFree(c->d_items);
     if (c->d_list) (void)hipFree(c->d_list);
@@ -2193,7 +2296,7 @@ int sp_set_option(sp_ctx *c, int key, long long v)
         if (v < 0 || v > 3) return fail(c, "order must be 0, 1, 2 or 3");
         c->order = (int)v;
         return 0;
-    case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;
+    case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;  // forward runs without snapshots, own or set stream
     case SP_OPT_KERNEL: c->kernel = v ? 1 : 0; return 0;
     case SP_OPT_FUSE: c->fuse = v ? 1 : 0; return 0;
     case SP_OPT_STRIP: {

@@ -285,3 +285,25 @@ def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order
         for a, b in zip(ga, gb):
             np.testing.assert_array_equal(a, b)
     _check_forward(prob, r1)
+
+
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+def test_graph_replayed_forward_equals_eager(ctx_factory, physics):
+    """Forward runs replayed from a captured graph (SP_OPT_GRAPH) record
+    exactly what eager launches record, on repeated runs with other work
+    (a gradient) between them; nt leaves a remainder past the last block."""
+    prob = _small(physics, nt=150, nshots=2)
+    data = _data_for(prob) if physics != _lib.SPIN else None
+    print("case: graph", physics, flush=True)
+    _, eager = _run_forward(ctx_factory, prob)
+    ctx = ctx_factory()
+    ctx.set_option(_lib.OPT_GRAPH, 1)
+    ids = configs.load_problem(ctx, prob)
+    for rep in range(3):
+        ctx.forward(0)
+        for i in ids:
+            np.testing.assert_array_equal(ctx.records(i), eager[i], err_msg="replay run %d" % rep)
+        if data is not None and rep == 0:
+            for i, d in zip(ids, data):
+                ctx.set_data(i, d)
+            ctx.gradient(prob.sae)
