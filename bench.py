@@ -100,6 +100,55 @@ def cpu_baseline(prob, warm=20, nsteps=200):
     return ncores, out
 
 
+def autograd_bench(true, init, args):
+    """Forward + backward of the waveform misfit through
+    seispie_amd.autograd (checkpointed adjoint with recomputation)."""
+    import torch
+    from seispie_amd import autograd
+    if len(init.grids) != 1 or init.physics == _lib_spin():
+        raise SystemExit("--autograd needs a one-grid SH or P-SV workload")
+    g = init.grids[0]
+    data = observed_data(true)
+    sim = autograd.Simulation(init.physics, g.nx, g.nz, g.dx, g.dz, init.nt, init.dt,
+                              [(s.src_k, s.stf, s.rec_k) for s in g.shots], sponge=g.sponge, sae=init.sae)
+    dev = sim.device
+    d = torch.tensor(np.stack([x[:sim.nmis] for x in data]), device=dev)
+    m = [torch.tensor(f, device=dev, requires_grad=True) for f in g.fields]
+
+    def step():
+        for t in m:
+            t.grad = None
+        loss = autograd.waveform_loss(sim(*m), d, init.dt)
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = init.cell_updates(passes=2)
+    print(json.dumps({
+        "metric": "Gcell-upd/s (autograd forward + backward)", "value": round(cells * args.steps / wall / 1e9, 3),
+        "unit": "Gcell-upd/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2), "higher_is_better": True, "dtype": "f32 storage, f64 arithmetic",
+        "data": "synthetic", "loss": float(loss.detach()),
+        "config": {"workload": init.name + " waveform misfit, loss.backward()", "shots": init.nshots, "nt": init.nt,
+                   "adjoint_interval": init.sae, "checkpoint_interval": sim.interval,
+                   "note": "cell updates count forward and adjoint once; the forward recomputed from "
+                           "checkpoints in backward is not counted"}}))
+    sim.close()
+    return 0
+
+
+def _lib_spin():
+    from seispie_amd import _lib
+    return _lib.SPIN
+
+
 def fwi_time_to_misfit(niter):
     """Time-to-misfit of the learned conditioner against plain L-BFGS; a
     failure is reported in the line instead of ending the bench."""
@@ -140,6 +189,8 @@ def main():
                     help="time forward passes (the step kernel) instead of gradients; for counter runs")
     ap.add_argument("--no-fwi", action="store_true", help="skip the time-to-misfit measurement")
     ap.add_argument("--fwi-iters", type=int, default=20)
+    ap.add_argument("--autograd", action="store_true",
+                    help="time loss.backward() through the torch.autograd.Function (checkpointed adjoint) instead")
     ap.add_argument("--graph", action="store_true",
                     help="replay forward runs without snapshots from a captured hipGraph")
     ap.add_argument("--no-fuse", action="store_true",
@@ -169,6 +220,8 @@ def main():
             for g in p.grids:
                 for s in g.shots:
                     s.stf = np.ascontiguousarray(s.stf[..., :args.nt])
+    if args.autograd:
+        return autograd_bench(true, init, args)
     t0 = time.time()
     data = observed_data(true)
     print("# observed data for %s in %.1fs" % (true.name, time.time() - t0), file=sys.stderr, flush=True)

@@ -107,6 +107,18 @@ int sp_num_components(const sp_ctx *ctx); /* records per receiver: SH 1, P-SV 2,
  * displacement at the receivers.  sae > 0 also keeps the adjoint snapshots
  * (solver.py:193-203, 236-242) on the device for sp_adjoint. */
 int sp_forward(sp_ctx *ctx, int sae);
+/* The forward run of every shot in pieces, for field exports during the run
+ * (save_snapshot, solver.py:244-261): steps [it0, it1), records included;
+ * it0 = 0 starts a run (clears the wavefields), a later call continues
+ * from where the previous one stopped. */
+int sp_forward_steps(sp_ctx *ctx, int it0, int it1);
+/* the current wavefield of a shot after sp_forward / sp_forward_steps:
+ * SP_WF_V velocities [n_v][N] (SH vy; P-SV vx, vz; spin vx, vz, vy_c),
+ * SP_WF_U displacements [n_v][N], SP_WF_S stresses [n_s][N] */
+#define SP_WF_V 0
+#define SP_WF_U 1
+#define SP_WF_S 2
+int sp_get_wavefield(sp_ctx *ctx, int shot, int kind, float *dst, int dst_on_device);
 /* records of a shot: [ncomp][nrec][nt] (SH y; P-SV x, z; spin x, z, y) */
 int sp_get_records(sp_ctx *ctx, int shot, float *dst, int dst_on_device);
 /* observed data ("syn" in fd2d.py:303-309), same layout as the records */

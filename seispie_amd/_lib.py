@@ -12,6 +12,7 @@ SH, PSV, SPIN = 1, 2, 3
 F_RHO, F_LAM, F_MU, F_NU, F_J, F_LAMC, F_MUC, F_NUC = range(8)
 K_LAM, K_MU, K_RHO, K_NU, K_J, K_MUC, K_VP, K_VS, K_RHOP, K_HESS = range(10)
 OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP, OPT_FUSE = 1, 2, 3, 4, 5, 6, 7
+WF_V, WF_U, WF_S = 0, 1, 2
 
 N_REC = {SH: 1, PSV: 2, SPIN: 3}   # record components: SH y; P-SV x, z; spin x, z, y
 N_INJ = {SH: 1, PSV: 2, SPIN: 2}   # source components: SH y; P-SV/spin x, z
@@ -93,6 +94,8 @@ def load():
         "sp_adjoint_ckpt": ([vp, i], i),
         "sp_set_adjoint_source": ([vp, i, vp, i], i),
         "sp_set_model_vps": ([vp, i, vp, vp, vp, i], i),
+        "sp_forward_steps": ([vp, i, i], i),
+        "sp_get_wavefield": ([vp, i, i, vp, i], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -109,7 +112,7 @@ def exported_symbols():
             "sp_add_shot", "sp_clear_shots", "sp_num_components", "sp_forward", "sp_get_records",
             "sp_set_data", "sp_misfit", "sp_adjoint", "sp_gradient", "sp_get_kernel", "sp_kernels_vps",
             "sp_smooth_kernel", "sp_smooth", "sp_last_timing", "sp_autotune", "sp_forward_ckpt",
-            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps"]
+            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps", "sp_forward_steps", "sp_get_wavefield"]
 
 
 def _f32(a, n=None):
@@ -230,6 +233,23 @@ class Context:
     # ------------------------------------------------------------- runs
     def forward(self, sae=0):
         self._chk(self.L.sp_forward(self._p, int(sae)))
+
+    def forward_steps(self, it0, it1):
+        """Steps [it0, it1) of the forward run of every shot (it0 = 0 starts one)."""
+        self._chk(self.L.sp_forward_steps(self._p, int(it0), int(it1)))
+
+    def wavefield(self, shot, kind=WF_V, out=None):
+        """Current velocities (WF_V), displacements (WF_U) or stresses (WF_S)
+        of a shot, [ncomp][nx*nz]."""
+        nx, nz = self.grids[self.shots[shot][0]]
+        nc = {WF_V: N_REC[self.physics], WF_U: N_REC[self.physics],
+              WF_S: {SH: 2, PSV: 3, SPIN: 6}[self.physics]}[kind]
+        if out is not None and _is_tensor(out):
+            self._chk(self.L.sp_get_wavefield(self._p, shot, kind, _devptr(out), 1))
+            return out
+        a = np.empty((nc, nx * nz), dtype=np.float32)
+        self._chk(self.L.sp_get_wavefield(self._p, shot, kind, _ptr(a), 0))
+        return a
 
     def records(self, shot, out=None):
         nrec = self.shots[shot][2]

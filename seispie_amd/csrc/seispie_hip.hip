@@ -1365,6 +1365,8 @@ struct sp_ctx {
     float *snap_pool = nullptr;
     size_t snap_pool_bytes = 0;
     int snaps_valid = 0, snaps_sae = 0;  // snapshots of every shot held from the last sp_forward
+    int cur_set = 0;      // stress/velocity set holding the state after the last forward run
+    int fwd_next = -1;    // next step of a forward run advanced by sp_forward_steps (-1: none)
     // batch descriptors
     ShotDesc *d_desc = nullptr;
     size_t d_desc_cap = 0;
@@ -1784,6 +1786,7 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     if (c->graph && sae == 0 && split && c->stream) {
         const int rc = run_forward_graph(c, nitems, nb, maxrec, set);
         if (rc == 0) {
+            c->cur_set = set;
             c->t_bytes += step_alg_bytes(c, batch) * c->nt;
             return 0;
         }
@@ -1808,6 +1811,7 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
         }
         it += ns;
     }
+    c->cur_set = set;
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
     return 0;
 }
@@ -2151,6 +2155,7 @@ int assign_snapshots(sp_ctx *c, const std::vector<int> &batch, int sae)
 
 int timing_begin(sp_ctx *c)
 {
+    c->fwd_next = -1;  // every run replaces the wavefields a stepwise forward run left
     c->t_launches = 0;
     c->t_bytes = 0;
     if (!c->ev0) {
@@ -2412,6 +2417,7 @@ int sp_set_field(sp_ctx *c, int grid, int field, const float *src, int on_dev)
     }
     c->snaps_valid = 0;
     c->ckpt_valid = 0;
+    c->fwd_next = -1;
     return 0;
 }
 
@@ -2427,6 +2433,7 @@ int sp_get_field(sp_ctx *c, int grid, int field, float *dst, int on_dev)
 int sp_add_shot(sp_ctx *c, int grid, int nsrc, const int *src_k, const float *stf, int nrec, const int *rec_k, int *shot_out)
 {
     if (int e = check_ctx(c)) return e;
+    c->fwd_next = -1;
     if (grid < 0 || grid >= (int)c->grids.size()) return fail(c, "bad grid id");
     if (nsrc < 0 || nrec < 0) return fail(c, "negative source or receiver count");
     if ((nsrc && (!src_k || !stf)) || (nrec && !rec_k)) return fail(c, "null source/receiver arrays");
@@ -2484,6 +2491,7 @@ int sp_clear_shots(sp_ctx *c)
     c->shots.clear();
     c->snaps_valid = 0;
     c->ckpt_valid = 0;
+    c->fwd_next = -1;
     return 0;
 }
 
@@ -2507,6 +2515,66 @@ int sp_forward(sp_ctx *c, int sae)
         c->snaps_valid = 1;
         c->snaps_sae = sae;
     }
+    return 0;
+}
+
+int sp_forward_steps(sp_ctx *c, int it0, int it1)
+{
+    if (int e = check_ctx(c)) return e;
+    if (c->shots.empty()) return fail(c, "no shots");
+    if (it0 < 0 || it1 <= it0 || it1 > c->nt) return fail(c, "steps must satisfy 0 <= it0 < it1 <= nt");
+    if (it0 > 0 && it0 != c->fwd_next) return fail(c, "sp_forward_steps must continue from step " + std::to_string(c->fwd_next));
+    std::vector<int> all(c->shots.size());
+    std::iota(all.begin(), all.end(), 0);
+    const int nb = (int)all.size();
+    int maxrec = 1;
+    for (int s : all) maxrec = std::max(maxrec, c->shots[s].nrec);
+    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
+    int nitems = 0;
+    if (prepare_batch(c, all, &nitems)) return 2;
+    if (it0 == 0) {
+        c->snaps_valid = 0;
+        c->ckpt_valid = 0;
+    c->fwd_next = -1;
+        if (clear_state(c, all)) return 2;
+        c->cur_set = 0;
+    }
+    if (timing_begin(c)) return 2;
+    int set = c->cur_set;
+    for (int it = it0; it < it1;) {
+        const int ns = steps_next(c, it, it1, 0, 0, 0);
+        if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
+        set ^= 1;
+        if (c->cur_kernel == 1 && ns == 1) {
+            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
+                               nullptr);
+            HIPCHECK(c, hipGetLastError());
+        }
+        it += ns;
+    }
+    c->cur_set = set;
+    c->fwd_next = it1 < c->nt ? it1 : -1;
+    c->t_bytes += step_alg_bytes(c, all) * (it1 - it0);
+    return timing_end(c);
+}
+
+int sp_get_wavefield(sp_ctx *c, int shot, int kind, float *dst, int on_dev)
+{
+    if (int e = check_ctx(c)) return e;
+    if (shot < 0 || shot >= (int)c->shots.size()) return fail(c, "bad shot id");
+    const int P = c->physics;
+    const Shot &S = c->shots[shot];
+    const Grid &G = c->grids[S.grid];
+    int first, n;
+    switch (kind) {
+    case SP_WF_V: first = slot_v(P, c->cur_set, 0), n = n_v(P); break;
+    case SP_WF_U: first = slot_u(P, 0), n = n_v(P); break;
+    case SP_WF_S: first = slot_s(P, c->cur_set, 0), n = n_s(P); break;
+    default: return fail(c, "bad wavefield kind");
+    }
+    for (int f = 0; f < n; f++)
+        if (dcopy(c, dst + (size_t)f * G.N, S.state + (size_t)(first + f) * G.Npad, (size_t)G.N * sizeof(float), on_dev, 1))
+            return 2;
     return 0;
 }
 
@@ -2593,6 +2661,7 @@ int sp_forward_ckpt(sp_ctx *c, int C)
     std::vector<int> all(c->shots.size());
     std::iota(all.begin(), all.end(), 0);
     c->ckpt_valid = 0;
+    c->fwd_next = -1;
     c->snaps_valid = 0;
     if (timing_begin(c)) return 2;
     if (run_forward_ckpt_batch(c, all, C)) return 2;
@@ -2640,6 +2709,7 @@ int sp_set_model_vps(sp_ctx *c, int grid, const float *vp, const float *vs, cons
     HIPCHECK(c, hipGetLastError());
     c->snaps_valid = 0;
     c->ckpt_valid = 0;
+    c->fwd_next = -1;
     return 0;
 }
 

@@ -243,9 +243,13 @@ class fd2d(base):
         isrc = self._isrc()
         ids = list(range(self.nsrc)) if isrc < 0 else [isrc]
         self._obs = {}
+        sfe = int(self.config.get('save_snapshot', 0) or 0)
         for physics, ctx in self._contexts():
             self._load_shots(physics, ctx, [ids])
-            ctx.forward(0)
+            if sfe > 0:
+                self._forward_exporting(physics, ctx, sfe)
+            else:
+                ctx.forward(0)
             rec = ctx.records(0)
             names = ['y'] if physics == _lib.SH else (['x', 'z', 'y'] if physics == _lib.SPIN else ['x', 'z'])
             for c, nm in enumerate(names):
@@ -262,6 +266,21 @@ class fd2d(base):
                 np.save('%s/uz_%06d.npy' % (tracedir, i), self._obs['z'].reshape([nrec, nt]))
                 if self.spin:
                     np.save('%s/ry_%06d.npy' % (tracedir, i), self._obs['y'].reshape([nrec, nt]))
+
+    def _forward_exporting(self, physics, ctx, sfe):
+        """solver.py:244-261: after every step it > 0 with it % sfe == 0 the
+        velocities are written as proc%06d_<name>.bin (it in the name):
+        vy (SH); vx, vz (P-SV); and ry, the spin velocity, with spin."""
+        names = ['vy'] if physics == _lib.SH else (['vx', 'vz', 'ry'] if physics == _lib.SPIN else ['vx', 'vz'])
+        start = 0
+        for it in range(sfe, self.nt, sfe):
+            ctx.forward_steps(start, it + 1)
+            start = it + 1
+            v = ctx.wavefield(0, _lib.WF_V)
+            for c, nm in enumerate(names):
+                self.export_field(v[c], nm, it)
+        if start < self.nt:
+            ctx.forward_steps(start, self.nt)
 
     @property
     def obs_x(self):

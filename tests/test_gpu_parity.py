@@ -307,3 +307,36 @@ def test_graph_replayed_forward_equals_eager(ctx_factory, physics):
             for i, d in zip(ids, data):
                 ctx.set_data(i, d)
             ctx.gradient(prob.sae)
+
+
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+def test_stepwise_forward_and_wavefields(ctx_factory, physics):
+    """A forward run in pieces records what one run records (bit for bit),
+    and the velocities read back after step k match the oracle's velocity
+    after step k (its adjoint snapshot of a k+1-step run)."""
+    prob = _small(physics, nt=120, nshots=2)
+    print("case: stepwise", physics, flush=True)
+    _, whole = _run_forward(ctx_factory, prob)
+    ctx = ctx_factory()
+    ids = configs.load_problem(ctx, prob)
+    k = 37
+    ctx.forward_steps(0, k + 1)
+    v = ctx.wavefield(ids[0], _lib.WF_V)
+    with pytest.raises(_lib.SeispieError, match="continue"):
+        ctx.forward_steps(k + 5, 90)
+    ctx.forward_steps(k + 1, 90)
+    ctx.forward_steps(90, prob.nt)
+    for i in ids:
+        np.testing.assert_array_equal(ctx.records(i), whole[i])
+    # oracle: k+1 steps with one snapshot at step k (v after the step)
+    g = prob.grids[0]
+    sh = g.shots[0]
+    short = configs.Problem(prob.physics, k + 1, prob.dt,
+                            [configs.Grid(g.nx, g.nz, g.dx, g.dz, g.fields, g.param, g.sponge,
+                                          [configs.Shot(sh.src_k, np.ascontiguousarray(sh.stf[..., :k + 1]),
+                                                        sh.rec_k)])], sae=k + 1)
+    _, snaps, _ = oracle_records(short, short.grids[0], short.grids[0].shots[0], sae=k + 1)
+    nv = v.shape[0]
+    want = np.asarray(snaps).reshape(-1, g.nx * g.nz)[nv:2 * nv]
+    assert np.abs(want).max() > 0
+    assert rel_l2(v, want) <= SEIS_TOL

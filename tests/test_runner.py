@@ -27,7 +27,7 @@ abs_right = yes
 abs_width = 12
 abs_alpha = 0.015
 save_coordinates = yes
-save_snapshot = 0
+save_snapshot = 100
 threads_per_block = 512
 combine_sources = yes
 
@@ -122,6 +122,16 @@ def test_forward_and_adjoint_workflows_match_the_oracle(tmp_path):
     assert np.abs(ref).max() > 0
     assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
     assert os.path.exists(os.path.join(fw, "output", "proc000000_x.bin"))
+    # save_snapshot = 100: vy after steps 100 and 200 (solver.py:244-261)
+    for it in (100, 200):
+        vy = np.fromfile(os.path.join(fw, "output", "proc%06d_vy.bin" % it), np.float32)
+        assert vy[:1].view(np.int32)[0] == s.nx * s.nz
+        short = O.Sim(O.SH, s.nx, s.nz, s.dx, s.dz, s.dt, it + 1, {"rho": rho, "lam": lam, "mu": mu}, bound,
+                      s.src_id, s.rec_id, -1)
+        _, snaps = short.forward([x.reshape(-1, s.nt)[:, :it + 1].reshape(-1) for x in stf], sae=it + 1)
+        want = np.asarray(snaps).reshape(2, -1)[1]
+        assert np.linalg.norm(vy[1:] - want) <= 1e-5 * np.linalg.norm(want)
+    assert not os.path.exists(os.path.join(fw, "output", "proc000300_vy.bin"))
 
     config, wd = sprun.parse_config(ad, "config.ini")
     wf = sprun.setup(config, wd)
