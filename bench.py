@@ -254,7 +254,9 @@ def main():
         return 0
     if not args.chunk:
         tuned = ctx.autotune(20)
-        print("# autotuned chunk=%d order=%d" % tuned, file=sys.stderr, flush=True)
+        print("# autotuned chunk=%d order=%d two-step launches %s" % (tuned + ("allowed" if ctx.get_option(_lib.OPT_FUSE)
+                                                                             else "off",)),
+              file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         ctx.gradient(sae)

@@ -161,11 +161,14 @@ int sp_smooth_kernel(sp_ctx *ctx, int grid, int which, int sigma);
 int sp_smooth(sp_ctx *ctx, float *field, int nx, int nz, int sigma, int on_device);
 
 /* Autotuner: times `steps` forward steps of the current shots for each
- * rows-per-wavefront chunk and work order, keeps the fastest for later runs
- * and reports it.  Overwrites wavefields and records (every run clears
+ * rows-per-wavefront chunk, work order and (SH, P-SV) one or two steps per
+ * launch, keeps the fastest for later runs and reports it; the choice is
+ * remembered for the process and reused for the same physics and grids.  Overwrites wavefields and records (every run clears
  * them first).  No reference counterpart (the reference launches one
  * thread per cell with threads_per_block from the config, fd2d.py:185-187). */
 int sp_autotune(sp_ctx *ctx, int steps, int *chunk_out, int *order_out);
+/* current value of an option (SP_OPT_*), e.g. the autotuner's SP_OPT_FUSE choice */
+int sp_get_option(const sp_ctx *ctx, int key, long long *value);
 
 /* Timing of the last sp_forward / sp_adjoint / sp_gradient on the
  * context's stream: step-kernel launches and their total milliseconds

@@ -95,6 +95,7 @@ def load():
         "sp_set_adjoint_source": ([vp, i, vp, i], i),
         "sp_set_model_vps": ([vp, i, vp, vp, vp, i], i),
         "sp_forward_steps": ([vp, i, i], i),
+        "sp_get_option": ([vp, i, ctypes.POINTER(ll)], i),
         "sp_get_wavefield": ([vp, i, i, vp, i], i),
     }
     for name, (args, res) in sig.items():
@@ -112,7 +113,7 @@ def exported_symbols():
             "sp_add_shot", "sp_clear_shots", "sp_num_components", "sp_forward", "sp_get_records",
             "sp_set_data", "sp_misfit", "sp_adjoint", "sp_gradient", "sp_get_kernel", "sp_kernels_vps",
             "sp_smooth_kernel", "sp_smooth", "sp_last_timing", "sp_autotune", "sp_forward_ckpt",
-            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps", "sp_forward_steps", "sp_get_wavefield"]
+            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps", "sp_forward_steps", "sp_get_wavefield", "sp_get_option"]
 
 
 def _f32(a, n=None):
@@ -180,6 +181,11 @@ class Context:
 
     def set_option(self, key, value):
         self._chk(self.L.sp_set_option(self._p, int(key), int(value)))
+
+    def get_option(self, key):
+        v = ctypes.c_longlong()
+        self._chk(self.L.sp_get_option(self._p, int(key), ctypes.byref(v)))
+        return v.value
 
     def set_physics(self, physics, nt, dt):
         self._chk(self.L.sp_set_physics(self._p, int(physics), int(nt), float(dt)))

@@ -28,6 +28,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -742,8 +745,8 @@ struct Row2 {
     float s[n_s(P)];     // s(n), row m
     float v[n_v(P)];     // v(n), row m-1
     float u[n_v(P)];     // u(n), row m-4
-    float rhoA, bxA;     // row m-1
-    float rhoC, bxC;     // row m-4
+    float rhoA;          // row m-1
+    float rhoC;          // row m-4
     float lamB, muB;     // row m-3
     float lamD, muD;     // row m-6
 };
@@ -775,8 +778,6 @@ __device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
     }
     p.rhoA = bld(RM, vj, M_RHO * NP4 + oa);
     p.rhoC = bld(RM, vj, M_RHO * NP4 + oc);
-    p.bxA = bxp[ma];
-    p.bxC = bxp[mc];
     p.muB = bld(RM, vj, M_MU * NP4 + ob);
     p.muD = bld(RM, vj, M_MU * NP4 + od);
     p.lamB = p.lamD = 0.f;
@@ -984,7 +985,8 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 float dsv[NV], vn[NV];
                 vel_row<P>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
                 inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
-                vel_finish<P>(vn, dsv, cur.v, cur.rhoA, cur.bxA, bzb, bzt, zint, rin, jin, dt);
+                const float bxr = bxp[min(max(r, 0), nx - 1)];
+                vel_finish<P>(vn, dsv, cur.v, cur.rhoA, bxr, bzb, bzt, zint, rin, jin, dt);
 #pragma unroll
                 for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
             }
@@ -1005,7 +1007,8 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 float dsv[NV], vn[NV];
                 vel_row<P>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
                 inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
-                vel_finish<P>(vn, dsv, W1[D], cur.rhoC, cur.bxC, bzb, bzt, zint, rin, jin, dt);
+                const float bxr = bxp[min(max(r, 0), nx - 1)];
+                vel_finish<P>(vn, dsv, W1[D], cur.rhoC, bxr, bzb, bzt, zint, rin, jin, dt);
                 float u1[NV], u2[NV];
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
@@ -2802,45 +2805,88 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
     steps = std::min(steps, c->nt);
     std::vector<int> all(c->shots.size());
     std::iota(all.begin(), all.end(), 0);
+    // results are kept per process for the batch's shape: physics, kernel
+    // choice and every shot's grid
+    std::string key = std::to_string(c->physics) + "/" + std::to_string(c->kernel) + "/" + std::to_string(c->fuse) +
+                      "/" + std::to_string(steps);
+    for (const Shot &sh : c->shots)
+        key += ":" + std::to_string(c->grids[sh.grid].nx) + "x" + std::to_string(c->grids[sh.grid].nz);
+    static std::mutex mu;
+    static std::map<std::string, std::array<int, 3>> cache;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto f = cache.find(key);
+        if (f != cache.end()) {
+            c->chunk = f->second[0];
+            c->order = f->second[1];
+            c->fuse = f->second[2];
+            if (chunk_out) *chunk_out = c->chunk;
+            if (order_out) *order_out = c->order;
+            return 0;
+        }
+    }
     const int chunks[] = {4, 8, 16, 32, 64};
-    const int keep_chunk = c->chunk, keep_order = c->order;
+    std::vector<int> fuses = {c->fuse};
+    if (c->fuse && fuse_ok(c, all)) fuses.push_back(0);  // one or two steps per launch
     float best = 1e30f;
-    int bc = 16, bo = 0;
+    int bc = 16, bo = 0, bf = c->fuse;
     if (timing_begin(c)) return 2;
-    for (int order = 0; order < 4; order++) {
-        for (int ch : chunks) {
-            c->chunk = ch;
-            c->order = order;
-            int nitems = 0;
-            if (prepare_batch(c, all, &nitems) || clear_state(c, all)) return 2;
-            if (launch_step(c, nitems, 0, -1, 0, 0)) return 2;
-            HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-            int set = 1;
-            for (int it = 0; it < steps;) {
-                const int ns = steps_next(c, it, steps, 0, 0, 0);
-                if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
-                set ^= 1;
-                it += ns;
-            }
-            HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
-            HIPCHECK(c, hipEventSynchronize(c->ev1));
-            float ms = 0;
-            HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            if (ms < best) {
-                best = ms;
-                bc = ch;
-                bo = order;
+    for (int fu : fuses) {
+        c->fuse = fu;
+        for (int order = 0; order < 4; order++) {
+            for (int ch : chunks) {
+                c->chunk = ch;
+                c->order = order;
+                int nitems = 0;
+                if (prepare_batch(c, all, &nitems) || clear_state(c, all)) return 2;
+                if (launch_step(c, nitems, 0, -1, 0, 0)) return 2;
+                HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+                int set = 1;
+                for (int it = 0; it < steps;) {
+                    const int ns = steps_next(c, it, steps, 0, 0, 0);
+                    if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
+                    set ^= 1;
+                    it += ns;
+                }
+                HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+                HIPCHECK(c, hipEventSynchronize(c->ev1));
+                float ms = 0;
+                HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                if (ms < best) {
+                    best = ms;
+                    bc = ch;
+                    bo = order;
+                    bf = fu;
+                }
             }
         }
     }
-    (void)keep_chunk;
-    (void)keep_order;
     c->chunk = bc;
     c->order = bo;
+    c->fuse = bf;
     c->snaps_valid = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        cache[key] = {bc, bo, bf};
+    }
     if (chunk_out) *chunk_out = bc;
     if (order_out) *order_out = bo;
     return 0;
+}
+
+int sp_get_option(const sp_ctx *c, int key, long long *value)
+{
+    if (!c || !value) return 3;
+    switch (key) {
+    case SP_OPT_CHUNK: *value = c->chunk; return 0;
+    case SP_OPT_ORDER: *value = c->order; return 0;
+    case SP_OPT_GRAPH: *value = c->graph; return 0;
+    case SP_OPT_SNAP_MB: *value = c->snap_mb; return 0;
+    case SP_OPT_KERNEL: *value = c->kernel; return 0;
+    case SP_OPT_STRIP: *value = ((long long)c->shift << 16) | c->wout; return 0;
+    case SP_OPT_FUSE: *value = c->fuse; return 0;
+    default: return 1;
+    }
 }
 
 int sp_last_timing(const sp_ctx *c, long long *launches, double *step_ms, double *total_ms, double *bytes)
