@@ -195,6 +195,9 @@ def main():
                     help="replay forward runs without snapshots from a captured hipGraph")
     ap.add_argument("--no-fuse", action="store_true",
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
+    ap.add_argument("--exact", action="store_true",
+                    help="step kernels in exact mode (the reference's float64 operations in order, no fused "
+                         "multiply-adds; records equal the oracle bit for bit)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB elementwise kernel whose HBM bytes are known, to scale "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -229,6 +232,8 @@ def main():
     ctx = _lib.Context(0)
     if args.no_fuse:
         ctx.set_option(_lib.OPT_FUSE, 0)
+    if args.exact:
+        ctx.set_option(_lib.OPT_EXACT, 1)
     if args.graph:
         ctx.set_option(_lib.OPT_GRAPH, 1)
     if args.chunk:

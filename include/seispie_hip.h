@@ -66,6 +66,11 @@ typedef struct sp_ctx sp_ctx;
 #define SP_OPT_STRIP 6      /* strip geometry (shift << 16) | width: output lanes [shift, shift + width) */
 #define SP_OPT_FUSE 7       /* 1 (default): SH and P-SV advance two steps per launch wherever no snapshot,
                                kernel interaction or checkpoint needs the state between them; 0: one step */
+#define SP_OPT_EXACT 8      /* 1 (default): the step kernels perform the reference's float64 operations one
+                               by one in its order (correctly rounded quotients, no fused multiply-add), so
+                               records equal the reference arithmetic bit for bit at any length; 0:
+                               multiplications by reciprocals and fused multiply-adds (about 5% faster;
+                               rounding-level differences that grow with the run length) */
 
 /* Context and device.  Replaces fd2d.setup (fd2d.py:111-120). */
 int sp_create(int device, sp_ctx **out);

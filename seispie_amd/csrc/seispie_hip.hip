@@ -22,9 +22,14 @@
 // Numerics follow the reference: float32 storage, arithmetic in float64
 // wherever Numba promotes (int literal * float32, Python-float dt) and the
 // few float32 products it keeps, rounded to float32 at the points where
-// the reference stores an array.  The only difference is that 1/(8dx),
-// 1/(24dx) and 1/rho are applied as multiplications, which moves results
-// by at most one float32 rounding step on rare cells.
+// the reference stores an array.  By default (SP_OPT_EXACT = 1) every
+// float64 operation is the reference's, in its order and with one rounding
+// each: divisions by 8h and 24h are products with a two-double split of the
+// quotient that the host verifies to round identically for every float32
+// input, divisions by rho use a correctly rounded reciprocal and one
+// correction step, and nothing is contracted into fused multiply-adds.
+// SP_OPT_EXACT = 0 multiplies by reciprocals and contracts, which moves
+// single results by one float32 rounding step on rare cells.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -80,6 +85,8 @@ struct ShotDesc {
     int nsrc, nrec, nx, nz;
     long long Npad;
     double c1x, c2x, c1z, c2z;  // 9/(8h), 1/(24h)
+    // exact mode: 9/(8h) and 1/(24h) as unevaluated sums hi + lo of two doubles
+    double k9hx, k9lx, k1hx, k1lx, k9hz, k9lz, k1hz, k1lz;
 };
 
 // read-only, wave-uniform data through the constant address space: scalar
@@ -148,6 +155,121 @@ __device__ __forceinline__ double rcp64(double x)
     return fma(r, fma(-x, r, 1.0), r);
 }
 
+// ---- exact mode: the reference's float64 operations, one rounding each, in
+// its order (no contraction), so results equal the oracle bit for bit.
+// a / c correctly rounded from rc = RN(1/c): q = RN(a rc), the remainder
+// a - q c is exact by FMA, and RN(q + r rc) is the correctly rounded quotient
+// (Markstein's correction step; also checked against IEEE division on
+// 2e8 random operands of these ranges).
+__device__ __forceinline__ double rn_div(double a, double c, double rc)
+{
+#pragma clang fp contract(off)
+    const double q = a * rc;
+    const double r = __builtin_fma(-q, c, a);
+    return __builtin_fma(r, rc, q);
+}
+// RN(1/x) for float32 x: three Newton steps from the hardware estimate.  Two
+// already give the correctly rounded reciprocal of every float32 mantissa
+// from any start within 2^-20 (exhaustive check); the third keeps it so from
+// starts as far off as 2^-14.
+__device__ __forceinline__ double rcp_rn(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+// RN(num x / den) as fma(x, hi, x lo) with num/den split into hi + lo (the
+// host checks the identity for every float32 mantissa of x)
+__device__ __forceinline__ double kmul(double x, double hi, double lo)
+{
+    return __builtin_fma(x, hi, x * lo);
+}
+// D- and D+ as the reference writes them: 9 (a) / (8h) - (b) / (24h)
+__device__ __forceinline__ double dminus_x(float m2, float m1, float c0, float p1, double k9h, double k9l,
+                                           double k1h, double k1l)
+{
+#pragma clang fp contract(off)
+    return kmul((double)sub_rn(c0, m1), k9h, k9l) - kmul((double)sub_rn(p1, m2), k1h, k1l);
+}
+__device__ __forceinline__ double dplus_x(float m1, float c0, float p1, float p2, double k9h, double k9l,
+                                          double k1h, double k1l)
+{
+#pragma clang fp contract(off)
+    return kmul((double)sub_rn(p1, c0), k9h, k9l) - kmul((double)sub_rn(p2, m1), k1h, k1l);
+}
+// add_vy / add_vxz: v = bound (v + dt ds / rho), then u = u + v dt
+__device__ __forceinline__ float vel_x(float vo, float ds, double dt, double rho, double irho, double b)
+{
+#pragma clang fp contract(off)
+    return (float)(b * ((double)vo + rn_div(dt * (double)ds, rho, irho)));
+}
+__device__ __forceinline__ float disp_x(float u, float v, double dt)
+{
+#pragma clang fp contract(off)
+    return (float)((double)u + (double)v * dt);
+}
+// s + f * g, one rounding per operation (add_sy and the spin couple stresses)
+__device__ __forceinline__ float sadd_x(float s, double f, double g)
+{
+#pragma clang fp contract(off)
+    return (float)((double)s + f * g);
+}
+// add_sxz: s + dt (l2m d1 + float32(lam d2))
+__device__ __forceinline__ float snorm_x(float s, double dt, double l2m, float d1, float lam_d2)
+{
+#pragma clang fp contract(off)
+    return (float)((double)s + dt * (l2m * (double)d1 + (double)lam_d2));
+}
+// add_sy_c: syy_c + 2 dt nu (-vy_c - 0.5 float32(dvzdx - dvxdz))
+__device__ __forceinline__ float srot_x(float s, double dt2, float nu, float vyc, float curl)
+{
+#pragma clang fp contract(off)
+    return (float)((double)s + dt2 * (double)nu * (-(double)vyc - 0.5 * (double)curl));
+}
+
+// the updates in either mode (fast: contraction allowed)
+template <bool EX>
+__device__ __forceinline__ float upd_sadd(float s, double f, double g)
+{
+    if constexpr (EX) return sadd_x(s, f, g);
+    else return (float)((double)s + f * g);
+}
+template <bool EX>
+__device__ __forceinline__ float upd_snorm(float s, double dt, double l2m, float d1, float lam_d2)
+{
+    if constexpr (EX) return snorm_x(s, dt, l2m, d1, lam_d2);
+    else return (float)((double)s + dt * (l2m * (double)d1 + (double)lam_d2));
+}
+template <bool EX>
+__device__ __forceinline__ float upd_srot(float s, double dt2, float nu, float vyc, float curl)
+{
+    if constexpr (EX) return srot_x(s, dt2, nu, vyc, curl);
+    else return (float)((double)s + dt2 * (double)nu * (-(double)vyc - 0.5 * (double)curl));
+}
+template <bool EX>
+__device__ __forceinline__ float upd_disp(float u, float v, double dt)
+{
+    if constexpr (EX) return disp_x(u, v, dt);
+    else return (float)((double)u + (double)v * dt);
+}
+// derivative coefficients of a shot in both forms
+struct DCo {
+    double c1, c2, k9h, k9l, k1h, k1l;
+};
+template <bool EX>
+__device__ __forceinline__ double dmn(float m2, float m1, float c0, float p1, const DCo &k)
+{
+    if constexpr (EX) return dminus_x(m2, m1, c0, p1, k.k9h, k.k9l, k.k1h, k.k1l);
+    else return dminus(m2, m1, c0, p1, k.c1, k.c2);
+}
+template <bool EX>
+__device__ __forceinline__ double dpl(float m1, float c0, float p1, float p2, const DCo &k)
+{
+    if constexpr (EX) return dplus_x(m1, c0, p1, p2, k.k9h, k.k9l, k.k1h, k.k1l);
+    else return dplus(m1, c0, p1, p2, k.c1, k.c2);
+}
+
 // Everything iteration m of the row march reads from memory, fetched one
 // iteration ahead so that a wavefront's loads overlap its own arithmetic.
 // Row and lane indices are clamped into the grid so every load is valid;
@@ -200,7 +322,7 @@ __device__ __forceinline__ void load_row(RowLoads<P> &p, const StepPtrs &q, int 
 #ifndef SP_STEP_WAVES
 #define SP_STEP_WAVES 1
 #endif
-template <int P>
+template <int P, bool EX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAVES))) void k_step(StepArgs a)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
@@ -225,7 +347,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
     const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
     const long long NP = d.Npad;
     const double dt = a.dt, dt2 = 2.0 * a.dt;
-    const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
+    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
     const int rd = a.rd, wr = a.rd ^ 1;
     const double bzb = d.bzb[jc], bzt = d.bzt[jc];
 
@@ -303,28 +425,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
             if constexpr (P == SP_SH) {
                 // div_sy (solver.py:54-58)
                 float zm2 = shfl(S2[1], L - 2), zm1 = shfl(S2[1], L - 1), zp1 = shfl(S2[1], L + 1);
-                double ddx = xm_ok ? dminus(S0[0], S1[0], S2[0], S3[0], c1x, c2x) : 0.0;
-                double ddz = zm_ok ? dminus(zm2, zm1, S2[1], zp1, c1z, c2z) : 0.0;
+                double ddx = xm_ok ? dmn<EX>(S0[0], S1[0], S2[0], S3[0], KX) : 0.0;
+                double ddz = zm_ok ? dmn<EX>(zm2, zm1, S2[1], zp1, KZ) : 0.0;
                 dsv[0] = (float)(ddx + ddz);
             } else {
                 // div_sxz (solver.py:61-66)
                 float xzm2 = shfl(S2[2], L - 2), xzm1 = shfl(S2[2], L - 1), xzp1 = shfl(S2[2], L + 1);
                 float zzm2 = shfl(S2[1], L - 2), zzm1 = shfl(S2[1], L - 1), zzp1 = shfl(S2[1], L + 1);
-                double dxx = xm_ok ? dminus(S0[0], S1[0], S2[0], S3[0], c1x, c2x) : 0.0;
-                double dxxz = xm_ok ? dminus(S0[2], S1[2], S2[2], S3[2], c1x, c2x) : 0.0;
-                double dzxz = zm_ok ? dminus(xzm2, xzm1, S2[2], xzp1, c1z, c2z) : 0.0;
-                double dzzz = zm_ok ? dminus(zzm2, zzm1, S2[1], zzp1, c1z, c2z) : 0.0;
+                double dxx = xm_ok ? dmn<EX>(S0[0], S1[0], S2[0], S3[0], KX) : 0.0;
+                double dxxz = xm_ok ? dmn<EX>(S0[2], S1[2], S2[2], S3[2], KX) : 0.0;
+                double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, S2[2], xzp1, KZ) : 0.0;
+                double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, S2[1], zzp1, KZ) : 0.0;
                 dsv[0] = (float)(dxx + dzxz);
                 dsv[1] = (float)(dxxz + dzzz);
                 if constexpr (P == SP_SPIN) {
                     // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
                     float yzm2 = shfl(S2[5], L - 2), yzm1 = shfl(S2[5], L - 1), yzp1 = shfl(S2[5], L + 1);
                     float yym2 = shfl(S2[4], L - 2), yym1 = shfl(S2[4], L - 1), yyp1 = shfl(S2[4], L + 1);
-                    double dxyx = xm_ok ? dminus(S0[3], S1[3], S2[3], S3[3], c1x, c2x) : 0.0;
-                    double dzyz = zm_ok ? dminus(yzm2, yzm1, S2[5], yzp1, c1z, c2z) : 0.0;
+                    double dxyx = xm_ok ? dmn<EX>(S0[3], S1[3], S2[3], S3[3], KX) : 0.0;
+                    double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, S2[5], yzp1, KZ) : 0.0;
                     float dsy = (float)(dxyx + dzyz);
-                    double dzyy = zm_ok ? dminus(yym2, yym1, S2[4], yyp1, c1z, c2z) : 0.0;
-                    double dxyy = xm_ok ? dminus(S0[4], S1[4], S2[4], S3[4], c1x, c2x) : 0.0;
+                    double dzyy = zm_ok ? dmn<EX>(yym2, yym1, S2[4], yyp1, KZ) : 0.0;
+                    double dxyy = xm_ok ? dmn<EX>(S0[4], S1[4], S2[4], S3[4], KX) : 0.0;
                     dsv[0] = (float)((double)dsv[0] + dzyy);
                     dsv[1] = (float)((double)dsv[1] - dxyy);
                     dsv[2] = (float)((double)dsy + 2.0 * (double)S2[4]);
@@ -344,21 +466,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
             // set_bound (fd2d.py:47-68): f32(f32(bx*bottom)*top)
             const float b1 = (float)((double)cur.bx * bzb);
             const double b = (double)(float)((double)b1 * bzt);
-            const double dtr = dt * rcp64((double)cur.rho);
+            if constexpr (EX) {
+                const double rr = (double)cur.rho, irr = rcp_rn(rr);
 #pragma unroll
-            for (int f = 0; f < NV; f++) {
-                double dtm = dtr;
-                if constexpr (P == SP_SPIN) {
-                    if (f == 2) dtm = dt * rcp64((double)cur.jj);
+                for (int f = 0; f < NV; f++) {
+                    double rf = rr, ir = irr;
+                    if constexpr (P == SP_SPIN) {
+                        if (f == 2) {
+                            rf = (double)cur.jj;
+                            ir = rcp_rn(rf);
+                        }
+                    }
+                    const float v = vel_x(cur.v[f], dsv[f], dt, rf, ir, b);
+                    vn[f] = jin ? v : 0.f;
                 }
-                vn[f] = jin ? (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm)) : 0.f;
+            } else {
+                const double dtr = dt * rcp64((double)cur.rho);
+#pragma unroll
+                for (int f = 0; f < NV; f++) {
+                    double dtm = dtr;
+                    if constexpr (P == SP_SPIN) {
+                        if (f == 2) dtm = dt * rcp64((double)cur.jj);
+                    }
+                    vn[f] = jin ? (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm)) : 0.f;
+                }
             }
             if (outl && r >= i0 && r < i1) {
                 const unsigned k = (unsigned)(r * nz + j);
                 float un[NV];
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
-                    un[f] = (float)((double)cur.u[f] + (double)vn[f] * dt);
+                    un[f] = upd_disp<EX>(cur.u[f], vn[f], dt);
                     u_out[f][k] = un[f];
                     v_out[f][k] = vn[f];
                 }
@@ -402,11 +540,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
             // div_vy + add_sy (solver.py:112-117, 128-133)
             float zm1 = shfl(W1[0], L - 1), zp1 = shfl(W1[0], L + 1), zp2 = shfl(W1[0], L + 2);
             if (outl) {
-                const float dvdx = xp_ok ? (float)dplus(W0[0], W1[0], W2[0], W3[0], c1x, c2x) : 0.f;
-                const float dvdz = zp_ok ? (float)dplus(zm1, W1[0], zp1, zp2, c1z, c2z) : 0.f;
+                const float dvdx = xp_ok ? (float)dpl<EX>(W0[0], W1[0], W2[0], W3[0], KX) : 0.f;
+                const float dvdz = zp_ok ? (float)dpl<EX>(zm1, W1[0], zp1, zp2, KZ) : 0.f;
                 const double dtmu = dt * (double)cur.mu;
-                s_out[0][kq] = (float)((double)S0[0] + dtmu * (double)dvdx);
-                s_out[1][kq] = (float)((double)S0[1] + dtmu * (double)dvdz);
+                s_out[0][kq] = upd_sadd<EX>(S0[0], dtmu, (double)dvdx);
+                s_out[1][kq] = upd_sadd<EX>(S0[1], dtmu, (double)dvdz);
             }
         } else {
             // div_vxz + add_sxz (solver.py:119-126, 143-149)
@@ -419,24 +557,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
                 cp2 = shfl(W1[2], L + 2);
             }
             if (outl) {
-                const float dvxdx = xp_ok ? (float)dplus(W0[0], W1[0], W2[0], W3[0], c1x, c2x) : 0.f;
-                const float dvzdx = xp_ok ? (float)dplus(W0[1], W1[1], W2[1], W3[1], c1x, c2x) : 0.f;
-                const float dvxdz = zp_ok ? (float)dplus(xm1, W1[0], xp1, xp2, c1z, c2z) : 0.f;
-                const float dvzdz = zp_ok ? (float)dplus(zm1, W1[1], zp1, zp2, c1z, c2z) : 0.f;
+                const float dvxdx = xp_ok ? (float)dpl<EX>(W0[0], W1[0], W2[0], W3[0], KX) : 0.f;
+                const float dvzdx = xp_ok ? (float)dpl<EX>(W0[1], W1[1], W2[1], W3[1], KX) : 0.f;
+                const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, W1[0], xp1, xp2, KZ) : 0.f;
+                const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, W1[1], zp1, zp2, KZ) : 0.f;
                 const float lam = cur.lam, mu = cur.mu;
                 const double l2m = (double)lam + 2.0 * (double)mu;
-                s_out[0][kq] = (float)((double)S0[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz)));
-                s_out[1][kq] = (float)((double)S0[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx)));
-                s_out[2][kq] = (float)((double)S0[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+                s_out[0][kq] = upd_snorm<EX>(S0[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
+                s_out[1][kq] = upd_snorm<EX>(S0[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
+                s_out[2][kq] = upd_sadd<EX>(S0[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
                 if constexpr (P == SP_SPIN) {
                     // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
-                    const float dcdx = xp_ok ? (float)dplus(W0[2], W1[2], W2[2], W3[2], c1x, c2x) : 0.f;
-                    const float dcdz = zp_ok ? (float)dplus(cm1, W1[2], cp1, cp2, c1z, c2z) : 0.f;
+                    const float dcdx = xp_ok ? (float)dpl<EX>(W0[2], W1[2], W2[2], W3[2], KX) : 0.f;
+                    const float dcdz = zp_ok ? (float)dpl<EX>(cm1, W1[2], cp1, cp2, KZ) : 0.f;
                     const double dtg = dt * (double)cur.gam;
-                    s_out[4][kq] = (float)((double)S0[4] + dt2 * (double)cur.nu
-                                           * (-(double)W1[2] - 0.5 * (double)sub_rn(dvzdx, dvxdz)));
-                    s_out[3][kq] = (float)((double)S0[3] + dtg * (double)dcdx);
-                    s_out[5][kq] = (float)((double)S0[5] + dtg * (double)dcdz);
+                    s_out[4][kq] = upd_srot<EX>(S0[4], dt2, cur.nu, W1[2], sub_rn(dvzdx, dvxdz));
+                    s_out[3][kq] = upd_sadd<EX>(S0[3], dtg, (double)dcdx);
+                    s_out[5][kq] = upd_sadd<EX>(S0[5], dtg, (double)dcdz);
                 }
             }
         }
@@ -517,7 +654,7 @@ __device__ __forceinline__ void load3(Row3<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
     }
 }
 
-template <int P>
+template <int P, bool EX>
 __global__ __launch_bounds__(256) void k_step3(StepArgs a)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
@@ -540,7 +677,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
     const int NP4 = (int)(d.Npad * 4);  // bytes per field
     const double dt = a.dt, dt2 = 2.0 * a.dt;
-    const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
+    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
     const int rd = a.rd, wr = a.rd ^ 1;
     const double bzb = d.bzb[jc], bzt = d.bzt[jc];
     const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
@@ -606,26 +743,26 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             float dsv[NV];
             if constexpr (P == SP_SH) {
                 float zm2 = shfl(Sc[1], L - 2), zm1 = shfl(Sc[1], L - 1), zp1 = shfl(Sc[1], L + 1);
-                double ddx = xm_ok ? dminus(Sa[0], Sb[0], Sc[0], Sd[0], c1x, c2x) : 0.0;
-                double ddz = zm_ok ? dminus(zm2, zm1, Sc[1], zp1, c1z, c2z) : 0.0;
+                double ddx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
+                double ddz = zm_ok ? dmn<EX>(zm2, zm1, Sc[1], zp1, KZ) : 0.0;
                 dsv[0] = (float)(ddx + ddz);
             } else {
                 float xzm2 = shfl(Sc[2], L - 2), xzm1 = shfl(Sc[2], L - 1), xzp1 = shfl(Sc[2], L + 1);
                 float zzm2 = shfl(Sc[1], L - 2), zzm1 = shfl(Sc[1], L - 1), zzp1 = shfl(Sc[1], L + 1);
-                double dxx = xm_ok ? dminus(Sa[0], Sb[0], Sc[0], Sd[0], c1x, c2x) : 0.0;
-                double dxxz = xm_ok ? dminus(Sa[2], Sb[2], Sc[2], Sd[2], c1x, c2x) : 0.0;
-                double dzxz = zm_ok ? dminus(xzm2, xzm1, Sc[2], xzp1, c1z, c2z) : 0.0;
-                double dzzz = zm_ok ? dminus(zzm2, zzm1, Sc[1], zzp1, c1z, c2z) : 0.0;
+                double dxx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
+                double dxxz = xm_ok ? dmn<EX>(Sa[2], Sb[2], Sc[2], Sd[2], KX) : 0.0;
+                double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, Sc[2], xzp1, KZ) : 0.0;
+                double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, Sc[1], zzp1, KZ) : 0.0;
                 dsv[0] = (float)(dxx + dzxz);
                 dsv[1] = (float)(dxxz + dzzz);
                 if constexpr (P == SP_SPIN) {
                     float yzm2 = shfl(Sc[5], L - 2), yzm1 = shfl(Sc[5], L - 1), yzp1 = shfl(Sc[5], L + 1);
                     float yym2 = shfl(Sc[4], L - 2), yym1 = shfl(Sc[4], L - 1), yyp1 = shfl(Sc[4], L + 1);
-                    double dxyx = xm_ok ? dminus(Sa[3], Sb[3], Sc[3], Sd[3], c1x, c2x) : 0.0;
-                    double dzyz = zm_ok ? dminus(yzm2, yzm1, Sc[5], yzp1, c1z, c2z) : 0.0;
+                    double dxyx = xm_ok ? dmn<EX>(Sa[3], Sb[3], Sc[3], Sd[3], KX) : 0.0;
+                    double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, Sc[5], yzp1, KZ) : 0.0;
                     float dsy = (float)(dxyx + dzyz);
-                    double dzyy = zm_ok ? dminus(yym2, yym1, Sc[4], yyp1, c1z, c2z) : 0.0;
-                    double dxyy = xm_ok ? dminus(Sa[4], Sb[4], Sc[4], Sd[4], c1x, c2x) : 0.0;
+                    double dzyy = zm_ok ? dmn<EX>(yym2, yym1, Sc[4], yyp1, KZ) : 0.0;
+                    double dxyy = xm_ok ? dmn<EX>(Sa[4], Sb[4], Sc[4], Sd[4], KX) : 0.0;
                     dsv[0] = (float)((double)dsv[0] + dzyy);
                     dsv[1] = (float)((double)dsv[1] - dxyy);
                     dsv[2] = (float)((double)dsy + 2.0 * (double)Sc[4]);
@@ -651,17 +788,34 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             }
             const float b1 = (float)((double)cur.bx * bzb);
             const double b = (double)(float)((double)b1 * bzt);
-            const double dtr = dt * rcp64((double)cur.rho);
             float vn[NV], un[NV];
+            if constexpr (EX) {
+                const double rr = (double)cur.rho, irr = rcp_rn(rr);
 #pragma unroll
-            for (int f = 0; f < NV; f++) {
-                double dtm = dtr;
-                if constexpr (P == SP_SPIN) {
-                    if (f == 2) dtm = dt * rcp64((double)cur.jj);
+                for (int f = 0; f < NV; f++) {
+                    double rf = rr, ir = irr;
+                    if constexpr (P == SP_SPIN) {
+                        if (f == 2) {
+                            rf = (double)cur.jj;
+                            ir = rcp_rn(rf);
+                        }
+                    }
+                    const float v = vel_x(cur.v[f], dsv[f], dt, rf, ir, b);
+                    vn[f] = (rin && jin) ? v : 0.f;
+                    un[f] = disp_x(cur.u[f], vn[f], dt);
                 }
-                const float v = (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm));
-                vn[f] = (rin && jin) ? v : 0.f;
-                un[f] = (float)((double)cur.u[f] + (double)vn[f] * dt);
+            } else {
+                const double dtr = dt * rcp64((double)cur.rho);
+#pragma unroll
+                for (int f = 0; f < NV; f++) {
+                    double dtm = dtr;
+                    if constexpr (P == SP_SPIN) {
+                        if (f == 2) dtm = dt * rcp64((double)cur.jj);
+                    }
+                    const float v = (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm));
+                    vn[f] = (rin && jin) ? v : 0.f;
+                    un[f] = (float)((double)cur.u[f] + (double)vn[f] * dt);
+                }
             }
             {
                 const unsigned ov = (outl && r >= i0 && r < i1) ? (unsigned)(r * nz + j) * 4u : OOB;
@@ -684,36 +838,31 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             const unsigned oq = (outl && qr >= i0 && qr < i1) ? (unsigned)(qr * nz + j) * 4u : OOB;
             if constexpr (P == SP_SH) {
                 float zm1 = shfl(Wb[0], L - 1), zp1 = shfl(Wb[0], L + 1), zp2 = shfl(Wb[0], L + 2);
-                const float dvdx = xp_ok ? (float)dplus(Wa[0], Wb[0], Wc[0], Wd[0], c1x, c2x) : 0.f;
-                const float dvdz = zp_ok ? (float)dplus(zm1, Wb[0], zp1, zp2, c1z, c2z) : 0.f;
+                const float dvdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
+                const float dvdz = zp_ok ? (float)dpl<EX>(zm1, Wb[0], zp1, zp2, KZ) : 0.f;
                 const double dtmu = dt * (double)cur.mu;
-                bst(RS, oq, slot_s(P, wr, 0) * NP4, (float)((double)Sa[0] + dtmu * (double)dvdx));
-                bst(RS, oq, slot_s(P, wr, 1) * NP4, (float)((double)Sa[1] + dtmu * (double)dvdz));
+                bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_sadd<EX>(Sa[0], dtmu, (double)dvdx));
+                bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_sadd<EX>(Sa[1], dtmu, (double)dvdz));
             } else {
                 float xm1 = shfl(Wb[0], L - 1), xp1 = shfl(Wb[0], L + 1), xp2 = shfl(Wb[0], L + 2);
                 float zm1 = shfl(Wb[1], L - 1), zp1 = shfl(Wb[1], L + 1), zp2 = shfl(Wb[1], L + 2);
-                const float dvxdx = xp_ok ? (float)dplus(Wa[0], Wb[0], Wc[0], Wd[0], c1x, c2x) : 0.f;
-                const float dvzdx = xp_ok ? (float)dplus(Wa[1], Wb[1], Wc[1], Wd[1], c1x, c2x) : 0.f;
-                const float dvxdz = zp_ok ? (float)dplus(xm1, Wb[0], xp1, xp2, c1z, c2z) : 0.f;
-                const float dvzdz = zp_ok ? (float)dplus(zm1, Wb[1], zp1, zp2, c1z, c2z) : 0.f;
+                const float dvxdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
+                const float dvzdx = xp_ok ? (float)dpl<EX>(Wa[1], Wb[1], Wc[1], Wd[1], KX) : 0.f;
+                const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, Wb[0], xp1, xp2, KZ) : 0.f;
+                const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, Wb[1], zp1, zp2, KZ) : 0.f;
                 const float lam = cur.lam, mu = cur.mu;
                 const double l2m = (double)lam + 2.0 * (double)mu;
-                bst(RS, oq, slot_s(P, wr, 0) * NP4,
-                    (float)((double)Sa[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz))));
-                bst(RS, oq, slot_s(P, wr, 1) * NP4,
-                    (float)((double)Sa[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx))));
-                bst(RS, oq, slot_s(P, wr, 2) * NP4,
-                    (float)((double)Sa[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx))));
+                bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_snorm<EX>(Sa[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz)));
+                bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_snorm<EX>(Sa[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx)));
+                bst(RS, oq, slot_s(P, wr, 2) * NP4, upd_sadd<EX>(Sa[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx))));
                 if constexpr (P == SP_SPIN) {
                     float cm1 = shfl(Wb[2], L - 1), cp1 = shfl(Wb[2], L + 1), cp2 = shfl(Wb[2], L + 2);
-                    const float dcdx = xp_ok ? (float)dplus(Wa[2], Wb[2], Wc[2], Wd[2], c1x, c2x) : 0.f;
-                    const float dcdz = zp_ok ? (float)dplus(cm1, Wb[2], cp1, cp2, c1z, c2z) : 0.f;
+                    const float dcdx = xp_ok ? (float)dpl<EX>(Wa[2], Wb[2], Wc[2], Wd[2], KX) : 0.f;
+                    const float dcdz = zp_ok ? (float)dpl<EX>(cm1, Wb[2], cp1, cp2, KZ) : 0.f;
                     const double dtg = dt * (double)cur.gam;
-                    bst(RS, oq, slot_s(P, wr, 4) * NP4,
-                        (float)((double)Sa[4] + dt2 * (double)cur.nu
-                                * (-(double)Wb[2] - 0.5 * (double)sub_rn(dvzdx, dvxdz))));
-                    bst(RS, oq, slot_s(P, wr, 3) * NP4, (float)((double)Sa[3] + dtg * (double)dcdx));
-                    bst(RS, oq, slot_s(P, wr, 5) * NP4, (float)((double)Sa[5] + dtg * (double)dcdz));
+                    bst(RS, oq, slot_s(P, wr, 4) * NP4, upd_srot<EX>(Sa[4], dt2, cur.nu, Wb[2], sub_rn(dvzdx, dvxdz)));
+                    bst(RS, oq, slot_s(P, wr, 3) * NP4, upd_sadd<EX>(Sa[3], dtg, (double)dcdx));
+                    bst(RS, oq, slot_s(P, wr, 5) * NP4, upd_sadd<EX>(Sa[5], dtg, (double)dcdz));
                 }
             }
         }
@@ -789,23 +938,23 @@ __device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
 
 // stress divergence of one velocity row (div_sy / div_sxz, solver.py:54-66)
 // from the stress window rows r-2..r+1 (sa..sd)
-template <int P>
+template <int P, bool EX>
 __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[n_s(P)], const float (&sb)[n_s(P)],
                                         const float (&sc)[n_s(P)], const float (&sd)[n_s(P)], bool xm_ok, bool zm_ok,
-                                        int L, double c1x, double c2x, double c1z, double c2z)
+                                        int L, const DCo &KX, const DCo &KZ)
 {
     if constexpr (P == SP_SH) {
         float zm2 = shfl(sc[1], L - 2), zm1 = shfl(sc[1], L - 1), zp1 = shfl(sc[1], L + 1);
-        double ddx = xm_ok ? dminus(sa[0], sb[0], sc[0], sd[0], c1x, c2x) : 0.0;
-        double ddz = zm_ok ? dminus(zm2, zm1, sc[1], zp1, c1z, c2z) : 0.0;
+        double ddx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
+        double ddz = zm_ok ? dmn<EX>(zm2, zm1, sc[1], zp1, KZ) : 0.0;
         dsv[0] = (float)(ddx + ddz);
     } else {
         float xzm2 = shfl(sc[2], L - 2), xzm1 = shfl(sc[2], L - 1), xzp1 = shfl(sc[2], L + 1);
         float zzm2 = shfl(sc[1], L - 2), zzm1 = shfl(sc[1], L - 1), zzp1 = shfl(sc[1], L + 1);
-        double dxx = xm_ok ? dminus(sa[0], sb[0], sc[0], sd[0], c1x, c2x) : 0.0;
-        double dxxz = xm_ok ? dminus(sa[2], sb[2], sc[2], sd[2], c1x, c2x) : 0.0;
-        double dzxz = zm_ok ? dminus(xzm2, xzm1, sc[2], xzp1, c1z, c2z) : 0.0;
-        double dzzz = zm_ok ? dminus(zzm2, zzm1, sc[1], zzp1, c1z, c2z) : 0.0;
+        double dxx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
+        double dxxz = xm_ok ? dmn<EX>(sa[2], sb[2], sc[2], sd[2], KX) : 0.0;
+        double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, sc[2], xzp1, KZ) : 0.0;
+        double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, sc[1], zzp1, KZ) : 0.0;
         dsv[0] = (float)(dxx + dzxz);
         dsv[1] = (float)(dxxz + dzzz);
     }
@@ -816,11 +965,25 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[
 // `zint`: every lane of the strip lies outside the top/bottom sponges; with
 // bx = 1 the sponge factor is exactly 1 and its arithmetic is skipped
 // (the same float64 result, bit for bit).
-template <int P>
+template <int P, bool EX>
 __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&dsv)[n_v(P)], const float (&vo)[n_v(P)],
                                            float rho, float bxr, double bzb, double bzt, bool zint, bool rin, bool jin,
                                            double dt)
 {
+    if constexpr (EX) {
+        double b = 1.0;
+        if (!(zint && bxr == 1.f)) {
+            const float b1 = (float)((double)bxr * bzb);
+            b = (double)(float)((double)b1 * bzt);
+        }
+        const double rr = (double)rho, irr = rcp_rn(rr);
+#pragma unroll
+        for (int f = 0; f < n_v(P); f++) {
+            const float v = vel_x(vo[f], dsv[f], dt, rr, irr, b);
+            vn[f] = (rin && jin) ? v : 0.f;
+        }
+        return;
+    }
     const double dtr = dt * rcp64((double)rho);
     if (zint && bxr == 1.f) {
 #pragma unroll
@@ -841,30 +1004,30 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
 
 // stress update of one row (add_sy / add_sxz, solver.py:120-148) from the
 // velocity window rows q-1..q+2 (wa..wd) and the old stresses
-template <int P>
+template <int P, bool EX>
 __device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n_v(P)], const float (&wb)[n_v(P)],
                                         const float (&wc)[n_v(P)], const float (&wd)[n_v(P)], const float (&so)[n_s(P)],
-                                        float lam, float mu, bool xp_ok, bool zp_ok, int L, double c1x, double c2x,
-                                        double c1z, double c2z, double dt)
+                                        float lam, float mu, bool xp_ok, bool zp_ok, int L, const DCo &KX, const DCo &KZ,
+                                        double dt)
 {
     if constexpr (P == SP_SH) {
         float zm1 = shfl(wb[0], L - 1), zp1 = shfl(wb[0], L + 1), zp2 = shfl(wb[0], L + 2);
-        const float dvdx = xp_ok ? (float)dplus(wa[0], wb[0], wc[0], wd[0], c1x, c2x) : 0.f;
-        const float dvdz = zp_ok ? (float)dplus(zm1, wb[0], zp1, zp2, c1z, c2z) : 0.f;
+        const float dvdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
+        const float dvdz = zp_ok ? (float)dpl<EX>(zm1, wb[0], zp1, zp2, KZ) : 0.f;
         const double dtmu = dt * (double)mu;
-        sn[0] = (float)((double)so[0] + dtmu * (double)dvdx);
-        sn[1] = (float)((double)so[1] + dtmu * (double)dvdz);
+        sn[0] = upd_sadd<EX>(so[0], dtmu, (double)dvdx);
+        sn[1] = upd_sadd<EX>(so[1], dtmu, (double)dvdz);
     } else {
         float xm1 = shfl(wb[0], L - 1), xp1 = shfl(wb[0], L + 1), xp2 = shfl(wb[0], L + 2);
         float zm1 = shfl(wb[1], L - 1), zp1 = shfl(wb[1], L + 1), zp2 = shfl(wb[1], L + 2);
-        const float dvxdx = xp_ok ? (float)dplus(wa[0], wb[0], wc[0], wd[0], c1x, c2x) : 0.f;
-        const float dvzdx = xp_ok ? (float)dplus(wa[1], wb[1], wc[1], wd[1], c1x, c2x) : 0.f;
-        const float dvxdz = zp_ok ? (float)dplus(xm1, wb[0], xp1, xp2, c1z, c2z) : 0.f;
-        const float dvzdz = zp_ok ? (float)dplus(zm1, wb[1], zp1, zp2, c1z, c2z) : 0.f;
+        const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
+        const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1], wb[1], wc[1], wd[1], KX) : 0.f;
+        const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, wb[0], xp1, xp2, KZ) : 0.f;
+        const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, wb[1], zp1, zp2, KZ) : 0.f;
         const double l2m = (double)lam + 2.0 * (double)mu;
-        sn[0] = (float)((double)so[0] + dt * (l2m * (double)dvxdx + (double)mul_rn(lam, dvzdz)));
-        sn[1] = (float)((double)so[1] + dt * (l2m * (double)dvzdz + (double)mul_rn(lam, dvxdx)));
-        sn[2] = (float)((double)so[2] + dt * (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+        sn[0] = upd_snorm<EX>(so[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
+        sn[1] = upd_snorm<EX>(so[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
+        sn[2] = upd_sadd<EX>(so[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
     }
 }
 
@@ -891,7 +1054,7 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)], int r, int j, int n
     src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
 }
 
-template <int P>
+template <int P, bool EX>
 __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 {
     static_assert(P == SP_SH || P == SP_PSV, "two-step kernel: SH and P-SV");
@@ -915,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
     const int NP4 = (int)(d.Npad * 4);
     const double dt = a.dt;
-    const double c1x = d.c1x, c2x = d.c2x, c1z = d.c1z, c2z = d.c2z;
+    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
     const int rd = a.rd, wr = a.rd ^ 1;
     const double bzb = d.bzb[jc], bzt = d.bzt[jc];
     const bool zint = __all(bzb == 1.0 && bzt == 1.0);
@@ -983,10 +1146,10 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int r = m - 1;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV], vn[NV];
-                vel_row<P>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
+                vel_row<P, EX>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, KX, KZ);
                 inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P>(vn, dsv, cur.v, cur.rhoA, bxr, bzb, bzt, zint, rin, jin, dt);
+                vel_finish<P, EX>(vn, dsv, cur.v, cur.rhoA, bxr, bzb, bzt, zint, rin, jin, dt);
 #pragma unroll
                 for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
             }
@@ -995,8 +1158,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int q = m - 3;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS];
-                str_row<P>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, xp_ok, zp_ok, L, c1x, c2x, c1z,
-                           c2z, dt);
+                str_row<P, EX>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, xp_ok, zp_ok, L, KX, KZ, dt);
 #pragma unroll
                 for (int f = 0; f < NS; f++) S1[A][f] = sn[f];
             }
@@ -1005,15 +1167,15 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int r = m - 4;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV], vn[NV];
-                vel_row<P>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, c1x, c2x, c1z, c2z);
+                vel_row<P, EX>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, KX, KZ);
                 inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P>(vn, dsv, W1[D], cur.rhoC, bxr, bzb, bzt, zint, rin, jin, dt);
+                vel_finish<P, EX>(vn, dsv, W1[D], cur.rhoC, bxr, bzb, bzt, zint, rin, jin, dt);
                 float u1[NV], u2[NV];
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
-                    u1[f] = (float)((double)cur.u[f] + (double)W1[D][f] * dt);
-                    u2[f] = (float)((double)u1[f] + (double)vn[f] * dt);
+                    u1[f] = upd_disp<EX>(cur.u[f], W1[D][f], dt);
+                    u2[f] = upd_disp<EX>(u1[f], vn[f], dt);
                     W2[D][f] = vn[f];
                 }
                 // rows outside the chunk: every lane's offset is out of range
@@ -1048,8 +1210,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int q = m - 6;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS];
-                str_row<P>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], cur.lamD, cur.muD, xp_ok, zp_ok, L, c1x, c2x, c1z,
-                           c2z, dt);
+                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], cur.lamD, cur.muD, xp_ok, zp_ok, L, KX, KZ, dt);
                 const bool qok = q >= i0 && q < i1;
                 const unsigned oq = qok ? vo : OOB;
                 const int orow = qok ? q * nz * 4 : 0;
@@ -1111,17 +1272,20 @@ struct InteractArgs {
     int cur;          // velocity set holding the adjoint velocity
     int isae;
     double ndt;
-    double c1x, c2x, c1z, c2z;
+    DCo KX, KZ;  // derivative constants (the exact split form is used)
 };
 
-__device__ __forceinline__ float dpx(const float *u, int i, long long k, int nx, int nz, double c1, double c2)
+// D+ in x and z in the reference's operation order (runs every sae steps,
+// so always in the exact form)
+__device__ __forceinline__ float dpx(const float *u, int i, long long k, int nx, int nz, const DCo &K)
 {
-    if (i >= 1 && i < nx - 2) return (float)dplus(u[k - nz], u[k], u[k + nz], u[k + 2 * (long long)nz], c1, c2);
+    if (i >= 1 && i < nx - 2)
+        return (float)dpl<true>(u[k - nz], u[k], u[k + nz], u[k + 2 * (long long)nz], K);
     return 0.f;
 }
-__device__ __forceinline__ float dpz(const float *u, int j, long long k, int nz, double c1, double c2)
+__device__ __forceinline__ float dpz(const float *u, int j, long long k, int nz, const DCo &K)
 {
-    if (j >= 1 && j < nz - 2) return (float)dplus(u[k - 1], u[k], u[k + 1], u[k + 2], c1, c2);
+    if (j >= 1 && j < nz - 2) return (float)dpl<true>(u[k - 1], u[k], u[k + 1], u[k + 2], K);
     return 0.f;
 }
 
@@ -1135,6 +1299,7 @@ __device__ __forceinline__ float dpz(const float *u, int j, long long k, int nz,
 template <int P>
 __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
 {
+#pragma clang fp contract(off)
     const long long N = (long long)a.nx * a.nz;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
@@ -1149,8 +1314,8 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
             const ShotDesc &d = a.shots[a.list[s]];
             const float *ua = d.state + slot_u(P, 0) * NP;
             const float *uf = d.snap + (long long)a.isae * n_snap(P) * NP;
-            const float ax = dpx(ua, i, k, nx, nz, a.c1x, a.c2x), fx = dpx(uf, i, k, nx, nz, a.c1x, a.c2x);
-            const float az = dpz(ua, j, k, nz, a.c1z, a.c2z), fz = dpz(uf, j, k, nz, a.c1z, a.c2z);
+            const float ax = dpx(ua, i, k, nx, nz, a.KX), fx = dpx(uf, i, k, nx, nz, a.KX);
+            const float az = dpz(ua, j, k, nz, a.KZ), fz = dpz(uf, j, k, nz, a.KZ);
             km = (float)((double)km + (double)add_rn(mul_rn(ax, fx), mul_rn(az, fz)) * a.ndt);
             const double vf = uf[NP + k], va = d.state[slot_v(P, a.cur, 0) * NP + k];
             kr = (double)(float)(kr + va * vf * a.ndt);
@@ -1174,10 +1339,10 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
         const float *uxa = d.state + slot_u(P, 0) * NP, *uza = d.state + slot_u(P, 1) * NP;
         const float *sn = d.snap + (long long)a.isae * n_snap(P) * NP;
         const float *uxf = sn, *uzf = sn + NP;
-        const double axx = dpx(uxa, i, k, nx, nz, a.c1x, a.c2x), axz = dpz(uxa, j, k, nz, a.c1z, a.c2z);
-        const double azx = dpx(uza, i, k, nx, nz, a.c1x, a.c2x), azz = dpz(uza, j, k, nz, a.c1z, a.c2z);
-        const double fxx = dpx(uxf, i, k, nx, nz, a.c1x, a.c2x), fxz = dpz(uxf, j, k, nz, a.c1z, a.c2z);
-        const double fzx = dpx(uzf, i, k, nx, nz, a.c1x, a.c2x), fzz = dpz(uzf, j, k, nz, a.c1z, a.c2z);
+        const double axx = dpx(uxa, i, k, nx, nz, a.KX), axz = dpz(uxa, j, k, nz, a.KZ);
+        const double azx = dpx(uza, i, k, nx, nz, a.KX), azz = dpz(uza, j, k, nz, a.KZ);
+        const double fxx = dpx(uxf, i, k, nx, nz, a.KX), fxz = dpz(uxf, j, k, nz, a.KZ);
+        const double fzx = dpx(uzf, i, k, nx, nz, a.KX), fzz = dpz(uzf, j, k, nz, a.KZ);
         const double vxa = d.state[slot_v(P, a.cur, 0) * NP + k], vza = d.state[slot_v(P, a.cur, 1) * NP + k];
         const double vxf = sn[(n_v(P) + 0) * NP + k], vzf = sn[(n_v(P) + 1) * NP + k];
         // the same expressions, in the same order, as the oracle
@@ -1193,8 +1358,8 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
             knu = (double)(float)(knu + 4.0 * ea * ef * a.ndt);
             const double vca = d.state[slot_v(P, a.cur, 2) * NP + k], vcf = sn[5 * NP + k];
             kj = (double)(float)(kj + vca * vcf * a.ndt);
-            const double cax = dpx(uca, i, k, nx, nz, a.c1x, a.c2x), caz = dpz(uca, j, k, nz, a.c1z, a.c2z);
-            const double cfx = dpx(ucf, i, k, nx, nz, a.c1x, a.c2x), cfz = dpz(ucf, j, k, nz, a.c1z, a.c2z);
+            const double cax = dpx(uca, i, k, nx, nz, a.KX), caz = dpz(uca, j, k, nz, a.KZ);
+            const double cfx = dpx(ucf, i, k, nx, nz, a.KX), cfz = dpz(ucf, j, k, nz, a.KZ);
             kc = (double)(float)(kc + (cax * cfx + caz * cfz) * a.ndt);
         }
     }
@@ -1217,8 +1382,8 @@ struct MisfitArgs {
     const int4 *triples;  // (shot, comp, rec, -)
     const float *const *data;  // per shot, [n_rec][nrec][nt]
     double *partial;
+    const double *taper;  // [nt], computed on the host with the C library's cos
     int n, nt;
-    double dt;
 };
 
 __global__ __launch_bounds__(256) void k_misfit(MisfitArgs a)
@@ -1232,15 +1397,9 @@ __global__ __launch_bounds__(256) void k_misfit(MisfitArgs a)
     const float *syn = a.data[t.x] + base;  // observed data ("syn" in fd2d.py:305)
     const float *obs = d.rec + base;         // current synthetics
     float *ad = d.adstf + base;
-    const double t_end = (double)(nt - 1) * a.dt;
-    const double tw = t_end / 10.0, t_min = tw, t_max = t_end - tw;
     double acc = 0.0;
     for (int it = threadIdx.x; it < nt; it += blockDim.x) {
-        const double tt = (double)it * a.dt;
-        double taper;
-        if (tt <= t_min) taper = 0.5 + 0.5 * cos(M_PI * (t_min - tt) / tw);
-        else if (tt >= t_max) taper = 0.5 + 0.5 * cos(M_PI * (t_max - tt) / tw);
-        else taper = 1.0;
+        const double taper = a.taper[it];
         const float res = (float)((double)sub_rn(syn[it], obs[it]) * taper);
         ad[nt - it - 1] = (float)((double)res * taper * 2.0);
         acc += (double)res * (double)res;
@@ -1260,6 +1419,7 @@ __global__ __launch_bounds__(256) void k_misfit(MisfitArgs a)
 // g[d] = exp(-d^2/(2 s^2)) / (sqrt(2 pi) s) for |i-n| = d, in float64.
 __global__ __launch_bounds__(256) void k_smooth_x(const float *data, float *gtmp, const double *g, int nx, int nz)
 {
+#pragma clang fp contract(off)
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)nx * nz) return;
     const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
@@ -1271,6 +1431,7 @@ __global__ __launch_bounds__(256) void k_smooth_x(const float *data, float *gtmp
 __global__ __launch_bounds__(256) void k_smooth_z(float *data, const float *gtmp, const double *g,
                                                   const double *sumx, const double *sumz, int nx, int nz)
 {
+#pragma clang fp contract(off)
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)nx * nz) return;
     const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
@@ -1310,6 +1471,7 @@ __global__ void k_gamma(float *gam, const float *muc, const float *nuc, long lon
 // mu = rho vs^2 (the parameterisation of vps2lm).
 __global__ void k_kern_vps(float *K, const float *mat, long long n, long long NP)
 {
+#pragma clang fp contract(off)
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double rho = mat[M_RHO * NP + k], lam = mat[M_LAM * NP + k], mu = mat[M_MU * NP + k];
@@ -1359,6 +1521,7 @@ struct sp_ctx {
     int cur_kernel = 1;        // kernel of the prepared batch
     int shift = 3, wout = 58;  // strip geometry of the step kernel
     int fuse = 1;              // two steps per launch where nothing observes the state between them
+    int exact = 1;             // step kernels follow the reference's float64 operations exactly
     int *d_counter = nullptr;  // step counter read by the kernels of a replayed graph
     const int *itp = nullptr;  // set while a graph is captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
@@ -1451,6 +1614,52 @@ size_t snap_bytes(const sp_ctx *c, const Shot &s, int sae)
     return (size_t)(c->nt / sae) * n_snap(c->physics) * (size_t)c->grids[s.grid].Npad * sizeof(float);
 }
 
+// num/den as hi + lo: hi = RN(num/den), lo = RN((num - hi den)/den), the
+// remainder num - hi den being exact by FMA
+void split_quotient(double num, double den, double &hi, double &lo)
+{
+    hi = num / den;
+    lo = std::fma(-hi, den, num) / den;
+}
+
+// Exact mode multiplies by the split constant, fma(x, hi, x lo), where the
+// reference divides, RN(num x / den).  For float32 x and a divisor 8h or 24h
+// of float32 h, num x / den is an integer of at most 28 bits over an odd
+// integer below 2^26: never a float64 rounding midpoint and at least 2^-80
+// (relative) from each, while the split product is within ~2^-105 of it, so
+// both round alike.  The identity is still checked here once per constant
+// over all 2^23 float32 mantissas (the exponent of x only scales both sides).
+bool split_quotient_exact(double num, double den)
+{
+    static std::mutex mu;
+    static std::map<std::pair<double, double>, bool> known;
+    std::lock_guard<std::mutex> lock(mu);
+    auto f = known.find({num, den});
+    if (f != known.end()) return f->second;
+    double hi, lo;
+    split_quotient(num, den, hi, lo);
+    bool ok = std::isfinite(hi) && std::isfinite(lo);
+    for (long m = 1L << 23; ok && m < (1L << 24); m++) {
+        const double x = (double)m;
+        ok = std::fma(x, hi, x * lo) == (num * x) / den;
+    }
+    known[{num, den}] = ok;
+    return ok;
+}
+
+int exact_mode_ok(sp_ctx *c, const std::vector<int> &batch)
+{
+    if (!c->exact) return 0;
+    for (int s : batch) {
+        const Grid &g = c->grids[c->shots[s].grid];
+        for (double h : {(double)g.dx, (double)g.dz})
+            if (!split_quotient_exact(9.0, 8.0 * h) || !split_quotient_exact(1.0, 24.0 * h))
+                return fail(c, "exact mode: the derivative constants of grid spacing " + std::to_string(h) +
+                                   " do not reproduce the reference's divisions; use the default mode");
+    }
+    return 0;
+}
+
 ShotDesc make_desc(const sp_ctx *c, const Shot &s)
 {
     const Grid &g = c->grids[s.grid];
@@ -1477,6 +1686,10 @@ ShotDesc make_desc(const sp_ctx *c, const Shot &s)
     d.c2x = 1.0 / (24.0 * (double)g.dx);
     d.c1z = 9.0 / (8.0 * (double)g.dz);
     d.c2z = 1.0 / (24.0 * (double)g.dz);
+    split_quotient(9.0, 8.0 * (double)g.dx, d.k9hx, d.k9lx);
+    split_quotient(1.0, 24.0 * (double)g.dx, d.k1hx, d.k1lx);
+    split_quotient(9.0, 8.0 * (double)g.dz, d.k9hz, d.k9lz);
+    split_quotient(1.0, 24.0 * (double)g.dz, d.k1hz, d.k1lz);
     return d;
 }
 
@@ -1534,6 +1747,7 @@ int auto_chunk(const sp_ctx *c, const std::vector<int> &batch)
 // Upload descriptors and the wavefront work list of a batch.
 int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
 {
+    if (exact_mode_ok(c, batch)) return 2;
     std::vector<ShotDesc> desc;
     for (int s : batch) desc.push_back(make_desc(c, c->shots[s]));
     const int chunk = auto_chunk(c, batch);
@@ -1643,23 +1857,48 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     a.shift = strip_shift(c);
     a.wout = strip_width(c);
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
+    const bool ex = c->exact;
     if (nsteps == 2) {
         if (!c->cur_fuse) return fail(c, "internal: two-step launch without the two-step geometry");
         switch (c->physics) {
-        case SP_SH: hipLaunchKernelGGL(k_step2<SP_SH>, grid, block, 0, c->stream, a); break;
-        default: hipLaunchKernelGGL(k_step2<SP_PSV>, grid, block, 0, c->stream, a); break;
+        case SP_SH:
+            if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_SH, false>), grid, block, 0, c->stream, a);
+            break;
+        default:
+            if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_PSV, false>), grid, block, 0, c->stream, a);
+            break;
         }
     } else if (c->cur_kernel == 1) {
         switch (c->physics) {
-        case SP_SH: hipLaunchKernelGGL(k_step3<SP_SH>, grid, block, 0, c->stream, a); break;
-        case SP_PSV: hipLaunchKernelGGL(k_step3<SP_PSV>, grid, block, 0, c->stream, a); break;
-        default: hipLaunchKernelGGL(k_step3<SP_SPIN>, grid, block, 0, c->stream, a); break;
+        case SP_SH:
+            if (ex) hipLaunchKernelGGL((k_step3<SP_SH, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step3<SP_SH, false>), grid, block, 0, c->stream, a);
+            break;
+        case SP_PSV:
+            if (ex) hipLaunchKernelGGL((k_step3<SP_PSV, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step3<SP_PSV, false>), grid, block, 0, c->stream, a);
+            break;
+        default:
+            if (ex) hipLaunchKernelGGL((k_step3<SP_SPIN, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step3<SP_SPIN, false>), grid, block, 0, c->stream, a);
+            break;
         }
     } else {
         switch (c->physics) {
-        case SP_SH: hipLaunchKernelGGL(k_step<SP_SH>, grid, block, 0, c->stream, a); break;
-        case SP_PSV: hipLaunchKernelGGL(k_step<SP_PSV>, grid, block, 0, c->stream, a); break;
-        default: hipLaunchKernelGGL(k_step<SP_SPIN>, grid, block, 0, c->stream, a); break;
+        case SP_SH:
+            if (ex) hipLaunchKernelGGL((k_step<SP_SH, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step<SP_SH, false>), grid, block, 0, c->stream, a);
+            break;
+        case SP_PSV:
+            if (ex) hipLaunchKernelGGL((k_step<SP_PSV, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step<SP_PSV, false>), grid, block, 0, c->stream, a);
+            break;
+        default:
+            if (ex) hipLaunchKernelGGL((k_step<SP_SPIN, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step<SP_SPIN, false>), grid, block, 0, c->stream, a);
+            break;
         }
     }
     HIPCHECK(c, hipGetLastError());
@@ -1838,9 +2077,23 @@ int run_misfit_batch(sp_ctx *c, const std::vector<int> &batch, double *norms_out
             for (int cc = 0; cc < ncm; cc++) norms_out[(size_t)batch[b] * ncm + cc] = 0.0;
         return 0;
     }
+    // cosine taper of the first and last tenth of the trace (waveform.py:5-25)
+    std::vector<double> taper(c->nt);
+    {
+        const double t_end = (double)(c->nt - 1) * c->dt;
+        const double tw = t_end / 10.0, t_min = tw, t_max = t_end - tw;
+        for (int it = 0; it < c->nt; it++) {
+            const double tt = (double)it * c->dt;
+            if (tt <= t_min) taper[it] = 0.5 + 0.5 * std::cos(M_PI * (t_min - tt) / tw);
+            else if (tt >= t_max) taper[it] = 0.5 + 0.5 * std::cos(M_PI * (t_max - tt) / tw);
+            else taper[it] = 1.0;
+        }
+    }
     int4 *d_trip = nullptr;
     const float **d_data = nullptr;
-    double *d_part = nullptr;
+    double *d_part = nullptr, *d_taper = nullptr;
+    HIPCHECK(c, hipMalloc(&d_taper, taper.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpyAsync(d_taper, taper.data(), taper.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMalloc(&d_trip, trip.size() * sizeof(int4)));
     HIPCHECK(c, hipMalloc(&d_data, data.size() * sizeof(float *)));
     HIPCHECK(c, hipMalloc(&d_part, trip.size() * sizeof(double)));
@@ -1851,9 +2104,9 @@ int run_misfit_batch(sp_ctx *c, const std::vector<int> &batch, double *norms_out
     a.triples = d_trip;
     a.data = d_data;
     a.partial = d_part;
+    a.taper = d_taper;
     a.n = (int)trip.size();
     a.nt = c->nt;
-    a.dt = c->dt;
     hipLaunchKernelGGL(k_misfit, dim3((unsigned)trip.size()), dim3(256), 0, c->stream, a);
     HIPCHECK(c, hipGetLastError());
     std::vector<double> part(trip.size());
@@ -1862,6 +2115,7 @@ int run_misfit_batch(sp_ctx *c, const std::vector<int> &batch, double *norms_out
     HIPCHECK(c, hipFree(d_trip));
     HIPCHECK(c, hipFree(d_data));
     HIPCHECK(c, hipFree(d_part));
+    HIPCHECK(c, hipFree(d_taper));
     size_t p = 0;
     for (int b = 0; b < (int)batch.size(); b++) {
         const Shot &s = c->shots[batch[b]];
@@ -1913,10 +2167,14 @@ int launch_interact(sp_ctx *c, const InteractPlan &pl, int cur, int isae, double
         a.cur = cur;
         a.isae = isae;
         a.ndt = ndt;
-        a.c1x = 9.0 / (8.0 * (double)G.dx);
-        a.c2x = 1.0 / (24.0 * (double)G.dx);
-        a.c1z = 9.0 / (8.0 * (double)G.dz);
-        a.c2z = 1.0 / (24.0 * (double)G.dz);
+        a.KX.c1 = 9.0 / (8.0 * (double)G.dx);
+        a.KX.c2 = 1.0 / (24.0 * (double)G.dx);
+        a.KZ.c1 = 9.0 / (8.0 * (double)G.dz);
+        a.KZ.c2 = 1.0 / (24.0 * (double)G.dz);
+        split_quotient(9.0, 8.0 * (double)G.dx, a.KX.k9h, a.KX.k9l);
+        split_quotient(1.0, 24.0 * (double)G.dx, a.KX.k1h, a.KX.k1l);
+        split_quotient(9.0, 8.0 * (double)G.dz, a.KZ.k9h, a.KZ.k9l);
+        split_quotient(1.0, 24.0 * (double)G.dz, a.KZ.k1h, a.KZ.k1l);
         dim3 gr((unsigned)((G.N + 255) / 256)), bl(256);
         switch (c->physics) {
         case SP_SH: hipLaunchKernelGGL(k_interact<SP_SH>, gr, bl, 0, c->stream, a); break;
@@ -2307,6 +2565,7 @@ int sp_set_option(sp_ctx *c, int key, long long v)
     case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;  // forward runs without snapshots, own or set stream
     case SP_OPT_KERNEL: c->kernel = v ? 1 : 0; return 0;
     case SP_OPT_FUSE: c->fuse = v ? 1 : 0; return 0;
+    case SP_OPT_EXACT: c->exact = v ? 1 : 0; return 0;
     case SP_OPT_STRIP: {
         const int sh = (int)(v >> 16), wo = (int)(v & 0xffff);
         if (sh < 3 || wo < 1 || sh + wo > 61) return fail(c, "strip geometry needs shift >= 3 and shift + width <= 61");
@@ -2806,9 +3065,9 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
     std::vector<int> all(c->shots.size());
     std::iota(all.begin(), all.end(), 0);
     // results are kept per process for the batch's shape: physics, kernel
-    // choice and every shot's grid
+    // choice, arithmetic mode and every shot's grid
     std::string key = std::to_string(c->physics) + "/" + std::to_string(c->kernel) + "/" + std::to_string(c->fuse) +
-                      "/" + std::to_string(steps);
+                      "/" + std::to_string(c->exact) + "/" + std::to_string(steps);
     for (const Shot &sh : c->shots)
         key += ":" + std::to_string(c->grids[sh.grid].nx) + "x" + std::to_string(c->grids[sh.grid].nz);
     static std::mutex mu;
@@ -2885,6 +3144,7 @@ int sp_get_option(const sp_ctx *c, int key, long long *value)
     case SP_OPT_KERNEL: *value = c->kernel; return 0;
     case SP_OPT_STRIP: *value = ((long long)c->shift << 16) | c->wout; return 0;
     case SP_OPT_FUSE: *value = c->fuse; return 0;
+    case SP_OPT_EXACT: *value = c->exact; return 0;
     default: return 1;
     }
 }

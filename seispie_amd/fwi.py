@@ -55,7 +55,7 @@ class _Stream:
 class VpSolver:
     """Batched P-SV solver inverting vp (vs = vp / sqrt(3), rho fixed)."""
 
-    def __init__(self, prob, data, sigma=5, device=0, chunk=0):
+    def __init__(self, prob, data, sigma=5, device=0, chunk=0, autotune=True):
         if prob.physics != _lib.PSV or len(prob.grids) != 1 or prob.grids[0].param != 0:
             raise ValueError("VpSolver needs a one-grid P-SV problem in vp/vs/rho")
         g = prob.grids[0]
@@ -69,6 +69,9 @@ class VpSolver:
         self.ids = configs.load_problem(self.ctx, prob)
         for i, d in zip(self.ids, data):
             self.ctx.set_data(i, d)
+        if autotune and not chunk:
+            # work decomposition for this batch shape (cached per process)
+            self.ctx.autotune(20)
         self.vp = np.ascontiguousarray(g.fields[0], np.float32).copy()
         self.rho = np.ascontiguousarray(g.fields[2], np.float32).copy()
         self.mu = _Field(self)

@@ -22,8 +22,10 @@ from seispie_amd import _lib, configs
 pytestmark = pytest.mark.gpu
 
 
-def _records(ctx_factory, prob, scale=1.0):
+def _records(ctx_factory, prob, scale=1.0, **opts):
     ctx = ctx_factory()
+    for k, v in opts.items():
+        ctx.set_option(getattr(_lib, "OPT_" + k.upper()), v)
     for g in prob.grids:
         for s in g.shots:
             s.stf = (s.stf * np.float32(scale)).astype(np.float32)
@@ -63,3 +65,25 @@ def test_records_are_exactly_linear_in_the_source_at_full_size(ctx_factory):
         worst = max(worst, np.linalg.norm(b - a2) / np.linalg.norm(a2))
     print("worst rel L2 of records(2 stf) - 2 records(stf): %.2e" % worst)
     assert worst <= 1e-4
+
+
+def test_one_shot_of_the_benchmark_matches_the_oracle(ctx_factory):
+    """Shot 0 of the 128-shot 1000^2 workload over all 4000 steps against
+    the oracle (float32 storage, float64 arithmetic; ~15 s of host time).
+    The default exact mode must equal it bit for bit.  The fast mode
+    (reciprocal multiplications and fused multiply-adds) is held to twice
+    the seismogram tolerance; its float32 rounding differences grow chaotically with the
+    number of steps, to ~1e-5 here, the size of the gap between the
+    oracle's own float32 and float64 modes (5.3e-6)."""
+    from tests.helpers import oracle_records, rel_l2
+    prob = configs.psv_config3()
+    g = prob.grids[0]
+    print("case: full-size shot 0 vs oracle", flush=True)
+    want = oracle_records(prob, g, g.shots[0])[0]
+    assert np.abs(want).max() > 0
+    exact = _records(ctx_factory, _single(prob, g, g.shots[0]))[0]
+    np.testing.assert_array_equal(exact, want)
+    fast = _records(ctx_factory, _single(prob, g, g.shots[0]), exact=0)[0]
+    err = rel_l2(fast, want)
+    print("shot 0, fast mode: rel L2 %.2e" % err)
+    assert err <= 2e-5
