@@ -244,6 +244,51 @@ def test_one_shot_gradient_equals_oracle_bit_for_bit(ctx_factory, physics):
                                       ((vp * vp - 2.0 * vs * vs) * kl + vs * vs * km + kr).astype(np.float32))
 
 
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+def test_against_the_float64_replay(ctx_factory, physics):
+    """The GPU path (float32 storage, the reference's arithmetic) against
+    the oracle's float64 replay of the same stencils: seismograms within
+    1e-5, lambda/mu/rho and vp/vs/rho kernels within 1e-4 (the spin-only
+    kernels, whose fields nearly cancel, within 5e-4)."""
+    prob = _small(physics, nt=200, nshots=2)
+    prob.sae = 10
+    data = _data_for(prob)
+    print("case: float64 replay", physics, flush=True)
+    _, recs = _run_forward(ctx_factory, prob)
+    s = 0
+    for g in prob.grids:
+        for shot in g.shots:
+            ref64 = oracle_records(prob, g, shot, precision="f64")[0]
+            err = rel_l2(recs[s], ref64)
+            print("shot", s, "records vs float64 rel L2 %.2e" % err)
+            assert err <= SEIS_TOL
+            s += 1
+    ctx = ctx_factory()
+    ids = configs.load_problem(ctx, prob)
+    for i, d in zip(ids, data):
+        ctx.set_data(i, d)
+    ctx.gradient(prob.sae)
+    kref, _ = oracle_gradient(prob, data, precision="f64")
+    names = [("mu", _lib.K_MU), ("rho", _lib.K_RHO)] if physics == _lib.SH else \
+        [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO)]
+    if physics == _lib.SPIN:
+        names += [("nu", _lib.K_NU), ("j", _lib.K_J), ("muc", _lib.K_MUC)]
+    for nm, w in names:
+        err = rel_l2(ctx.kernel(0, w), kref[0][nm])
+        print(nm, "kernel vs float64 rel L2 %.2e" % err)
+        assert err <= (KERN_TOL if nm in ("lam", "mu", "rho") else 5e-4), (nm, err)
+    if physics != _lib.SH:
+        ctx.kernels_vps(0)
+        lam, mu, rho = (ctx.get_field(0, f).astype(np.float64) for f in (_lib.F_LAM, _lib.F_MU, _lib.F_RHO))
+        vp, vs = np.sqrt((lam + 2 * mu) / rho), np.sqrt(mu / rho)
+        kl, km, kr = (kref[0][k] for k in ("lam", "mu", "rho"))
+        for w, want in ((_lib.K_VP, 2 * rho * vp * kl), (_lib.K_VS, 2 * rho * vs * (km - 2 * kl)),
+                        (_lib.K_RHOP, (vp ** 2 - 2 * vs ** 2) * kl + vs ** 2 * km + kr)):
+            err = rel_l2(ctx.kernel(0, w), want)
+            print("vps kernel", w, "vs float64 rel L2 %.2e" % err)
+            assert err <= KERN_TOL
+
+
 def test_forward_then_adjoint_equals_gradient_and_sub_batches(ctx_factory):
     """sp_forward(sae) + sp_misfit + sp_adjoint == sp_gradient, and a
     snapshot budget that forces one shot per sub-batch gives the same."""
