@@ -52,7 +52,7 @@ class Simulation:
     [nrec]) with grid indices k = i*nz + j; every shot has the same nrec."""
 
     def __init__(self, physics, nx, nz, dx, dz, nt, dt, shots, sponge=None, interval=None, sae=1, device=0,
-                 init_model=None):
+                 init_model=None, autotune=True):
         if physics not in (_lib.SH, _lib.PSV):
             raise ValueError("autograd covers SH and P-SV physics")
         if nt % sae:
@@ -77,6 +77,10 @@ class Simulation:
         self.nshot = len(shots)
         for src_k, stf, rec_k in shots:
             self.ctx.add_shot(self.grid, src_k, stf, rec_k)
+        if autotune:
+            # work decomposition and steps per launch for this batch shape
+            # (cached per process); results do not depend on the choice
+            self.ctx.autotune(20)
         self.ncomp = _lib.N_REC[physics]
         self.nmis = _lib.N_MIS[physics]
         self._gen = 0      # forward runs so far: backward re-runs the forward if another came between
