@@ -224,10 +224,13 @@ class fd2d(base):
         return np.stack([pick(self.stf_x), pick(self.stf_z)])
 
     def _load_shots(self, physics, ctx, shots):
-        """shots: list of source-index lists, one simulation each."""
+        """shots: list of source-index lists, one simulation each.  The work
+        decomposition is autotuned once per batch shape (the library caches
+        the choice per process; results do not depend on it)."""
         ctx.clear_shots()
         for ids in shots:
             ctx.add_shot(self._grid, self.src_id[list(ids)], self._stf(physics, ids), self.rec_id)
+        ctx.autotune(20)
 
     def _isrc(self):
         return -1 if self.config['combine_sources'] == 'yes' else self.taskid
