@@ -87,3 +87,20 @@ def test_one_shot_of_the_benchmark_matches_the_oracle(ctx_factory):
     err = rel_l2(fast, want)
     print("shot 0, fast mode: rel L2 %.2e" % err)
     assert err <= 2e-5
+
+
+@pytest.mark.parametrize("name", ["2", "3b", "4", "5"])
+def test_one_shot_per_grid_of_each_workload_equals_the_oracle(ctx_factory, name):
+    """At the other benchmark workloads' full sizes and lengths (P-SV 400^2
+    x 3000 steps; the mixed 300x600 / 500x500 / 800x400 grids x 4000; spin
+    600^2 x 3000; the FWI model 500^2 x 3000), the first shot of every grid
+    run alone records exactly what the oracle records."""
+    from tests.helpers import oracle_records
+    prob = {"2": configs.psv_config2, "3b": configs.psv_config3_mixed, "4": configs.spin_config4,
+            "5": configs.fwi_config5}[name]()
+    for g in prob.grids:
+        print("case: workload", name, "grid %dx%d" % (g.nx, g.nz), "nt", prob.nt, flush=True)
+        want = oracle_records(prob, g, g.shots[0])[0]
+        assert np.abs(want).max() > 0
+        got = _records(ctx_factory, _single(prob, g, g.shots[0]))[0]
+        np.testing.assert_array_equal(got, want)
