@@ -10,8 +10,8 @@ cell-updates / wall time (forward and adjoint both counted, nothing
 recomputed).
 
 The JSON line adds
-  roofline:     the dominant kernel (the step kernel; for SH and P-SV it
-                advances two time steps per launch) against the 8 TB/s HBM
+  roofline:     the dominant kernel (the step kernel, which advances two
+                time steps per launch where it can) against the 8 TB/s HBM
                 peak: algorithmic bytes per launch (per time step, 8 B per
                 state field per shot cell + 4 B per material field per grid
                 cell, times the steps per launch) over its mean launch time

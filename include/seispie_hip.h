@@ -64,7 +64,7 @@ typedef struct sp_ctx sp_ctx;
 #define SP_OPT_SNAP_MB 4    /* snapshot memory budget in MiB (0 = auto) */
 #define SP_OPT_KERNEL 5     /* step kernel: 1 buffer-addressed unrolled march (default), 0 plain */
 #define SP_OPT_STRIP 6      /* strip geometry (shift << 16) | width: output lanes [shift, shift + width) */
-#define SP_OPT_FUSE 7       /* 1 (default): SH and P-SV advance two steps per launch wherever no snapshot,
+#define SP_OPT_FUSE 7       /* 1 (default): shots advance two steps per launch wherever no snapshot,
                                kernel interaction or checkpoint needs the state between them; 0: one step */
 #define SP_OPT_EXACT 8      /* 1 (default): the step kernels perform the reference's float64 operations one
                                by one in its order (correctly rounded quotients, no fused multiply-add), so
@@ -166,7 +166,7 @@ int sp_smooth_kernel(sp_ctx *ctx, int grid, int which, int sigma);
 int sp_smooth(sp_ctx *ctx, float *field, int nx, int nz, int sigma, int on_device);
 
 /* Autotuner: times `steps` forward steps of the current shots for each
- * rows-per-wavefront chunk, work order and (SH, P-SV) one or two steps per
+ * rows-per-wavefront chunk, work order and one or two steps per
  * launch, keeps the fastest for later runs and reports it; the choice is
  * remembered for the process and reused for the same physics and grids.  Overwrites wavefields and records (every run clears
  * them first).  No reference counterpart (the reference launches one

@@ -3,9 +3,13 @@
 // What the reference does (seispie/solver/fd2d/solver.py) is a Python time
 // loop that launches 6 (SH), 7 (P-SV) or 13 (spin) Numba kernels per step,
 // one thread per cell, each kernel streaming whole arrays through HBM.
-// Here one launch advances every shot of a batch by one time step:
+// Here one launch advances every shot of a batch by one or two time steps:
 //
-//   * k_step<PHYS>: each wavefront owns a strip of 64 consecutive z cells
+//   * k_step2<PHYS> (two steps per launch, the default wherever the state
+//     between the steps is not needed), k_step3<PHYS> (one step; the same
+//     march with buffer loads and stores) and k_step<PHYS> (plain one-step
+//     fallback for grids too large for 32-bit buffer offsets).  In each,
+//     a wavefront owns a strip of 64 consecutive z cells
 //     (z is the contiguous axis, k = i*nz + j) and marches along x through
 //     a chunk of rows.  It keeps a 4-row register window of the old
 //     stresses and of the new velocities, takes z neighbours with
@@ -13,7 +17,7 @@
 //     for the 58 interior lanes.  Stress divergence, source injection,
 //     sponge, velocity/displacement update, velocity gradient, stress
 //     update, receiver recording and adjoint snapshots are all fused, so a
-//     cell's state is read and written once per step.  Stresses and
+//     cell's state is read and written once per launch.  Stresses and
 //     velocities ping-pong between two buffers because neighbouring strips
 //     read the old values in their halo.
 //   * k_interact<PHYS>: kernel accumulation every sae adjoint steps.
@@ -894,10 +898,12 @@ struct Row2 {
     float s[n_s(P)];     // s(n), row m
     float v[n_v(P)];     // v(n), row m-1
     float u[n_v(P)];     // u(n), row m-4
-    float rhoA;          // row m-1
-    float rhoC;          // row m-4
+    float rhoA, jA;      // row m-1 (j: micro-inertia, spin only)
+    float rhoC, jC;      // row m-4
     float lamB, muB;     // row m-3
     float lamD, muD;     // row m-6
+    float nuB, gamB;     // row m-3 (spin only: nu, mu_c + nu_c)
+    float nuD, gamD;     // row m-6
 };
 
 // The rows are wave-uniform, so each load's row offset goes into the scalar
@@ -934,6 +940,16 @@ __device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
         p.lamB = bld(RM, vj, M_LAM * NP4 + ob);
         p.lamD = bld(RM, vj, M_LAM * NP4 + od);
     }
+    p.jA = p.jC = 1.f;
+    p.nuB = p.gamB = p.nuD = p.gamD = 0.f;
+    if constexpr (P == SP_SPIN) {
+        p.jA = bld(RM, vj, M_J * NP4 + oa);
+        p.jC = bld(RM, vj, M_J * NP4 + oc);
+        p.nuB = bld(RM, vj, M_NU * NP4 + ob);
+        p.gamB = bld(RM, vj, M_GAM * NP4 + ob);
+        p.nuD = bld(RM, vj, M_NU * NP4 + od);
+        p.gamD = bld(RM, vj, M_GAM * NP4 + od);
+    }
 }
 
 // stress divergence of one velocity row (div_sy / div_sxz, solver.py:54-66)
@@ -957,6 +973,19 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[
         double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, sc[1], zzp1, KZ) : 0.0;
         dsv[0] = (float)(dxx + dzxz);
         dsv[1] = (float)(dxxz + dzzz);
+        if constexpr (P == SP_SPIN) {
+            // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
+            float yzm2 = shfl(sc[5], L - 2), yzm1 = shfl(sc[5], L - 1), yzp1 = shfl(sc[5], L + 1);
+            float yym2 = shfl(sc[4], L - 2), yym1 = shfl(sc[4], L - 1), yyp1 = shfl(sc[4], L + 1);
+            double dxyx = xm_ok ? dmn<EX>(sa[3], sb[3], sc[3], sd[3], KX) : 0.0;
+            double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, sc[5], yzp1, KZ) : 0.0;
+            float dsy = (float)(dxyx + dzyz);
+            double dzyy = zm_ok ? dmn<EX>(yym2, yym1, sc[4], yyp1, KZ) : 0.0;
+            double dxyy = xm_ok ? dmn<EX>(sa[4], sb[4], sc[4], sd[4], KX) : 0.0;
+            dsv[0] = (float)((double)dsv[0] + dzyy);
+            dsv[1] = (float)((double)dsv[1] - dxyy);
+            dsv[2] = (float)((double)dsy + 2.0 * (double)sc[4]);
+        }
     }
 }
 
@@ -967,8 +996,8 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[
 // (the same float64 result, bit for bit).
 template <int P, bool EX>
 __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&dsv)[n_v(P)], const float (&vo)[n_v(P)],
-                                           float rho, float bxr, double bzb, double bzt, bool zint, bool rin, bool jin,
-                                           double dt)
+                                           float rho, float jj, float bxr, double bzb, double bzt, bool zint, bool rin,
+                                           bool jin, double dt)
 {
     if constexpr (EX) {
         double b = 1.0;
@@ -979,16 +1008,25 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
         const double rr = (double)rho, irr = rcp_rn(rr);
 #pragma unroll
         for (int f = 0; f < n_v(P); f++) {
-            const float v = vel_x(vo[f], dsv[f], dt, rr, irr, b);
+            double rf = rr, ir = irr;
+            if constexpr (P == SP_SPIN) {
+                if (f == 2) {  // vy_c divides by the micro-inertia j (solver.py:226)
+                    rf = (double)jj;
+                    ir = rcp_rn(rf);
+                }
+            }
+            const float v = vel_x(vo[f], dsv[f], dt, rf, ir, b);
             vn[f] = (rin && jin) ? v : 0.f;
         }
         return;
     }
     const double dtr = dt * rcp64((double)rho);
+    double dtj = dtr;
+    if constexpr (P == SP_SPIN) dtj = dt * rcp64((double)jj);
     if (zint && bxr == 1.f) {
 #pragma unroll
         for (int f = 0; f < n_v(P); f++) {
-            const float v = (float)((double)vo[f] + (double)dsv[f] * dtr);
+            const float v = (float)((double)vo[f] + (double)dsv[f] * (f == 2 ? dtj : dtr));
             vn[f] = (rin && jin) ? v : 0.f;
         }
         return;
@@ -997,7 +1035,7 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
     const double b = (double)(float)((double)b1 * bzt);
 #pragma unroll
     for (int f = 0; f < n_v(P); f++) {
-        const float v = (float)(b * ((double)vo[f] + (double)dsv[f] * dtr));
+        const float v = (float)(b * ((double)vo[f] + (double)dsv[f] * (f == 2 ? dtj : dtr)));
         vn[f] = (rin && jin) ? v : 0.f;
     }
 }
@@ -1007,8 +1045,8 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
 template <int P, bool EX>
 __device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n_v(P)], const float (&wb)[n_v(P)],
                                         const float (&wc)[n_v(P)], const float (&wd)[n_v(P)], const float (&so)[n_s(P)],
-                                        float lam, float mu, bool xp_ok, bool zp_ok, int L, const DCo &KX, const DCo &KZ,
-                                        double dt)
+                                        float lam, float mu, float nu, float gam, bool xp_ok, bool zp_ok, int L,
+                                        const DCo &KX, const DCo &KZ, double dt)
 {
     if constexpr (P == SP_SH) {
         float zm1 = shfl(wb[0], L - 1), zp1 = shfl(wb[0], L + 1), zp2 = shfl(wb[0], L + 2);
@@ -1028,6 +1066,16 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n
         sn[0] = upd_snorm<EX>(so[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
         sn[1] = upd_snorm<EX>(so[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
         sn[2] = upd_sadd<EX>(so[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+        if constexpr (P == SP_SPIN) {
+            // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
+            float cm1 = shfl(wb[2], L - 1), cp1 = shfl(wb[2], L + 1), cp2 = shfl(wb[2], L + 2);
+            const float dcdx = xp_ok ? (float)dpl<EX>(wa[2], wb[2], wc[2], wd[2], KX) : 0.f;
+            const float dcdz = zp_ok ? (float)dpl<EX>(cm1, wb[2], cp1, cp2, KZ) : 0.f;
+            const double dtg = dt * (double)gam;
+            sn[4] = upd_srot<EX>(so[4], 2.0 * dt, nu, wb[2], sub_rn(dvzdx, dvxdz));
+            sn[3] = upd_sadd<EX>(so[3], dtg, (double)dcdx);
+            sn[5] = upd_sadd<EX>(so[5], dtg, (double)dcdz);
+        }
     }
 }
 
@@ -1057,7 +1105,6 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)], int r, int j, int n
 template <int P, bool EX>
 __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 {
-    static_assert(P == SP_SH || P == SP_PSV, "two-step kernel: SH and P-SV");
     constexpr int NS = n_s(P), NV = n_v(P);
     const int nb = gridDim.x, bb = blockIdx.x;
     const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
@@ -1149,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 vel_row<P, EX>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, KX, KZ);
                 inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, cur.v, cur.rhoA, bxr, bzb, bzt, zint, rin, jin, dt);
+                vel_finish<P, EX>(vn, dsv, cur.v, cur.rhoA, cur.jA, bxr, bzb, bzt, zint, rin, jin, dt);
 #pragma unroll
                 for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
             }
@@ -1158,7 +1205,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int q = m - 3;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS];
-                str_row<P, EX>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, xp_ok, zp_ok, L, KX, KZ, dt);
+                str_row<P, EX>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB, xp_ok, zp_ok, L, KX, KZ, dt);
 #pragma unroll
                 for (int f = 0; f < NS; f++) S1[A][f] = sn[f];
             }
@@ -1170,7 +1217,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 vel_row<P, EX>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, KX, KZ);
                 inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, W1[D], cur.rhoC, bxr, bzb, bzt, zint, rin, jin, dt);
+                vel_finish<P, EX>(vn, dsv, W1[D], cur.rhoC, cur.jC, bxr, bzb, bzt, zint, rin, jin, dt);
                 float u1[NV], u2[NV];
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
@@ -1210,7 +1257,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int q = m - 6;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS];
-                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], cur.lamD, cur.muD, xp_ok, zp_ok, L, KX, KZ, dt);
+                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], cur.lamD, cur.muD, cur.nuD, cur.gamD, xp_ok, zp_ok, L, KX, KZ, dt);
                 const bool qok = q >= i0 && q < i1;
                 const unsigned oq = qok ? vo : OOB;
                 const int orow = qok ? q * nz * 4 : 0;
@@ -1713,7 +1760,7 @@ int strip_width(const sp_ctx *c) { return c->cur_fuse ? 52 : (c->cur_kernel == 1
 
 int fuse_ok(const sp_ctx *c, const std::vector<int> &batch)
 {
-    return c->fuse && c->physics != SP_SPIN && batch_kernel(c, batch) == 1;
+    return c->fuse && batch_kernel(c, batch) == 1;
 }
 
 // Steps the launch at step `it` may take: two when fusion is on and the
@@ -1865,9 +1912,13 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
             if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true>), grid, block, 0, c->stream, a);
             else hipLaunchKernelGGL((k_step2<SP_SH, false>), grid, block, 0, c->stream, a);
             break;
-        default:
+        case SP_PSV:
             if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true>), grid, block, 0, c->stream, a);
             else hipLaunchKernelGGL((k_step2<SP_PSV, false>), grid, block, 0, c->stream, a);
+            break;
+        default:
+            if (ex) hipLaunchKernelGGL((k_step2<SP_SPIN, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_SPIN, false>), grid, block, 0, c->stream, a);
             break;
         }
     } else if (c->cur_kernel == 1) {

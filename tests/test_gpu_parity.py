@@ -344,7 +344,7 @@ def test_errors_are_reported_not_raised_through_the_abi(ctx_factory):
         ctx.gradient(10)
 
 
-@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV])
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
 @pytest.mark.parametrize("chunk,order", [(16, 0), (4, 1), (32, 2)])
 def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order):
     """Two steps per launch (k_step2) give the records, misfit and kernels
