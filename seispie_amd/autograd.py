@@ -29,8 +29,15 @@ density kernel (adjoint velocity times forward velocity) is approximate;
 lam and mu are not affected.
 
 SH uses mu and rho only (its vp gradient is zero); P-SV uses all three.
-The micropolar-spin physics has further parameters and is driven through
-fd2d / the C ABI instead.
+
+Micropolar-spin shots take the reference's spin fields instead (fd2d.py:172):
+    rec = sim(lam, mu, nu, j, lam_c, mu_c, nu_c, rho)   # [nshot, 3, nrec, nt]
+with gradients K/dt from the library's spin kernels: lam, mu, nu, j and rho
+from their own kernels; mu_c and nu_c both from the (mu_c + nu_c) kernel,
+since they enter the equations only as that sum (solver.py:140-141); lam_c
+is unused by the equations (fd2d.py:210), so its gradient is zero.  Only
+the x and z records can carry a loss: the adjoint injects x and z sources,
+as the misfit does, and a nonzero gradient on the rotation records raises.
 """
 import math
 
@@ -53,8 +60,8 @@ class Simulation:
 
     def __init__(self, physics, nx, nz, dx, dz, nt, dt, shots, sponge=None, interval=None, sae=1, device=0,
                  init_model=None, autotune=True):
-        if physics not in (_lib.SH, _lib.PSV):
-            raise ValueError("autograd covers SH and P-SV physics")
+        if physics not in (_lib.SH, _lib.PSV, _lib.SPIN):
+            raise ValueError("unknown physics %r" % (physics,))
         if nt % sae:
             raise ValueError("nt must be a multiple of sae")
         nrec = {len(np.asarray(s[2]).reshape(-1)) for s in shots}
@@ -69,9 +76,13 @@ class Simulation:
         self.device = torch.device("cuda", device)
         self.ctx = _lib.Context(device)
         self.ctx.set_physics(physics, nt, dt)
+        self.param = 1 if physics == _lib.SPIN else 0
         if init_model is None:
-            init_model = (np.full(nx * nz, 3000, "f4"), np.full(nx * nz, 1700, "f4"), np.full(nx * nz, 2500, "f4"))
-        self.grid = self.ctx.add_grid(nx, nz, dx, dz, init_model, 0)
+            ones = np.ones(nx * nz, "f4")
+            init_model = ((3000 * ones, 1700 * ones, 2500 * ones) if self.param == 0 else
+                          (1e10 * ones, 8e9 * ones, 1e9 * ones, 2700 * ones, 0 * ones, 1.5e8 * ones, 3e8 * ones,
+                           2600 * ones))
+        self.grid = self.ctx.add_grid(nx, nz, dx, dz, init_model, self.param)
         if sponge is not None:
             self.ctx.set_sponge(self.grid, *sponge)
         self.nshot = len(shots)
@@ -86,24 +97,36 @@ class Simulation:
         self._gen = 0      # forward runs so far: backward re-runs the forward if another came between
         self._last = None  # model of the last forward run
 
-    def __call__(self, vp, vs, rho):
-        return SeismicRecords.apply(vp, vs, rho, self)
+    def __call__(self, *fields):
+        """(vp, vs, rho) for SH and P-SV; (lam, mu, nu, j, lam_c, mu_c,
+        nu_c, rho) for spin.  Returns the records [nshot, ncomp, nrec, nt]."""
+        if self.param == 1:
+            if len(fields) != len(SPIN_FIELDS):
+                raise ValueError("spin records take the 8 fields %s" % (SPIN_FIELDS,))
+            return SpinRecords.apply(self, *fields)
+        if len(fields) != 3:
+            raise ValueError("SH / P-SV records take (vp, vs, rho)")
+        return SeismicRecords.apply(*fields, self)
 
     # ------------------------------------------------------------ plumbing
     def _bind_stream(self):
         self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
 
-    def _set_model(self, vp, vs, rho):
+    def _set_model(self, *fields):
         self._bind_stream()
-        flat = [t.detach().to(self.device, torch.float32).contiguous().reshape(-1) for t in (vp, vs, rho)]
+        flat = [t.detach().to(self.device, torch.float32).contiguous().reshape(-1) for t in fields]
         for t in flat:
             if t.numel() != self.nx * self.nz:
                 raise ValueError("model fields must have nx*nz = %d cells" % (self.nx * self.nz))
-        self.ctx.set_model_vps(self.grid, *flat)
+        if self.param == 1:
+            for f, t in zip(SPIN_FIELD_IDS, flat):
+                self.ctx.set_field(self.grid, f, t)
+        else:
+            self.ctx.set_model_vps(self.grid, *flat)
         return flat
 
-    def run_forward(self, vp, vs, rho):
-        flat = self._set_model(vp, vs, rho)
+    def run_forward(self, *fields):
+        flat = self._set_model(*fields)
         self.ctx.forward_ckpt(self.interval)
         self._gen += 1
         self._last = flat
@@ -112,18 +135,26 @@ class Simulation:
             self.ctx.records(i, out[i])
         return out, self._gen
 
-    def run_adjoint(self, grad_records):
-        """Kernels (lam, mu, rho) of L for dL/d(records) = grad_records."""
+    def run_adjoint(self, grad_records, which=(_lib.K_LAM, _lib.K_MU, _lib.K_RHO)):
+        """Kernels `which` of L for dL/d(records) = grad_records."""
         self._bind_stream()
         ad = (-grad_records[:, :self.nmis]).flip(-1).to(torch.float32).contiguous()
         for i in range(self.nshot):
             self.ctx.set_adjoint_source(i, ad[i])
         self.ctx.adjoint_ckpt(self.sae)
         n = self.nx * self.nz
-        ks = [torch.empty(n, device=self.device, dtype=torch.float32) for _ in range(3)]
-        for t, which in zip(ks, (_lib.K_LAM, _lib.K_MU, _lib.K_RHO)):
-            self.ctx.kernel(self.grid, which, t)
+        ks = [torch.empty(n, device=self.device, dtype=torch.float32) for _ in which]
+        for t, w in zip(ks, which):
+            self.ctx.kernel(self.grid, w, t)
         return ks
+
+    def _refresh(self, fn_ctx, fields):
+        """Rebuild the checkpoints if another forward run replaced them."""
+        if self._gen != fn_ctx.gen:
+            self._set_model(*fields)
+            self.ctx.forward_ckpt(self.interval)
+            self._gen += 1
+            fn_ctx.gen = self._gen
 
     def hessian(self):
         """Pseudo-Hessian diagonal of the last backward pass: dt * sum over
@@ -135,6 +166,11 @@ class Simulation:
 
     def close(self):
         self.ctx.close()
+
+
+# the spin model fields in the reference's order (fd2d.py:172) and their ids
+SPIN_FIELDS = ("lam", "mu", "nu", "j", "lam_c", "mu_c", "nu_c", "rho")
+SPIN_FIELD_IDS = (_lib.F_LAM, _lib.F_MU, _lib.F_NU, _lib.F_J, _lib.F_LAMC, _lib.F_MUC, _lib.F_NUC, _lib.F_RHO)
 
 
 class SeismicRecords(torch.autograd.Function):
@@ -153,13 +189,7 @@ class SeismicRecords(torch.autograd.Function):
     def backward(ctx, grad):
         sim = ctx.sim
         vp, vs, rho = ctx.saved_tensors
-        if sim._gen != ctx.gen:
-            # another forward run replaced the checkpoints: rebuild them
-            sim._bind_stream()
-            sim.ctx.set_model_vps(sim.grid, vp, vs, rho)
-            sim.ctx.forward_ckpt(sim.interval)
-            sim._gen += 1
-            ctx.gen = sim._gen
+        sim._refresh(ctx, (vp, vs, rho))
         kl, km, kr = sim.run_adjoint(grad.contiguous())
         inv = 1.0 / sim.dt
         vp64, vs64, rho64 = (t.double() for t in (vp, vs, rho))
@@ -171,6 +201,36 @@ class SeismicRecords(torch.autograd.Function):
         grho = torch.where(live, vp64 * vp64 - 2.0 * vs64 * vs64, torch.zeros_like(vp64)) * kl + vs64 * vs64 * km + kr
         shape = ctx.shape
         return (gvp.float().reshape(shape), gvs.float().reshape(shape), grho.float().reshape(shape), None)
+
+
+class SpinRecords(torch.autograd.Function):
+    """records = F(lam, mu, nu, j, lam_c, mu_c, nu_c, rho) for spin shots;
+    backward = the hand-written adjoint with the spin kernels."""
+
+    @staticmethod
+    def forward(ctx, sim, *fields):
+        out, gen = sim.run_forward(*fields)
+        ctx.sim = sim
+        ctx.gen = gen
+        ctx.shape = fields[0].shape
+        ctx.save_for_backward(*sim._last)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        sim = ctx.sim
+        if bool((grad[:, sim.nmis:] != 0).any()):
+            # the adjoint injects x and z sources only, as the misfit does
+            raise NotImplementedError("spin: losses on the rotation (y) records cannot be differentiated; "
+                                      "use the x and z components")
+        sim._refresh(ctx, ctx.saved_tensors)
+        kl, km, knu, kj, kc, kr = sim.run_adjoint(
+            grad.contiguous(), (_lib.K_LAM, _lib.K_MU, _lib.K_NU, _lib.K_J, _lib.K_MUC, _lib.K_RHO))
+        inv = 1.0 / sim.dt
+        g = [(k.double() * inv).float().reshape(ctx.shape) for k in (kl, km, knu, kj)]
+        gc = (kc.double() * inv).float().reshape(ctx.shape)
+        gr = (kr.double() * inv).float().reshape(ctx.shape)
+        return (None, g[0], g[1], g[2], g[3], torch.zeros_like(gc), gc, gc.clone(), gr)
 
 
 def waveform_loss(records, data, dt):
