@@ -90,7 +90,8 @@ int sp_set_physics(sp_ctx *ctx, int physics, int nt, double dt);
  * param 1 takes the reference's spin list {lambda, mu, nu, j, lambda_c,
  * mu_c, nu_c, rho} (fd2d.py:172); param 2 takes {lambda, mu, rho}. */
 int sp_add_grid(sp_ctx *ctx, int nx, int nz, float dx, float dz, int param,
-                const float *const *fields, int *grid_out);
+                const float *const *fields, int *grid_out);  /* fields: 3 pointers (param 0, 2) or 8 (param 1);
+                                                                 on failure no grid is added */
 /* set_bound: fd2d.py:47-68, 218-228 */
 int sp_set_sponge(sp_ctx *ctx, int grid, int width, double alpha, int left, int right,
                   int bottom, int top);

@@ -192,6 +192,11 @@ class Context:
         self.physics, self.nt, self.dt = int(physics), int(nt), float(dt)
 
     def add_grid(self, nx, nz, dx, dz, fields, param):
+        """fields: (vp, vs, rho) for param 0, the 8 spin fields (lam, mu, nu,
+        j, lam_c, mu_c, nu_c, rho) for param 1, (lam, mu, rho) for param 2."""
+        need = {0: 3, 1: 8, 2: 3}.get(int(param))
+        if need is not None and len(fields) != need:
+            raise ValueError("param %d takes %d fields, got %d" % (param, need, len(fields)))
         n = nx * nz
         arrs = [_f32(f, n) for f in fields]
         ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])

@@ -1609,6 +1609,27 @@ namespace {
         }                                                                                    \
     } while (0)
 
+// Runs `undo` when the scope ends unless dismissed: partial allocations of
+// a failed call are released and the context is left as it was.
+template <typename F>
+struct ScopeUndo {
+    F undo;
+    bool armed = true;
+    ~ScopeUndo() { if (armed) undo(); }
+};
+template <typename F>
+ScopeUndo<F> on_failure(F f) { return ScopeUndo<F>{f}; }
+
+// device buffer owned for the length of one call
+template <typename T>
+struct DevTmp {
+    T *p = nullptr;
+    DevTmp() = default;
+    DevTmp(const DevTmp &) = delete;
+    DevTmp &operator=(const DevTmp &) = delete;
+    ~DevTmp() { if (p) (void)hipFree(p); }
+};
+
 int fail(sp_ctx *ctx, const std::string &msg)
 {
     ctx->err = msg;
@@ -2140,33 +2161,29 @@ int run_misfit_batch(sp_ctx *c, const std::vector<int> &batch, double *norms_out
             else taper[it] = 1.0;
         }
     }
-    int4 *d_trip = nullptr;
-    const float **d_data = nullptr;
-    double *d_part = nullptr, *d_taper = nullptr;
-    HIPCHECK(c, hipMalloc(&d_taper, taper.size() * sizeof(double)));
-    HIPCHECK(c, hipMemcpyAsync(d_taper, taper.data(), taper.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMalloc(&d_trip, trip.size() * sizeof(int4)));
-    HIPCHECK(c, hipMalloc(&d_data, data.size() * sizeof(float *)));
-    HIPCHECK(c, hipMalloc(&d_part, trip.size() * sizeof(double)));
-    HIPCHECK(c, hipMemcpyAsync(d_trip, trip.data(), trip.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(d_data, data.data(), data.size() * sizeof(float *), hipMemcpyHostToDevice, c->stream));
+    DevTmp<int4> d_trip;
+    DevTmp<const float *> d_data;
+    DevTmp<double> d_part, d_taper;
+    HIPCHECK(c, hipMalloc(&d_taper.p, taper.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpyAsync(d_taper.p, taper.data(), taper.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMalloc(&d_trip.p, trip.size() * sizeof(int4)));
+    HIPCHECK(c, hipMalloc(&d_data.p, data.size() * sizeof(float *)));
+    HIPCHECK(c, hipMalloc(&d_part.p, trip.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpyAsync(d_trip.p, trip.data(), trip.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_data.p, data.data(), data.size() * sizeof(float *), hipMemcpyHostToDevice, c->stream));
     MisfitArgs a;
     a.shots = c->d_desc;
-    a.triples = d_trip;
-    a.data = d_data;
-    a.partial = d_part;
-    a.taper = d_taper;
+    a.triples = d_trip.p;
+    a.data = d_data.p;
+    a.partial = d_part.p;
+    a.taper = d_taper.p;
     a.n = (int)trip.size();
     a.nt = c->nt;
     hipLaunchKernelGGL(k_misfit, dim3((unsigned)trip.size()), dim3(256), 0, c->stream, a);
     HIPCHECK(c, hipGetLastError());
     std::vector<double> part(trip.size());
-    HIPCHECK(c, hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(part.data(), d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    HIPCHECK(c, hipFree(d_trip));
-    HIPCHECK(c, hipFree(d_data));
-    HIPCHECK(c, hipFree(d_part));
-    HIPCHECK(c, hipFree(d_taper));
     size_t p = 0;
     for (int b = 0; b < (int)batch.size(); b++) {
         const Shot &s = c->shots[batch[b]];
@@ -2544,6 +2561,15 @@ int sp_create(int device, sp_ctx **out)
     return 0;
 }
 
+static void free_grid(Grid &g)
+{
+    if (g.mat) (void)hipFree(g.mat);
+    if (g.bx) (void)hipFree(g.bx);
+    if (g.bz) (void)hipFree(g.bz);
+    if (g.kern) (void)hipFree(g.kern);
+    g = Grid();
+}
+
 static void free_shot(Shot &s)
 {
     if (s.state2) (void)hipFree(s.state2);
@@ -2564,10 +2590,7 @@ int sp_destroy(sp_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     for (auto &s : c->shots) free_shot(s);
     for (auto &g : c->grids) {
-        if (g.mat) (void)hipFree(g.mat);
-        if (g.bx) (void)hipFree(g.bx);
-        if (g.bz) (void)hipFree(g.bz);
-        if (g.kern) (void)hipFree(g.kern);
+        free_grid(g);
     }
     if (c->snap_pool) (void)hipFree(c->snap_pool);
     if (c->d_desc) (void)hipFree(c->d_desc);
@@ -2652,7 +2675,11 @@ int sp_add_grid(sp_ctx *c, int nx, int nz, float dx, float dz, int param, const 
     if (nx < 1 || nz < 1) return fail(c, "grid must have nx >= 1 and nz >= 1");
     if (!(dx > 0) || !(dz > 0)) return fail(c, "dx and dz must be positive");
     if (!fields) return fail(c, "fields is null");
+    if (param < 0 || param > 2) return fail(c, "param must be 0 (vp, vs, rho), 1 (spin list) or 2 (lambda, mu, rho)");
+    for (int f = 0; f < (param == 1 ? 8 : 3); f++)
+        if (!fields[f]) return fail(c, "missing model field");
     Grid g;
+    auto undo = on_failure([&] { free_grid(g); });
     g.nx = nx;
     g.nz = nz;
     g.dx = dx;
@@ -2664,13 +2691,11 @@ int sp_add_grid(sp_ctx *c, int nx, int nz, float dx, float dz, int param, const 
     HIPCHECK(c, hipMalloc(&g.kern, (size_t)SP_NKERN * g.Npad * sizeof(float)));
     HIPCHECK(c, hipMalloc(&g.bx, (size_t)nx * sizeof(float)));
     HIPCHECK(c, hipMalloc(&g.bz, 2 * (size_t)nz * sizeof(double)));
-    c->grids.push_back(g);
-    Grid &G = c->grids.back();
+    Grid &G = g;
     if (fill(c, G.mat, (long long)N_MAT * G.Npad, 0.f)) return 2;
     if (fill(c, G.kern, (long long)SP_NKERN * G.Npad, 0.f)) return 2;
     float *m = G.mat;
     auto up = [&](int slot, const float *src) -> int {
-        if (!src) return fail(c, "missing model field");
         HIPCHECK(c, hipMemcpyAsync(m + (size_t)slot * G.Npad, src, nb, hipMemcpyHostToDevice, c->stream));
         return 0;
     };
@@ -2686,10 +2711,8 @@ int sp_add_grid(sp_ctx *c, int nx, int nz, float dx, float dz, int param, const 
         hipLaunchKernelGGL(k_gamma, dim3((unsigned)((G.N + 255) / 256)), dim3(256), 0, c->stream,
                            m + (size_t)M_GAM * G.Npad, m + (size_t)SP_F_MUC * G.Npad, m + (size_t)SP_F_NUC * G.Npad, G.N);
         HIPCHECK(c, hipGetLastError());
-    } else if (param == 2) {  // lambda, mu, rho
+    } else {  // 2: lambda, mu, rho
         if (up(M_LAM, fields[0]) || up(M_MU, fields[1]) || up(M_RHO, fields[2])) return 2;
-    } else {
-        return fail(c, "param must be 0 (vp, vs, rho), 1 (spin list) or 2 (lambda, mu, rho)");
     }
     // no sponge until sp_set_sponge: all factors 1
     std::vector<float> bx(nx, 1.0f);
@@ -2697,6 +2720,8 @@ int sp_add_grid(sp_ctx *c, int nx, int nz, float dx, float dz, int param, const 
     HIPCHECK(c, hipMemcpyAsync(G.bx, bx.data(), bx.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(G.bz, bz.data(), bz.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    undo.armed = false;
+    c->grids.push_back(g);
     if (grid_out) *grid_out = (int)c->grids.size() - 1;
     return 0;
 }
@@ -2756,6 +2781,7 @@ int sp_add_shot(sp_ctx *c, int grid, int nsrc, const int *src_k, const float *st
     for (int r = 0; r < nrec; r++)
         if (rec_k[r] < 0 || rec_k[r] >= G.N) return fail(c, "receiver index outside the grid");
     Shot s;
+    auto undo = on_failure([&] { free_shot(s); });
     s.grid = grid;
     s.nsrc = nsrc;
     s.nrec = nrec;
@@ -2786,10 +2812,10 @@ int sp_add_shot(sp_ctx *c, int grid, int nsrc, const int *src_k, const float *st
     const long long nr = std::max(rec_floats(c, s), 1LL), na = std::max(adj_floats(c, s), 1LL);
     HIPCHECK(c, hipMalloc(&s.rec, (size_t)nr * sizeof(float)));
     HIPCHECK(c, hipMalloc(&s.adstf, (size_t)na * sizeof(float)));
-    c->shots.push_back(s);
-    Shot &S = c->shots.back();
-    if (fill(c, S.rec, nr, 0.f) || fill(c, S.adstf, na, 0.f)) return 2;
+    if (fill(c, s.rec, nr, 0.f) || fill(c, s.adstf, na, 0.f)) return 2;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    undo.armed = false;
+    c->shots.push_back(s);
     c->snaps_valid = 0;
     if (shot_out) *shot_out = (int)c->shots.size() - 1;
     return 0;
@@ -3063,21 +3089,19 @@ static int smooth_dev(sp_ctx *c, float *field, int nx, int nz, int sigma)
         for (int n = 0; n < nz; n++) a += g[std::abs(j - n)];
         sz[j] = a;
     }
-    double *dg = nullptr;
-    float *tmp = nullptr;
+    DevTmp<double> dg;
+    DevTmp<float> tmp;
     const long long N = (long long)nx * nz;
-    HIPCHECK(c, hipMalloc(&dg, (size_t)(nm + nx + nz) * sizeof(double)));
-    HIPCHECK(c, hipMalloc(&tmp, (size_t)N * sizeof(float)));
-    HIPCHECK(c, hipMemcpyAsync(dg, g.data(), nm * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(dg + nm, sx.data(), nx * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(dg + nm + nx, sz.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMalloc(&dg.p, (size_t)(nm + nx + nz) * sizeof(double)));
+    HIPCHECK(c, hipMalloc(&tmp.p, (size_t)N * sizeof(float)));
+    HIPCHECK(c, hipMemcpyAsync(dg.p, g.data(), nm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dg.p + nm, sx.data(), nx * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dg.p + nm + nx, sz.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
     const dim3 gr((unsigned)((N + 255) / 256)), bl(256);
-    hipLaunchKernelGGL(k_smooth_x, gr, bl, 0, c->stream, field, tmp, dg, nx, nz);
-    hipLaunchKernelGGL(k_smooth_z, gr, bl, 0, c->stream, field, tmp, dg, dg + nm, dg + nm + nx, nx, nz);
+    hipLaunchKernelGGL(k_smooth_x, gr, bl, 0, c->stream, field, tmp.p, dg.p, nx, nz);
+    hipLaunchKernelGGL(k_smooth_z, gr, bl, 0, c->stream, field, tmp.p, dg.p, dg.p + nm, dg.p + nm + nx, nx, nz);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    HIPCHECK(c, hipFree(dg));
-    HIPCHECK(c, hipFree(tmp));
     return 0;
 }
 

@@ -333,11 +333,16 @@ def test_errors_are_reported_not_raised_through_the_abi(ctx_factory):
     ctx.set_physics(_lib.PSV, 100, 5e-4)
     with pytest.raises(_lib.SeispieError, match="no shots"):
         ctx.forward(0)
-    g = ctx.add_grid(10, 10, 5.0, 5.0, [np.full(100, 3000, np.float32), np.full(100, 1700, np.float32),
-                                        np.full(100, 2600, np.float32)], 0)
+    three = [np.full(100, 3000, np.float32), np.full(100, 1700, np.float32), np.full(100, 2600, np.float32)]
+    with pytest.raises(_lib.SeispieError, match="param"):
+        ctx.add_grid(10, 10, 5.0, 5.0, three, 5)
+    with pytest.raises(ValueError, match="8 fields"):
+        ctx.add_grid(10, 10, 5.0, 5.0, three, 1)
+    g = ctx.add_grid(10, 10, 5.0, 5.0, three, 0)
+    assert g == 0, "a failed sp_add_grid left a grid behind"
     with pytest.raises(_lib.SeispieError, match="outside the grid"):
         ctx.add_shot(g, [100], np.zeros((2, 1, 100), np.float32), [0])
-    ctx.add_shot(g, [5], np.zeros((2, 1, 100), np.float32), [0])
+    assert ctx.add_shot(g, [5], np.zeros((2, 1, 100), np.float32), [0]) == 0
     with pytest.raises(_lib.SeispieError, match="multiple"):
         ctx.gradient(7)
     with pytest.raises(_lib.SeispieError, match="sp_set_data"):
