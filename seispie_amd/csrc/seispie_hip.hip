@@ -1661,6 +1661,7 @@ int ensure(sp_ctx *ctx, T *&p, size_t &cap, size_t n)
     if (n <= cap) return 0;
     if (p) HIPCHECK(ctx, hipFree(p));
     p = nullptr;
+    cap = 0;  // a failed allocation leaves an empty buffer, never a stale size
     HIPCHECK(ctx, hipMalloc(&p, n * sizeof(T)));
     cap = n;
     return 0;
