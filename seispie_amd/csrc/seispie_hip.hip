@@ -107,7 +107,13 @@ struct StepArgs {
     const int *itp;     // device step counter of a replayed graph, or null
     double dt;
     int rd;        // stress/velocity set read this step
-    int snap_idx;  // -1: no snapshot this step
+    int snap_idx;  // -1: no snapshot this step (one-step kernel k_step)
+    // two-step kernel: snapshot halves only it holds (-1: none).  snap_v:
+    // the snapshot of the pair's first step, whose velocity after that step
+    // is intermediate; snap_u: the snapshot of its second step, whose
+    // displacement before that step is intermediate.  The other halves are
+    // copied from the state before / after the launch.
+    int snap_v, snap_u;
     int adjoint;   // inject adjoint sources at the receivers, record nothing
     int shift;     // lanes before the first output cell of a strip (>= 3)
     int wout;      // output cells per strip (shift + wout <= 61)
@@ -1234,6 +1240,19 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                     bst(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
                     bst(RS, ov, slot_v(P, wr, f) * NP4 + orow, vn[f]);
                 }
+                // adjoint snapshots (solver.py:193-203, 236-242) that need the
+                // state between the two steps: [isa][u before | v after][NV]
+                if (a.snap_v >= 0) {
+                    const rsrc_t RP = make_rsrc(d.snap + ((long long)a.snap_v * 2 + 1) * NV * (NP4 / 4),
+                                                (long long)NV * NP4);
+#pragma unroll
+                    for (int f = 0; f < NV; f++) bst(RP, ov, f * NP4 + orow, W1[D][f]);
+                }
+                if (a.snap_u >= 0) {
+                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), (long long)NV * NP4);
+#pragma unroll
+                    for (int f = 0; f < NV; f++) bst(RP, ov, f * NP4 + orow, u1[f]);
+                }
                 // save_obs of both steps (solver.py:151-156) for receivers in
                 // this row; rows past the chunk belong to the next chunk
                 if (r == rec_row && r < i1) {
@@ -1786,14 +1805,13 @@ int fuse_ok(const sp_ctx *c, const std::vector<int> &batch)
 }
 
 // Steps the launch at step `it` may take: two when fusion is on and the
-// state after step it is observed by nothing -- no snapshot on step it or
-// it+1 (each needs the state before and after its step), no interaction
-// after step it, no checkpoint before step it+1 -- and both steps are
-// before `end`.
-int steps_next(const sp_ctx *c, int it, int end, int snap_sae, int inter_sae, int ckpt_C)
+// state after step it is observed by nothing on the host side -- no
+// interaction after step it, no checkpoint before step it+1 -- and both
+// steps are before `end`.  Adjoint snapshots do not split a pair: the
+// two-step kernel writes the halves that fall between its steps.
+int steps_next(const sp_ctx *c, int it, int end, int inter_sae, int ckpt_C)
 {
     if (!c->cur_fuse || it + 2 > end) return 1;
-    if (snap_sae > 0 && ((it + 1) % snap_sae == 0 || (it + 2) % snap_sae == 0)) return 1;
     if (inter_sae > 0 && it % inter_sae == 0) return 1;
     if (ckpt_C > 0 && (it + 1) % ckpt_C == 0) return 1;
     return 2;
@@ -1910,9 +1928,11 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
 // One launch from step `it`: `nsteps` (1 or 2) steps of every shot of the
 // prepared batch, reading stress/velocity set `rd` and writing set rd ^ 1.
 int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd, int nsteps = 1,
-                const ShotDesc *desc = nullptr)
+                const ShotDesc *desc = nullptr, int snap_v = -1, int snap_u = -1)
 {
     StepArgs a;
+    a.snap_v = snap_v;
+    a.snap_u = snap_u;
     a.shots = desc ? desc : c->d_desc;
     a.items = c->d_items;
     a.nitems = nitems;
@@ -2019,7 +2039,7 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
     auto launches_in = [&](int steps) {
         int n = 0;
         for (int it = 0; it < steps;) {
-            it += steps_next(c, it, steps, 0, 0, 0);
+            it += steps_next(c, it, steps, 0, 0);
             n++;
         }
         return n;
@@ -2037,7 +2057,7 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
     c->itp = c->d_counter;
     int s = set, rc = 0, nl = 0;
     for (int it = 0; it < block && !rc;) {
-        const int ns = steps_next(c, it, block, 0, 0, 0);
+        const int ns = steps_next(c, it, block, 0, 0);
         rc = launch_step(c, nitems, it, -1, 0, s, ns);
         s ^= 1;
         nl++;
@@ -2068,7 +2088,7 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
     c->t_launches += (long long)nl * (nblk - 1);  // the capture counted one block
     // the remaining steps, eagerly
     for (int it = nblk * block; it < nt;) {
-        const int ns = steps_next(c, it, nt, 0, 0, 0);
+        const int ns = steps_next(c, it, nt, 0, 0);
         if (launch_step(c, nitems, it, -1, 0, s, ns)) return 2;
         s ^= 1;
         if (ns == 1)
@@ -2108,16 +2128,20 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
         if (rc != 1) return rc;
     }
     for (int it = 0; it < c->nt;) {
-        const int ns = steps_next(c, it, c->nt, sae, 0, 0);
-        int isa = -1;
-        if (sae > 0 && (it + 1) % sae == 0) isa = nsa - (it + 1) / sae;  // single steps only
+        const int ns = steps_next(c, it, c->nt, 0, 0);
+        // snapshots after step it and (two-step launches) after step it+1,
+        // stored in reverse order (solver.py:193-203)
+        const int isa = (sae > 0 && (it + 1) % sae == 0) ? nsa - (it + 1) / sae : -1;
+        const int isb = (ns == 2 && sae > 0 && (it + 2) % sae == 0) ? nsa - (it + 2) / sae : -1;
         if (split && isa >= 0)
             hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 0, 0, c->physics);
-        if (launch_step(c, nitems, it, split ? -1 : isa, 0, set, ns)) return 2;
+        if (launch_step(c, nitems, it, split ? -1 : isa, 0, set, ns, nullptr, ns == 2 ? isa : -1, isb)) return 2;
         set ^= 1;
         if (split) {
-            if (isa >= 0)
-                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 1, set, c->physics);
+            // the velocity half after the launch's last step
+            const int last = ns == 2 ? isb : isa;
+            if (last >= 0)
+                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, last, 1, set, c->physics);
             // the two-step kernel records both of its steps itself
             if (ns == 1)
                 hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt,
@@ -2265,7 +2289,7 @@ int run_adjoint_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     const double ndt = (double)sae * c->dt;
     int set = 0;
     for (int it = 0; it < c->nt;) {
-        const int ns = steps_next(c, it, c->nt, 0, sae, 0);
+        const int ns = steps_next(c, it, c->nt, sae, 0);
         if (launch_step(c, nitems, it, -1, 1, set, ns)) return 2;
         set ^= 1;
         const int last = it + ns - 1;
@@ -2329,7 +2353,7 @@ int run_forward_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
         if (it % C == 0)
             for (int s : batch)
                 if (ckpt_copy(c, c->shots[s], it / C, true, set)) return 2;
-        const int ns = steps_next(c, it, c->nt, 0, 0, C);
+        const int ns = steps_next(c, it, c->nt, 0, C);
         if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
         set ^= 1;
         if (c->cur_kernel == 1 && ns == 1) {
@@ -2392,7 +2416,7 @@ int run_adjoint_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int sae, in
     const double ndt = (double)sae * c->dt;
     int seg_loaded = -1, isa_base = 0, set = 0;
     for (int a = 0; a < nt;) {
-        const int ns = steps_next(c, a, nt, 0, sae, 0);
+        const int ns = steps_next(c, a, nt, sae, 0);
         if (launch_step(c, nitems, a, -1, 1, set, ns)) return 2;
         set ^= 1;
         const int last = a + ns - 1;
@@ -2406,16 +2430,18 @@ int run_adjoint_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int sae, in
                 if (ckpt_copy(c, c->shots[s], seg, false, 0)) return 2;
             int set2 = 0;
             for (int it = lo; it < hi;) {
-                const int ns2 = steps_next(c, it, hi, sae, 0, 0);
-                const int isa = (it + 1) % sae == 0 ? nsa - (it + 1) / sae : -1;  // single steps only
+                const int ns2 = steps_next(c, it, hi, 0, 0);
+                const int isa = (it + 1) % sae == 0 ? nsa - (it + 1) / sae - isa_base : -1;
+                const int isb = (ns2 == 2 && (it + 2) % sae == 0) ? nsa - (it + 2) / sae - isa_base : -1;
                 if (isa >= 0)
-                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa - isa_base, 0, 0,
+                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa, 0, 0,
                                        c->physics);
-                if (launch_step(c, nitems, it, -1, 0, set2, ns2, c->d_desc2)) return 2;
+                if (launch_step(c, nitems, it, -1, 0, set2, ns2, c->d_desc2, ns2 == 2 ? isa : -1, isb)) return 2;
                 set2 ^= 1;
-                if (isa >= 0)
-                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa - isa_base, 1,
-                                       set2, c->physics);
+                const int last = ns2 == 2 ? isb : isa;
+                if (last >= 0)
+                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, last, 1, set2,
+                                       c->physics);
                 it += ns2;
             }
             HIPCHECK(c, hipGetLastError());
@@ -2882,7 +2908,7 @@ int sp_forward_steps(sp_ctx *c, int it0, int it1)
     if (timing_begin(c)) return 2;
     int set = c->cur_set;
     for (int it = it0; it < it1;) {
-        const int ns = steps_next(c, it, it1, 0, 0, 0);
+        const int ns = steps_next(c, it, it1, 0, 0);
         if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
         set ^= 1;
         if (c->cur_kernel == 1 && ns == 1) {
@@ -3178,7 +3204,7 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
                 HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
                 int set = 1;
                 for (int it = 0; it < steps;) {
-                    const int ns = steps_next(c, it, steps, 0, 0, 0);
+                    const int ns = steps_next(c, it, steps, 0, 0);
                     if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
                     set ^= 1;
                     it += ns;

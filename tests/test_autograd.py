@@ -28,11 +28,12 @@ def _ctx_with(ctx_factory, prob, data):
 
 
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
-def test_checkpointed_adjoint_equals_stored_snapshots(ctx_factory, physics):
+@pytest.mark.parametrize("sae", [10, 5])
+def test_checkpointed_adjoint_equals_stored_snapshots(ctx_factory, physics, sae):
     prob = _small(physics, nt=200, nshots=2)
-    prob.sae = 10
+    prob.sae = sae
     data = _data_for(prob)
-    print("case: ckpt", physics, flush=True)
+    print("case: ckpt", physics, "sae", sae, flush=True)
     ctx, ids = _ctx_with(ctx_factory, prob, data)
     norms = ctx.gradient(prob.sae)
     ref_k = [ctx.kernel(0, w) for w in range(_lib.K_VP)] + [ctx.kernel(0, _lib.K_HESS)]

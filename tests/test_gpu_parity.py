@@ -350,12 +350,15 @@ def test_errors_are_reported_not_raised_through_the_abi(ctx_factory):
 
 
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
-@pytest.mark.parametrize("chunk,order", [(16, 0), (4, 1), (32, 2)])
-def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order):
+@pytest.mark.parametrize("chunk,order,sae", [(16, 0, 10), (4, 1, 5), (32, 2, 1)])
+def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order, sae):
     """Two steps per launch (k_step2) give the records, misfit and kernels
     of one step per launch bit for bit: ragged grids, receivers on every
     row (chunk boundaries included) and on shared cells, sources in the
-    rows a chunk recomputes for its neighbours and on the grid edges."""
+    rows a chunk recomputes for its neighbours and on the grid edges.
+    Adjoint snapshots fall on the second step of a pair (sae 10), on either
+    step (sae 5) and on both (sae 1); the kernel writes the halves between
+    its steps."""
     nt = 120
     probs = [_small(physics, 67, 121, nt, 2, seed=4), _small(physics, 40, 59, nt, 1, seed=5)]
     for p in probs:
@@ -368,9 +371,9 @@ def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order
         stf = configs.shot_stf(physics, comps)
         stf_many = np.concatenate([stf * (1 + 0.1 * q) for q in range(src_k.size)], axis=1)
         g.shots = [configs.Shot(src_k, stf_many, np.array(rec, np.int32)) for _ in g.shots]
-    prob = configs.Problem(physics, nt, 5e-4, [p.grids[0] for p in probs], sae=10)
+    prob = configs.Problem(physics, nt, 5e-4, [p.grids[0] for p in probs], sae=sae)
     data = [d * 1.01 for d in _data_for(prob)]
-    print("case: fuse", physics, chunk, order, flush=True)
+    print("case: fuse", physics, chunk, order, "sae", sae, flush=True)
     out = []
     for fuse in (0, 1):
         ctx = ctx_factory()
