@@ -108,11 +108,9 @@ struct StepArgs {
     double dt;
     int rd;        // stress/velocity set read this step
     int snap_idx;  // -1: no snapshot this step (one-step kernel k_step)
-    // two-step kernel: snapshot halves only it holds (-1: none).  snap_v:
-    // the snapshot of the pair's first step, whose velocity after that step
-    // is intermediate; snap_u: the snapshot of its second step, whose
-    // displacement before that step is intermediate.  The other halves are
-    // copied from the state before / after the launch.
+    // two-step kernel: the snapshot of the pair's first step (snap_v) and
+    // of its second step (snap_u), -1: none; the kernel writes both halves
+    // of each (u before the step, v after it)
     int snap_v, snap_u;
     int adjoint;   // inject adjoint sources at the receivers, record nothing
     int shift;     // lanes before the first output cell of a strip (>= 3)
@@ -1240,18 +1238,23 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                     bst(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
                     bst(RS, ov, slot_v(P, wr, f) * NP4 + orow, vn[f]);
                 }
-                // adjoint snapshots (solver.py:193-203, 236-242) that need the
-                // state between the two steps: [isa][u before | v after][NV]
+                // adjoint snapshots of either step (solver.py:193-203, 236-242):
+                // [isa][u before the step | v after it][NV]
                 if (a.snap_v >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + ((long long)a.snap_v * 2 + 1) * NV * (NP4 / 4),
-                                                (long long)NV * NP4);
+                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
-                    for (int f = 0; f < NV; f++) bst(RP, ov, f * NP4 + orow, W1[D][f]);
+                    for (int f = 0; f < NV; f++) {
+                        bst(RP, ov, f * NP4 + orow, cur.u[f]);
+                        bst(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
+                    }
                 }
                 if (a.snap_u >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), (long long)NV * NP4);
+                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
-                    for (int f = 0; f < NV; f++) bst(RP, ov, f * NP4 + orow, u1[f]);
+                    for (int f = 0; f < NV; f++) {
+                        bst(RP, ov, f * NP4 + orow, u1[f]);
+                        bst(RP, ov, (NV + f) * NP4 + orow, vn[f]);
+                    }
                 }
                 // save_obs of both steps (solver.py:151-156) for receivers in
                 // this row; rows past the chunk belong to the next chunk
@@ -2133,15 +2136,16 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
         // stored in reverse order (solver.py:193-203)
         const int isa = (sae > 0 && (it + 1) % sae == 0) ? nsa - (it + 1) / sae : -1;
         const int isb = (ns == 2 && sae > 0 && (it + 2) % sae == 0) ? nsa - (it + 2) / sae : -1;
-        if (split && isa >= 0)
+        // single steps of the buffer-addressed kernel: snapshot copies before
+        // and after; two-step launches write their snapshots themselves
+        const bool copy = split && ns == 1 && isa >= 0;
+        if (copy)
             hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 0, 0, c->physics);
         if (launch_step(c, nitems, it, split ? -1 : isa, 0, set, ns, nullptr, ns == 2 ? isa : -1, isb)) return 2;
         set ^= 1;
         if (split) {
-            // the velocity half after the launch's last step
-            const int last = ns == 2 ? isb : isa;
-            if (last >= 0)
-                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, last, 1, set, c->physics);
+            if (copy)
+                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 1, set, c->physics);
             // the two-step kernel records both of its steps itself
             if (ns == 1)
                 hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt,
@@ -2433,14 +2437,14 @@ int run_adjoint_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int sae, in
                 const int ns2 = steps_next(c, it, hi, 0, 0);
                 const int isa = (it + 1) % sae == 0 ? nsa - (it + 1) / sae - isa_base : -1;
                 const int isb = (ns2 == 2 && (it + 2) % sae == 0) ? nsa - (it + 2) / sae - isa_base : -1;
-                if (isa >= 0)
+                const bool copy = ns2 == 1 && isa >= 0;
+                if (copy)
                     hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa, 0, 0,
                                        c->physics);
                 if (launch_step(c, nitems, it, -1, 0, set2, ns2, c->d_desc2, ns2 == 2 ? isa : -1, isb)) return 2;
                 set2 ^= 1;
-                const int last = ns2 == 2 ? isb : isa;
-                if (last >= 0)
-                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, last, 1, set2,
+                if (copy)
+                    hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa, 1, set2,
                                        c->physics);
                 it += ns2;
             }
