@@ -903,11 +903,10 @@ struct Row2 {
     float v[n_v(P)];     // v(n), row m-1
     float u[n_v(P)];     // u(n), row m-4
     float rhoA, jA;      // row m-1 (j: micro-inertia, spin only)
-    float rhoC, jC;      // row m-4
     float lamB, muB;     // row m-3
-    float lamD, muD;     // row m-6
     float nuB, gamB;     // row m-3 (spin only: nu, mu_c + nu_c)
-    float nuD, gamD;     // row m-6
+    // rows m-4 and m-6 of the same materials are the values loaded three
+    // iterations earlier; the march keeps them in register windows
 };
 
 // The rows are wave-uniform, so each load's row offset goes into the scalar
@@ -923,11 +922,9 @@ __device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
     const int ms = min(max(mp, max(i0 - 6, 0)), min(i1 + 5, nx - 1));
     const int ma = min(max(mp - 1, max(i0 - 4, 0)), min(i1 + 4, nx - 1));
     const int mb = min(max(mp - 3, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
-    const int mc = min(max(mp - 4, max(i0 - 1, 0)), min(i1 + 1, nx - 1));
     const int mu_ = min(max(mp - 4, i0), i1 - 1);
-    const int md = min(max(mp - 6, i0), i1 - 1);
     const int rb = nz * 4;
-    const int os = ms * rb, oa = ma * rb, ob = mb * rb, oc = mc * rb, ou = mu_ * rb, od = md * rb;
+    const int os = ms * rb, oa = ma * rb, ob = mb * rb, ou = mu_ * rb;
 #pragma unroll
     for (int f = 0; f < NS; f++) p.s[f] = bld(RS, vj, slot_s(P, rd, f) * NP4 + os);
 #pragma unroll
@@ -936,23 +933,15 @@ __device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
         p.u[f] = bld(RS, vj, slot_u(P, f) * NP4 + ou);
     }
     p.rhoA = bld(RM, vj, M_RHO * NP4 + oa);
-    p.rhoC = bld(RM, vj, M_RHO * NP4 + oc);
     p.muB = bld(RM, vj, M_MU * NP4 + ob);
-    p.muD = bld(RM, vj, M_MU * NP4 + od);
-    p.lamB = p.lamD = 0.f;
-    if constexpr (P != SP_SH) {
-        p.lamB = bld(RM, vj, M_LAM * NP4 + ob);
-        p.lamD = bld(RM, vj, M_LAM * NP4 + od);
-    }
-    p.jA = p.jC = 1.f;
-    p.nuB = p.gamB = p.nuD = p.gamD = 0.f;
+    p.lamB = 0.f;
+    if constexpr (P != SP_SH) p.lamB = bld(RM, vj, M_LAM * NP4 + ob);
+    p.jA = 1.f;
+    p.nuB = p.gamB = 0.f;
     if constexpr (P == SP_SPIN) {
         p.jA = bld(RM, vj, M_J * NP4 + oa);
-        p.jC = bld(RM, vj, M_J * NP4 + oc);
         p.nuB = bld(RM, vj, M_NU * NP4 + ob);
         p.gamB = bld(RM, vj, M_GAM * NP4 + ob);
-        p.nuD = bld(RM, vj, M_NU * NP4 + od);
-        p.gamD = bld(RM, vj, M_GAM * NP4 + od);
     }
 }
 
@@ -1158,6 +1147,14 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
 
     float S0[4][NS], W1[4][NV], S1[4][NS], W2[4][NV];
+    // materials of rows m-1 (rho, j) and m-3 (lam, mu, nu, gam) by
+    // iteration: three iterations later they are rows m-4 and m-6
+    float RW[4], JW[4], LW[4], MW[4], NW[4], GW[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        RW[u] = JW[u] = 1.f;
+        LW[u] = MW[u] = NW[u] = GW[u] = 0.f;
+    }
     Row2<P> PF[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -1201,6 +1198,8 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
                 vel_finish<P, EX>(vn, dsv, cur.v, cur.rhoA, cur.jA, bxr, bzb, bzt, zint, rin, jin, dt);
+                RW[D] = cur.rhoA;
+                JW[D] = cur.jA;
 #pragma unroll
                 for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
             }
@@ -1210,6 +1209,10 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS];
                 str_row<P, EX>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB, xp_ok, zp_ok, L, KX, KZ, dt);
+                LW[D] = cur.lamB;
+                MW[D] = cur.muB;
+                NW[D] = cur.nuB;
+                GW[D] = cur.gamB;
 #pragma unroll
                 for (int f = 0; f < NS; f++) S1[A][f] = sn[f];
             }
@@ -1221,7 +1224,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 vel_row<P, EX>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, KX, KZ);
                 inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, W1[D], cur.rhoC, cur.jC, bxr, bzb, bzt, zint, rin, jin, dt);
+                vel_finish<P, EX>(vn, dsv, W1[D], RW[A], JW[A], bxr, bzb, bzt, zint, rin, jin, dt);
                 float u1[NV], u2[NV];
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
@@ -1279,7 +1282,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int q = m - 6;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS];
-                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], cur.lamD, cur.muD, cur.nuD, cur.gamD, xp_ok, zp_ok, L, KX, KZ, dt);
+                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok, L, KX, KZ, dt);
                 const bool qok = q >= i0 && q < i1;
                 const unsigned oq = qok ? vo : OOB;
                 const int orow = qok ? q * nz * 4 : 0;
