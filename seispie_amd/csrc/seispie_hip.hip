@@ -147,7 +147,19 @@ __device__ __forceinline__ double dplus(float m1, float c0, float p1, float p2, 
     return c1 * (double)sub_rn(p1, c0) - c2 * (double)sub_rn(p2, m1);
 }
 
-__device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, 64); }
+// z neighbours by whole-wavefront DPP rotations: lane L receives the value
+// of lane L-1 (lane_m1) or L+1 (lane_p1), lane 0 / 63 wrapping to 63 / 0 as
+// a 64-wide shuffle does (checked on gfx950).  One VALU move each, where a
+// shuffle through the LDS crossbar waits on the LDS queue.  Every caller
+// runs them with all 64 lanes active.
+__device__ __forceinline__ float lane_m1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C /* wave_ror:1 */, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_p1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134 /* wave_rol:1 */, 0xf, 0xf, false));
+}
 
 // ------------------------------------------------------------ step kernel
 // Global-address-space pointers: loads through them become global_load with
@@ -432,14 +444,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
             float dsv[NV];
             if constexpr (P == SP_SH) {
                 // div_sy (solver.py:54-58)
-                float zm2 = shfl(S2[1], L - 2), zm1 = shfl(S2[1], L - 1), zp1 = shfl(S2[1], L + 1);
+                float zm1 = lane_m1(S2[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(S2[1]);
                 double ddx = xm_ok ? dmn<EX>(S0[0], S1[0], S2[0], S3[0], KX) : 0.0;
                 double ddz = zm_ok ? dmn<EX>(zm2, zm1, S2[1], zp1, KZ) : 0.0;
                 dsv[0] = (float)(ddx + ddz);
             } else {
                 // div_sxz (solver.py:61-66)
-                float xzm2 = shfl(S2[2], L - 2), xzm1 = shfl(S2[2], L - 1), xzp1 = shfl(S2[2], L + 1);
-                float zzm2 = shfl(S2[1], L - 2), zzm1 = shfl(S2[1], L - 1), zzp1 = shfl(S2[1], L + 1);
+                float xzm1 = lane_m1(S2[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(S2[2]);
+                float zzm1 = lane_m1(S2[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(S2[1]);
                 double dxx = xm_ok ? dmn<EX>(S0[0], S1[0], S2[0], S3[0], KX) : 0.0;
                 double dxxz = xm_ok ? dmn<EX>(S0[2], S1[2], S2[2], S3[2], KX) : 0.0;
                 double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, S2[2], xzp1, KZ) : 0.0;
@@ -448,8 +460,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
                 dsv[1] = (float)(dxxz + dzzz);
                 if constexpr (P == SP_SPIN) {
                     // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
-                    float yzm2 = shfl(S2[5], L - 2), yzm1 = shfl(S2[5], L - 1), yzp1 = shfl(S2[5], L + 1);
-                    float yym2 = shfl(S2[4], L - 2), yym1 = shfl(S2[4], L - 1), yyp1 = shfl(S2[4], L + 1);
+                    float yzm1 = lane_m1(S2[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(S2[5]);
+                    float yym1 = lane_m1(S2[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(S2[4]);
                     double dxyx = xm_ok ? dmn<EX>(S0[3], S1[3], S2[3], S3[3], KX) : 0.0;
                     double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, S2[5], yzp1, KZ) : 0.0;
                     float dsy = (float)(dxyx + dzyz);
@@ -546,7 +558,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
         const unsigned kq = (unsigned)(qr * nz + j);
         if constexpr (P == SP_SH) {
             // div_vy + add_sy (solver.py:112-117, 128-133)
-            float zm1 = shfl(W1[0], L - 1), zp1 = shfl(W1[0], L + 1), zp2 = shfl(W1[0], L + 2);
+            float zm1 = lane_m1(W1[0]), zp1 = lane_p1(W1[0]), zp2 = lane_p1(zp1);
             if (outl) {
                 const float dvdx = xp_ok ? (float)dpl<EX>(W0[0], W1[0], W2[0], W3[0], KX) : 0.f;
                 const float dvdz = zp_ok ? (float)dpl<EX>(zm1, W1[0], zp1, zp2, KZ) : 0.f;
@@ -556,13 +568,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAV
             }
         } else {
             // div_vxz + add_sxz (solver.py:119-126, 143-149)
-            float xm1 = shfl(W1[0], L - 1), xp1 = shfl(W1[0], L + 1), xp2 = shfl(W1[0], L + 2);
-            float zm1 = shfl(W1[1], L - 1), zp1 = shfl(W1[1], L + 1), zp2 = shfl(W1[1], L + 2);
+            float xm1 = lane_m1(W1[0]), xp1 = lane_p1(W1[0]), xp2 = lane_p1(xp1);
+            float zm1 = lane_m1(W1[1]), zp1 = lane_p1(W1[1]), zp2 = lane_p1(zp1);
             float cm1 = 0.f, cp1 = 0.f, cp2 = 0.f;
             if constexpr (P == SP_SPIN) {
-                cm1 = shfl(W1[2], L - 1);
-                cp1 = shfl(W1[2], L + 1);
-                cp2 = shfl(W1[2], L + 2);
+                cm1 = lane_m1(W1[2]);
+                cp1 = lane_p1(W1[2]);
+                cp2 = lane_p1(cp1);
             }
             if (outl) {
                 const float dvxdx = xp_ok ? (float)dpl<EX>(W0[0], W1[0], W2[0], W3[0], KX) : 0.f;
@@ -750,13 +762,13 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             const bool xm_ok = r >= 2 && r < nx - 2;
             float dsv[NV];
             if constexpr (P == SP_SH) {
-                float zm2 = shfl(Sc[1], L - 2), zm1 = shfl(Sc[1], L - 1), zp1 = shfl(Sc[1], L + 1);
+                float zm1 = lane_m1(Sc[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(Sc[1]);
                 double ddx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
                 double ddz = zm_ok ? dmn<EX>(zm2, zm1, Sc[1], zp1, KZ) : 0.0;
                 dsv[0] = (float)(ddx + ddz);
             } else {
-                float xzm2 = shfl(Sc[2], L - 2), xzm1 = shfl(Sc[2], L - 1), xzp1 = shfl(Sc[2], L + 1);
-                float zzm2 = shfl(Sc[1], L - 2), zzm1 = shfl(Sc[1], L - 1), zzp1 = shfl(Sc[1], L + 1);
+                float xzm1 = lane_m1(Sc[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(Sc[2]);
+                float zzm1 = lane_m1(Sc[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(Sc[1]);
                 double dxx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
                 double dxxz = xm_ok ? dmn<EX>(Sa[2], Sb[2], Sc[2], Sd[2], KX) : 0.0;
                 double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, Sc[2], xzp1, KZ) : 0.0;
@@ -764,8 +776,8 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
                 dsv[0] = (float)(dxx + dzxz);
                 dsv[1] = (float)(dxxz + dzzz);
                 if constexpr (P == SP_SPIN) {
-                    float yzm2 = shfl(Sc[5], L - 2), yzm1 = shfl(Sc[5], L - 1), yzp1 = shfl(Sc[5], L + 1);
-                    float yym2 = shfl(Sc[4], L - 2), yym1 = shfl(Sc[4], L - 1), yyp1 = shfl(Sc[4], L + 1);
+                    float yzm1 = lane_m1(Sc[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(Sc[5]);
+                    float yym1 = lane_m1(Sc[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(Sc[4]);
                     double dxyx = xm_ok ? dmn<EX>(Sa[3], Sb[3], Sc[3], Sd[3], KX) : 0.0;
                     double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, Sc[5], yzp1, KZ) : 0.0;
                     float dsy = (float)(dxyx + dzyz);
@@ -845,15 +857,15 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             const bool xp_ok = qr >= 1 && qr < nx - 2;
             const unsigned oq = (outl && qr >= i0 && qr < i1) ? (unsigned)(qr * nz + j) * 4u : OOB;
             if constexpr (P == SP_SH) {
-                float zm1 = shfl(Wb[0], L - 1), zp1 = shfl(Wb[0], L + 1), zp2 = shfl(Wb[0], L + 2);
+                float zm1 = lane_m1(Wb[0]), zp1 = lane_p1(Wb[0]), zp2 = lane_p1(zp1);
                 const float dvdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
                 const float dvdz = zp_ok ? (float)dpl<EX>(zm1, Wb[0], zp1, zp2, KZ) : 0.f;
                 const double dtmu = dt * (double)cur.mu;
                 bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_sadd<EX>(Sa[0], dtmu, (double)dvdx));
                 bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_sadd<EX>(Sa[1], dtmu, (double)dvdz));
             } else {
-                float xm1 = shfl(Wb[0], L - 1), xp1 = shfl(Wb[0], L + 1), xp2 = shfl(Wb[0], L + 2);
-                float zm1 = shfl(Wb[1], L - 1), zp1 = shfl(Wb[1], L + 1), zp2 = shfl(Wb[1], L + 2);
+                float xm1 = lane_m1(Wb[0]), xp1 = lane_p1(Wb[0]), xp2 = lane_p1(xp1);
+                float zm1 = lane_m1(Wb[1]), zp1 = lane_p1(Wb[1]), zp2 = lane_p1(zp1);
                 const float dvxdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
                 const float dvzdx = xp_ok ? (float)dpl<EX>(Wa[1], Wb[1], Wc[1], Wd[1], KX) : 0.f;
                 const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, Wb[0], xp1, xp2, KZ) : 0.f;
@@ -864,7 +876,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
                 bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_snorm<EX>(Sa[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx)));
                 bst(RS, oq, slot_s(P, wr, 2) * NP4, upd_sadd<EX>(Sa[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx))));
                 if constexpr (P == SP_SPIN) {
-                    float cm1 = shfl(Wb[2], L - 1), cp1 = shfl(Wb[2], L + 1), cp2 = shfl(Wb[2], L + 2);
+                    float cm1 = lane_m1(Wb[2]), cp1 = lane_p1(Wb[2]), cp2 = lane_p1(cp1);
                     const float dcdx = xp_ok ? (float)dpl<EX>(Wa[2], Wb[2], Wc[2], Wd[2], KX) : 0.f;
                     const float dcdz = zp_ok ? (float)dpl<EX>(cm1, Wb[2], cp1, cp2, KZ) : 0.f;
                     const double dtg = dt * (double)cur.gam;
@@ -953,13 +965,13 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[
                                         int L, const DCo &KX, const DCo &KZ)
 {
     if constexpr (P == SP_SH) {
-        float zm2 = shfl(sc[1], L - 2), zm1 = shfl(sc[1], L - 1), zp1 = shfl(sc[1], L + 1);
+        float zm1 = lane_m1(sc[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(sc[1]);
         double ddx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
         double ddz = zm_ok ? dmn<EX>(zm2, zm1, sc[1], zp1, KZ) : 0.0;
         dsv[0] = (float)(ddx + ddz);
     } else {
-        float xzm2 = shfl(sc[2], L - 2), xzm1 = shfl(sc[2], L - 1), xzp1 = shfl(sc[2], L + 1);
-        float zzm2 = shfl(sc[1], L - 2), zzm1 = shfl(sc[1], L - 1), zzp1 = shfl(sc[1], L + 1);
+        float xzm1 = lane_m1(sc[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(sc[2]);
+        float zzm1 = lane_m1(sc[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(sc[1]);
         double dxx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
         double dxxz = xm_ok ? dmn<EX>(sa[2], sb[2], sc[2], sd[2], KX) : 0.0;
         double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, sc[2], xzp1, KZ) : 0.0;
@@ -968,8 +980,8 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[
         dsv[1] = (float)(dxxz + dzzz);
         if constexpr (P == SP_SPIN) {
             // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
-            float yzm2 = shfl(sc[5], L - 2), yzm1 = shfl(sc[5], L - 1), yzp1 = shfl(sc[5], L + 1);
-            float yym2 = shfl(sc[4], L - 2), yym1 = shfl(sc[4], L - 1), yyp1 = shfl(sc[4], L + 1);
+            float yzm1 = lane_m1(sc[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(sc[5]);
+            float yym1 = lane_m1(sc[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(sc[4]);
             double dxyx = xm_ok ? dmn<EX>(sa[3], sb[3], sc[3], sd[3], KX) : 0.0;
             double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, sc[5], yzp1, KZ) : 0.0;
             float dsy = (float)(dxyx + dzyz);
@@ -1042,15 +1054,15 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n
                                         const DCo &KX, const DCo &KZ, double dt)
 {
     if constexpr (P == SP_SH) {
-        float zm1 = shfl(wb[0], L - 1), zp1 = shfl(wb[0], L + 1), zp2 = shfl(wb[0], L + 2);
+        float zm1 = lane_m1(wb[0]), zp1 = lane_p1(wb[0]), zp2 = lane_p1(zp1);
         const float dvdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
         const float dvdz = zp_ok ? (float)dpl<EX>(zm1, wb[0], zp1, zp2, KZ) : 0.f;
         const double dtmu = dt * (double)mu;
         sn[0] = upd_sadd<EX>(so[0], dtmu, (double)dvdx);
         sn[1] = upd_sadd<EX>(so[1], dtmu, (double)dvdz);
     } else {
-        float xm1 = shfl(wb[0], L - 1), xp1 = shfl(wb[0], L + 1), xp2 = shfl(wb[0], L + 2);
-        float zm1 = shfl(wb[1], L - 1), zp1 = shfl(wb[1], L + 1), zp2 = shfl(wb[1], L + 2);
+        float xm1 = lane_m1(wb[0]), xp1 = lane_p1(wb[0]), xp2 = lane_p1(xp1);
+        float zm1 = lane_m1(wb[1]), zp1 = lane_p1(wb[1]), zp2 = lane_p1(zp1);
         const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
         const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1], wb[1], wc[1], wd[1], KX) : 0.f;
         const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, wb[0], xp1, xp2, KZ) : 0.f;
@@ -1061,7 +1073,7 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n
         sn[2] = upd_sadd<EX>(so[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
         if constexpr (P == SP_SPIN) {
             // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
-            float cm1 = shfl(wb[2], L - 1), cp1 = shfl(wb[2], L + 1), cp2 = shfl(wb[2], L + 2);
+            float cm1 = lane_m1(wb[2]), cp1 = lane_p1(wb[2]), cp2 = lane_p1(cp1);
             const float dcdx = xp_ok ? (float)dpl<EX>(wa[2], wb[2], wc[2], wd[2], KX) : 0.f;
             const float dcdz = zp_ok ? (float)dpl<EX>(cm1, wb[2], cp1, cp2, KZ) : 0.f;
             const double dtg = dt * (double)gam;
