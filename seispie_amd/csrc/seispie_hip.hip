@@ -13,7 +13,7 @@
 //     (z is the contiguous axis, k = i*nz + j) and marches along x through
 //     a chunk of rows.  It keeps a 4-row register window of the old
 //     stresses and of the new velocities, takes z neighbours with
-//     wavefront shuffles, and writes new velocity, displacement and stress
+//     wavefront (DPP) rotations, and writes new velocity, displacement and stress
 //     for the 58 interior lanes.  Stress divergence, source injection,
 //     sponge, velocity/displacement update, velocity gradient, stress
 //     update, receiver recording and adjoint snapshots are all fused, so a
