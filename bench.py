@@ -202,6 +202,9 @@ def main():
                     help="replay forward runs without snapshots from a captured hipGraph")
     ap.add_argument("--no-fuse", action="store_true",
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
+    ap.add_argument("--fuse", type=int, default=-1,
+                    help="steps per launch option: 0 one step, 1 two steps (k_step2), 2 two steps with two "
+                         "cells per lane (k_step2w; SH and P-SV with even nz)")
     ap.add_argument("--fast", action="store_true",
                     help="step kernels in fast mode (reciprocal multiplications and fused multiply-adds) instead "
                          "of the default exact mode, whose records equal the oracle bit for bit")
@@ -239,6 +242,8 @@ def main():
     ctx = _lib.Context(0)
     if args.no_fuse:
         ctx.set_option(_lib.OPT_FUSE, 0)
+    if args.fuse >= 0:
+        ctx.set_option(_lib.OPT_FUSE, args.fuse)
     if args.fast:
         ctx.set_option(_lib.OPT_EXACT, 0)
     if args.graph:
@@ -266,8 +271,7 @@ def main():
         return 0
     if not args.chunk:
         tuned = ctx.autotune(20)
-        print("# autotuned chunk=%d order=%d two-step launches %s" % (tuned + ("allowed" if ctx.get_option(_lib.OPT_FUSE)
-                                                                             else "off",)),
+        print("# autotuned chunk=%d order=%d steps-per-launch option %d" % (tuned + (ctx.get_option(_lib.OPT_FUSE),)),
               file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
@@ -291,7 +295,12 @@ def main():
     bytes_per_launch = ft["alg_bytes"] / ft["step_launches"]
     achieved = bytes_per_launch / (launch_ms * 1e-3)
     steps_per_launch = bytes_per_launch / init.alg_bytes_per_step()
-    kname = "k_step2<%s> (two time steps per launch)" if steps_per_launch > 1.5 else "k_step3<%s>"
+    wide = ctx.get_option(_lib.OPT_FUSE) == 2 and init.physics != _lib.SPIN and all(g.nz % 2 == 0 for g in init.grids)
+    if steps_per_launch > 1.5:
+        kname = ("k_step2w<%s> (two time steps per launch, two cells per lane)" if wide
+                 else "k_step2<%s> (two time steps per launch)")
+    else:
+        kname = "k_step3<%s>"
     kname = kname % {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]
 
     cells = init.cell_updates(passes=2)
@@ -299,7 +308,8 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tf):
-        t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0] + ("_fused" if steps_per_launch > 1.5 else "")
+        t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0]
+                                    + (("_wide" if wide else "_fused") if steps_per_launch > 1.5 else "")
                                     + ("_fast" if args.fast else ""))
         if t:
             traffic = t["hbm_bytes_per_launch"]

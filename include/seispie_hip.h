@@ -65,7 +65,10 @@ typedef struct sp_ctx sp_ctx;
 #define SP_OPT_KERNEL 5     /* step kernel: 1 buffer-addressed unrolled march (default), 0 plain */
 #define SP_OPT_STRIP 6      /* strip geometry (shift << 16) | width: output lanes [shift, shift + width) */
 #define SP_OPT_FUSE 7       /* 1 (default): shots advance two steps per launch wherever no snapshot,
-                               kernel interaction or checkpoint needs the state between them; 0: one step */
+                               kernel interaction or checkpoint needs the state between them; 2: the same
+                               with two z cells per lane (SH and P-SV batches whose grids all have even nz;
+                               others use 1); 0: one step.  All give the same results bit for bit; the
+                               autotuner picks among them */
 #define SP_OPT_EXACT 8      /* 1 (default): the step kernels perform the reference's float64 operations one
                                by one in its order (correctly rounded quotients, no fused multiply-add), so
                                records equal the reference arithmetic bit for bit at any length; 0:

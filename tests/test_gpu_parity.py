@@ -402,6 +402,55 @@ def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order
     _check_forward(prob, r1)
 
 
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV])
+@pytest.mark.parametrize("chunk,order,sae", [(16, 0, 10), (4, 1, 5), (32, 3, 1)])
+def test_two_cells_per_lane_equals_one(ctx_factory, physics, chunk, order, sae):
+    """SP_OPT_FUSE 2 (k_step2w, two z cells per lane) gives the records,
+    misfit and kernels of k_step2 bit for bit, and the records equal the
+    oracle: even-nz ragged grids with receivers on every row and on the
+    strip edges (z 114..117 and 0..3), sources on the grid corners."""
+    nt = 120
+    probs = [_small(physics, 67, 250, nt, 2, seed=6), _small(physics, 40, 60, nt, 1, seed=7)]
+    for p in probs:
+        g = p.grids[0]
+        nz = g.nz
+        rec = [i * nz + (i % 4) + (114 if nz > 120 and i % 2 else 0) for i in range(g.nx)]
+        rec += [16 * nz + 20, 16 * nz + 20, 31 * nz + 7, 5 * nz + nz - 1]
+        src = [15 * nz + 30, 16 * nz + min(117, nz - 2), 0, g.nx * nz - 1, 3 * nz + 55, 20 * nz + 1]
+        src_k = np.array(sorted(src), np.int32)
+        comps = configs.ricker_stf(nt, 5e-4, 30.0, ang=30.0)
+        stf = configs.shot_stf(physics, comps)
+        stf_many = np.concatenate([stf * (1 + 0.1 * q) for q in range(src_k.size)], axis=1)
+        g.shots = [configs.Shot(src_k, stf_many, np.array(rec, np.int32)) for _ in g.shots]
+    prob = configs.Problem(physics, nt, 5e-4, [p.grids[0] for p in probs], sae=sae)
+    data = [d * 1.01 for d in _data_for(prob)]
+    print("case: wide", physics, chunk, order, "sae", sae, flush=True)
+    out = []
+    for fuse in (1, 2):
+        ctx = ctx_factory()
+        ctx.set_option(_lib.OPT_FUSE, fuse)
+        ctx.set_option(_lib.OPT_CHUNK, chunk)
+        ctx.set_option(_lib.OPT_ORDER, order)
+        ids = configs.load_problem(ctx, prob)
+        ctx.forward(0)
+        recs = [ctx.records(i) for i in ids]
+        for i, d in zip(ids, data):
+            ctx.set_data(i, d)
+        norms = ctx.gradient(prob.sae)
+        kern = [[ctx.kernel(g, w) for w in (_lib.K_LAM, _lib.K_MU, _lib.K_RHO, _lib.K_HESS)]
+                for g in range(len(prob.grids))]
+        out.append((recs, norms, kern))
+    (r1, n1, k1), (r2, n2, k2) = out
+    for a, b in zip(r1, r2):
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(n1, n2)
+    for ga, gb in zip(k1, k2):
+        for a, b in zip(ga, gb):
+            np.testing.assert_array_equal(a, b)
+    _check_forward(prob, r2, exact=True)
+
+
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
 def test_graph_replayed_forward_equals_eager(ctx_factory, physics):
     """Forward runs replayed from a captured graph (SP_OPT_GRAPH) record
