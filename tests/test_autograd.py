@@ -52,6 +52,35 @@ def test_checkpointed_adjoint_equals_stored_snapshots(ctx_factory, physics, sae)
             np.testing.assert_array_equal(a, b, err_msg="kernel %d interval %d" % (w, C))
 
 
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV])
+def test_checkpointed_adjoint_with_two_cells_per_lane(ctx_factory, physics):
+    """The checkpointed forward and adjoint with SP_OPT_FUSE 2 (k_step2w for
+    the two-step launches, including the recomputed segments) give the
+    stored-snapshot gradient of one cell per lane bit for bit."""
+    prob = _small(physics, 60, 130, 200, 2)
+    prob.sae = 10
+    data = _data_for(prob)
+    print("case: ckpt two cells per lane", physics, flush=True)
+    ctx, ids = _ctx_with(ctx_factory, prob, data)
+    ctx.set_option(_lib.OPT_CHUNK, 16)
+    norms = ctx.gradient(prob.sae)
+    ref_k = [ctx.kernel(0, w) for w in range(_lib.K_VP)] + [ctx.kernel(0, _lib.K_HESS)]
+    ref_r = [ctx.records(i) for i in ids]
+    ctx2, ids2 = _ctx_with(ctx_factory, prob, data)
+    ctx2.set_option(_lib.OPT_FUSE, 2)
+    ctx2.set_option(_lib.OPT_CHUNK, 16)
+    for C in (10, 6):
+        print("interval", C, flush=True)
+        ctx2.forward_ckpt(C)
+        for i in ids2:
+            np.testing.assert_array_equal(ctx2.records(i), ref_r[i])
+        np.testing.assert_array_equal(ctx2.misfit(), norms)
+        ctx2.adjoint_ckpt(prob.sae)
+        got = [ctx2.kernel(0, w) for w in range(_lib.K_VP)] + [ctx2.kernel(0, _lib.K_HESS)]
+        for w, (x, y) in enumerate(zip(got, ref_k)):
+            np.testing.assert_array_equal(x, y, err_msg="kernel %d interval %d" % (w, C))
+
+
 def test_checkpoint_errors(ctx_factory):
     prob = _small(_lib.PSV, nt=40, nshots=1)
     ctx = ctx_factory()

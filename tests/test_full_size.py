@@ -2,7 +2,9 @@
 size-independent properties that hold bit for bit.
 
 * Batching: every shot of the 128-shot 1000^2 batch (and of the mixed
-  64-shot batch on three grids) records exactly what it records alone.
+  64-shot batch on three grids) records exactly what it records alone;
+  the batch runs the two-step kernel with two cells per lane, the single
+  shots the one with one cell per lane.
 * Linearity: scaling every source time function by 2 scales every record
   by 2.  A power-of-two scale commutes with every float32 and float64
   rounding except in the subnormal range: the wavefront's leading edge
@@ -45,7 +47,7 @@ def _single(prob, g, shot):
 def test_batched_shots_equal_single_runs_at_full_size(ctx_factory, name):
     prob = configs.psv_config3() if name == "3a" else configs.psv_config3_mixed()
     print("case: full-size batch", prob.name, prob.nshots, "shots", flush=True)
-    batch = _records(ctx_factory, prob)
+    batch = _records(ctx_factory, prob, fuse=2, chunk=128)
     flat = [(g, s) for g in prob.grids for s in g.shots]
     for k in (0, len(flat) // 2 + 1, len(flat) - 1):
         g, s = flat[k]
@@ -70,7 +72,7 @@ def test_records_are_exactly_linear_in_the_source_at_full_size(ctx_factory):
 def test_one_shot_of_the_benchmark_matches_the_oracle(ctx_factory):
     """Shot 0 of the 128-shot 1000^2 workload over all 4000 steps against
     the oracle (float32 storage, float64 arithmetic; ~15 s of host time).
-    The default exact mode must equal it bit for bit.  The fast mode
+    The default exact mode must equal it bit for bit, with one or two cells per lane.  The fast mode
     (reciprocal multiplications and fused multiply-adds) is held to twice
     the seismogram tolerance; its float32 rounding differences grow chaotically with the
     number of steps, to ~1e-5 here, the size of the gap between the
@@ -83,6 +85,8 @@ def test_one_shot_of_the_benchmark_matches_the_oracle(ctx_factory):
     assert np.abs(want).max() > 0
     exact = _records(ctx_factory, _single(prob, g, g.shots[0]))[0]
     np.testing.assert_array_equal(exact, want)
+    wide = _records(ctx_factory, _single(prob, g, g.shots[0]), fuse=2, chunk=128)[0]
+    np.testing.assert_array_equal(wide, want)
     fast = _records(ctx_factory, _single(prob, g, g.shots[0]), exact=0)[0]
     err = rel_l2(fast, want)
     print("shot 0, fast mode: rel L2 %.2e" % err)
