@@ -113,8 +113,8 @@ struct StepArgs {
     // of each (u before the step, v after it)
     int snap_v, snap_u;
     int adjoint;   // inject adjoint sources at the receivers, record nothing
-    int shift;     // lanes before the first output cell of a strip (>= 3)
-    int wout;      // output cells per strip (shift + wout <= 61)
+    int shift;     // cells of a strip before its first output cell (>= 3)
+    int wout;      // output cells per strip (shift + wout <= 61; k_step2w: 6 + 116)
 };
 
 // float32 operations the reference performs in float32 (float32 * float32
