@@ -451,17 +451,19 @@ def test_two_cells_per_lane_equals_one(ctx_factory, physics, chunk, order, sae):
     _check_forward(prob, r2, exact=True)
 
 
-@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
-def test_graph_replayed_forward_equals_eager(ctx_factory, physics):
+@pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
+def test_graph_replayed_forward_equals_eager(ctx_factory, physics, fuse):
     """Forward runs replayed from a captured graph (SP_OPT_GRAPH) record
     exactly what eager launches record, on repeated runs with other work
-    (a gradient) between them; nt leaves a remainder past the last block."""
-    prob = _small(physics, nt=150, nshots=2)
+    (a gradient) between them; nt leaves a remainder past the last block.
+    fuse 2 runs the two-cell kernel (even nz) in both."""
+    prob = _small(physics, nt=150, nshots=2) if fuse == 1 else _small(physics, 70, 130, 150, 2)
     data = _data_for(prob) if physics != _lib.SPIN else None
-    print("case: graph", physics, flush=True)
-    _, eager = _run_forward(ctx_factory, prob)
+    print("case: graph", physics, "fuse", fuse, flush=True)
+    _, eager = _run_forward(ctx_factory, prob, fuse=fuse)
     ctx = ctx_factory()
     ctx.set_option(_lib.OPT_GRAPH, 1)
+    ctx.set_option(_lib.OPT_FUSE, fuse)
     ids = configs.load_problem(ctx, prob)
     for rep in range(3):
         ctx.forward(0)
@@ -473,15 +475,17 @@ def test_graph_replayed_forward_equals_eager(ctx_factory, physics):
             ctx.gradient(prob.sae)
 
 
-@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
-def test_stepwise_forward_and_wavefields(ctx_factory, physics):
+@pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
+def test_stepwise_forward_and_wavefields(ctx_factory, physics, fuse):
     """A forward run in pieces records what one run records (bit for bit),
     and the velocities read back after step k equal the oracle's velocity
-    after step k (its adjoint snapshot of a k+1-step run)."""
-    prob = _small(physics, nt=120, nshots=2)
-    print("case: stepwise", physics, flush=True)
-    _, whole = _run_forward(ctx_factory, prob)
+    after step k (its adjoint snapshot of a k+1-step run).  fuse 2 runs the
+    two-cell kernel (even nz)."""
+    prob = _small(physics, nt=120, nshots=2) if fuse == 1 else _small(physics, 70, 130, 120, 2)
+    print("case: stepwise", physics, "fuse", fuse, flush=True)
+    _, whole = _run_forward(ctx_factory, prob, fuse=fuse)
     ctx = ctx_factory()
+    ctx.set_option(_lib.OPT_FUSE, fuse)
     ids = configs.load_problem(ctx, prob)
     k = 37
     ctx.forward_steps(0, k + 1)
