@@ -145,12 +145,14 @@ def test_edge_and_coincident_sources_and_receivers(ctx_factory):
     np.testing.assert_array_equal(recs[0][:, 4], recs[0][:, 5])
 
 
-def test_tiny_grids(ctx_factory):
-    """Grids smaller than the stencil bands and than one strip."""
+@pytest.mark.parametrize("fuse", [1, 2])
+def test_tiny_grids(ctx_factory, fuse):
+    """Grids smaller than the stencil bands and than one strip (fuse 2: the
+    even-nz ones with two cells per lane)."""
     for nx, nz in ((5, 4), (3, 70), (9, 9)):
         prob = _small(_lib.SH, nx, nz, 50, 1, nrec=2, sponge=(3, 0.1, 1, 1, 1, 1))
-        print("case: tiny", nx, nz, flush=True)
-        _, recs = _run_forward(ctx_factory, prob)
+        print("case: tiny", nx, nz, "fuse", fuse, flush=True)
+        _, recs = _run_forward(ctx_factory, prob, fuse=fuse)
         _check_forward(prob, recs, nonzero=False)
 
 
