@@ -6,9 +6,11 @@
 // Here one launch advances every shot of a batch by one or two time steps:
 //
 //   * k_step2<PHYS> (two steps per launch, the default wherever the state
-//     between the steps is not needed), k_step3<PHYS> (one step; the same
-//     march with buffer loads and stores) and k_step<PHYS> (plain one-step
-//     fallback for grids too large for 32-bit buffer offsets).  In each,
+//     between the steps is not needed), k_step2w<SH|PSV> (the same with two
+//     z cells per lane: 8-byte accesses, 116 outputs per wavefront),
+//     k_step3<PHYS> (one step; the same march with buffer loads and stores)
+//     and k_step<PHYS> (plain one-step fallback for grids too large for
+//     32-bit buffer offsets).  In each,
 //     a wavefront owns a strip of 64 consecutive z cells
 //     (z is the contiguous axis, k = i*nz + j) and marches along x through
 //     a chunk of rows.  It keeps a 4-row register window of the old
@@ -34,6 +36,12 @@
 // correction step, and nothing is contracted into fused multiply-adds.
 // SP_OPT_EXACT = 0 multiplies by reciprocals and contracts, which moves
 // single results by one float32 rounding step on rare cells.
+//
+// This file holds the host side (contexts, batches, runs, autotuner) and the
+// C ABI.  The device code is in the headers included below, one per part:
+// sp_layout.h (state slots, descriptors), sp_arith.h (the difference
+// arithmetic), k_step.h, k_step3.h, k_step2.h, k_step2w.h (step kernels) and
+// sp_kernels.h (records, snapshots, interaction, misfit, smoothing).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,1905 +59,13 @@
 
 namespace {
 
-// ------------------------------------------------------------------ layout
-// Stress, velocity and displacement fields per physics.
-//   SH:   s = {sxy, szy}                         v = {vy}          u = {uy}
-//   P-SV: s = {sxx, szz, sxz}                    v = {vx, vz}      u = {ux, uz}
-//   spin: s = {sxx, szz, sxz, syx_c, syy_c, syz_c} v = {vx, vz, vy_c} u = {ux, uz, uy_c}
-// State slots of a shot: two stress sets, two velocity sets, one u set.
-__host__ __device__ constexpr int n_s(int p) { return p == SP_SH ? 2 : (p == SP_PSV ? 3 : 6); }
-__host__ __device__ constexpr int n_v(int p) { return p == SP_SH ? 1 : (p == SP_PSV ? 2 : 3); }
-__host__ __device__ constexpr int n_slots(int p) { return 2 * n_s(p) + 3 * n_v(p); }
-__host__ __device__ constexpr int slot_s(int p, int set, int f) { return set * n_s(p) + f; }
-__host__ __device__ constexpr int slot_v(int p, int set, int f) { return 2 * n_s(p) + set * n_v(p) + f; }
-__host__ __device__ constexpr int slot_u(int p, int f) { return 2 * n_s(p) + 2 * n_v(p) + f; }
-// snapshot fields: u (before the step) then v (after it), solver.py:193-242
-__host__ __device__ constexpr int n_snap(int p) { return 2 * n_v(p); }
-// components of the records (SH y; P-SV x, z; spin x, z, y_c), of the
-// injected sources (SH y; P-SV/spin x, z) and of the misfit.
-__host__ __device__ constexpr int n_rec(int p) { return n_v(p); }
-__host__ __device__ constexpr int n_inj(int p) { return p == SP_SH ? 1 : 2; }
-__host__ __device__ constexpr int n_mis(int p) { return p == SP_SH ? 1 : 2; }
-// material slots of a grid
-constexpr int M_RHO = SP_F_RHO, M_LAM = SP_F_LAM, M_MU = SP_F_MU, M_NU = SP_F_NU, M_J = SP_F_J;
-constexpr int M_GAM = 8;  // float32(mu_c + nu_c), as the reference adds them (solver.py:140)
-constexpr int N_MAT = 9;
-
-struct ShotDesc {
-    const float *mat;   // [N_MAT][Npad]
-    const float *bx;    // [nx] x-side sponge factors (already float32-rounded)
-    const double *bzb;  // [nz] bottom factor or 1
-    const double *bzt;  // [nz] top factor or 1
-    float *state;       // [n_slots][Npad]
-    float *rec;         // [n_rec][nrec][nt]
-    float *snap;        // [nsa][n_snap][Npad] or null
-    const float *stf;   // [n_inj][nsrc][nt]
-    float *adstf;       // [n_mis][nrec][nt]
-    const int *src_k, *src_row, *rec_k, *rec_row;  // sorted by k
-    int nsrc, nrec, nx, nz;
-    long long Npad;
-    double c1x, c2x, c1z, c2z;  // 9/(8h), 1/(24h)
-    // exact mode: 9/(8h) and 1/(24h) as unevaluated sums hi + lo of two doubles
-    double k9hx, k9lx, k1hx, k1lx, k9hz, k9lz, k1hz, k1lz;
-};
-
-// read-only, wave-uniform data through the constant address space: scalar
-// loads, counted by lgkmcnt and never by the vector memory counter
-typedef const int __attribute__((address_space(4))) *cint_p;
-typedef const float __attribute__((address_space(4))) *cflt_p;
-
-struct StepArgs {
-    const ShotDesc *shots;
-    const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-4) nz,
-                        // the first receiver with k >= (i0-4) nz and the first with k >= i0 nz
-    int nitems;
-    int it, nt;         // step of this launch (plus *itp when itp is set)
-    const int *itp;     // device step counter of a replayed graph, or null
-    double dt;
-    int rd;        // stress/velocity set read this step
-    int snap_idx;  // -1: no snapshot this step (one-step kernel k_step)
-    // two-step kernel: the snapshot of the pair's first step (snap_v) and
-    // of its second step (snap_u), -1: none; the kernel writes both halves
-    // of each (u before the step, v after it)
-    int snap_v, snap_u;
-    int adjoint;   // inject adjoint sources at the receivers, record nothing
-    int shift;     // cells of a strip before its first output cell (>= 3)
-    int wout;      // output cells per strip (shift + wout <= 61; k_step2w: 6 + 116)
-};
-
-// float32 operations the reference performs in float32 (float32 * float32
-// stays float32 under Numba's typing); contraction off so they are never
-// folded into a float64 multiply-add.
-__device__ __forceinline__ float mul_rn(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a * b;
-}
-__device__ __forceinline__ float add_rn(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a + b;
-}
-__device__ __forceinline__ float sub_rn(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a - b;
-}
-
-// D- at row r from values at r-2, r-1, r, r+1 (solver.py:22-27, 38-43)
-__device__ __forceinline__ double dminus(float m2, float m1, float c0, float p1, double c1, double c2)
-{
-    return c1 * (double)sub_rn(c0, m1) - c2 * (double)sub_rn(p1, m2);
-}
-// D+ at row q from values at q-1, q, q+1, q+2 (solver.py:30-35, 46-51)
-__device__ __forceinline__ double dplus(float m1, float c0, float p1, float p2, double c1, double c2)
-{
-    return c1 * (double)sub_rn(p1, c0) - c2 * (double)sub_rn(p2, m1);
-}
-
-// z neighbours by whole-wavefront DPP rotations: lane L receives the value
-// of lane L-1 (lane_m1) or L+1 (lane_p1), lane 0 / 63 wrapping to 63 / 0 as
-// a 64-wide shuffle does (checked on gfx950).  One VALU move each, where a
-// shuffle through the LDS crossbar waits on the LDS queue.  Every caller
-// runs them with all 64 lanes active.
-__device__ __forceinline__ float lane_m1(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C /* wave_ror:1 */, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float lane_p1(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134 /* wave_rol:1 */, 0xf, 0xf, false));
-}
-
-// ------------------------------------------------------------ step kernel
-// Global-address-space pointers: loads through them become global_load with
-// a scalar base and one 32-bit offset per row shared by every field.
-typedef const float __attribute__((address_space(1))) *gcf;
-typedef float __attribute__((address_space(1))) *gf;
-
-// 1/x in float64 to within an ulp: hardware estimate + two Newton steps
-__device__ __forceinline__ double rcp64(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return fma(r, fma(-x, r, 1.0), r);
-}
-
-// ---- exact mode: the reference's float64 operations, one rounding each, in
-// its order (no contraction), so results equal the oracle bit for bit.
-// a / c correctly rounded from rc = RN(1/c): q = RN(a rc), the remainder
-// a - q c is exact by FMA, and RN(q + r rc) is the correctly rounded quotient
-// (Markstein's correction step; also checked against IEEE division on
-// 2e8 random operands of these ranges).
-__device__ __forceinline__ double rn_div(double a, double c, double rc)
-{
-#pragma clang fp contract(off)
-    const double q = a * rc;
-    const double r = __builtin_fma(-q, c, a);
-    return __builtin_fma(r, rc, q);
-}
-// RN(1/x) for float32 x: three Newton steps from the hardware estimate.  Two
-// already give the correctly rounded reciprocal of every float32 mantissa
-// from any start within 2^-20 (exhaustive check); the third keeps it so from
-// starts as far off as 2^-14.
-__device__ __forceinline__ double rcp_rn(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-}
-// RN(num x / den) as fma(x, hi, x lo) with num/den split into hi + lo (the
-// host checks the identity for every float32 mantissa of x)
-__device__ __forceinline__ double kmul(double x, double hi, double lo)
-{
-    return __builtin_fma(x, hi, x * lo);
-}
-// D- and D+ as the reference writes them: 9 (a) / (8h) - (b) / (24h)
-__device__ __forceinline__ double dminus_x(float m2, float m1, float c0, float p1, double k9h, double k9l,
-                                           double k1h, double k1l)
-{
-#pragma clang fp contract(off)
-    return kmul((double)sub_rn(c0, m1), k9h, k9l) - kmul((double)sub_rn(p1, m2), k1h, k1l);
-}
-__device__ __forceinline__ double dplus_x(float m1, float c0, float p1, float p2, double k9h, double k9l,
-                                          double k1h, double k1l)
-{
-#pragma clang fp contract(off)
-    return kmul((double)sub_rn(p1, c0), k9h, k9l) - kmul((double)sub_rn(p2, m1), k1h, k1l);
-}
-// add_vy / add_vxz: v = bound (v + dt ds / rho), then u = u + v dt
-__device__ __forceinline__ float vel_x(float vo, float ds, double dt, double rho, double irho, double b)
-{
-#pragma clang fp contract(off)
-    return (float)(b * ((double)vo + rn_div(dt * (double)ds, rho, irho)));
-}
-__device__ __forceinline__ float disp_x(float u, float v, double dt)
-{
-#pragma clang fp contract(off)
-    return (float)((double)u + (double)v * dt);
-}
-// s + f * g, one rounding per operation (add_sy and the spin couple stresses)
-__device__ __forceinline__ float sadd_x(float s, double f, double g)
-{
-#pragma clang fp contract(off)
-    return (float)((double)s + f * g);
-}
-// add_sxz: s + dt (l2m d1 + float32(lam d2))
-__device__ __forceinline__ float snorm_x(float s, double dt, double l2m, float d1, float lam_d2)
-{
-#pragma clang fp contract(off)
-    return (float)((double)s + dt * (l2m * (double)d1 + (double)lam_d2));
-}
-// add_sy_c: syy_c + 2 dt nu (-vy_c - 0.5 float32(dvzdx - dvxdz))
-__device__ __forceinline__ float srot_x(float s, double dt2, float nu, float vyc, float curl)
-{
-#pragma clang fp contract(off)
-    return (float)((double)s + dt2 * (double)nu * (-(double)vyc - 0.5 * (double)curl));
-}
-
-// the updates in either mode (fast: contraction allowed)
-template <bool EX>
-__device__ __forceinline__ float upd_sadd(float s, double f, double g)
-{
-    if constexpr (EX) return sadd_x(s, f, g);
-    else return (float)((double)s + f * g);
-}
-template <bool EX>
-__device__ __forceinline__ float upd_snorm(float s, double dt, double l2m, float d1, float lam_d2)
-{
-    if constexpr (EX) return snorm_x(s, dt, l2m, d1, lam_d2);
-    else return (float)((double)s + dt * (l2m * (double)d1 + (double)lam_d2));
-}
-template <bool EX>
-__device__ __forceinline__ float upd_srot(float s, double dt2, float nu, float vyc, float curl)
-{
-    if constexpr (EX) return srot_x(s, dt2, nu, vyc, curl);
-    else return (float)((double)s + dt2 * (double)nu * (-(double)vyc - 0.5 * (double)curl));
-}
-template <bool EX>
-__device__ __forceinline__ float upd_disp(float u, float v, double dt)
-{
-    if constexpr (EX) return disp_x(u, v, dt);
-    else return (float)((double)u + (double)v * dt);
-}
-// derivative coefficients of a shot in both forms
-struct DCo {
-    double c1, c2, k9h, k9l, k1h, k1l;
-};
-template <bool EX>
-__device__ __forceinline__ double dmn(float m2, float m1, float c0, float p1, const DCo &k)
-{
-    if constexpr (EX) return dminus_x(m2, m1, c0, p1, k.k9h, k.k9l, k.k1h, k.k1l);
-    else return dminus(m2, m1, c0, p1, k.c1, k.c2);
-}
-template <bool EX>
-__device__ __forceinline__ double dpl(float m1, float c0, float p1, float p2, const DCo &k)
-{
-    if constexpr (EX) return dplus_x(m1, c0, p1, p2, k.k9h, k.k9l, k.k1h, k.k1l);
-    else return dplus(m1, c0, p1, p2, k.c1, k.c2);
-}
-
-// Everything iteration m of the row march reads from memory, fetched one
-// iteration ahead so that a wavefront's loads overlap its own arithmetic.
-// Row and lane indices are clamped into the grid so every load is valid;
-// values fetched for rows or lanes outside the grid are never used.
-template <int P>
-struct RowLoads {
-    float s[n_s(P)];   // old stresses, row m
-    float v[n_v(P)];   // old velocities, row r = m-1
-    float u[n_v(P)];   // displacement, row r
-    float rho, jj, bx; // row r
-    float lam, mu, nu, gam;  // row q = m-3
-};
-
-struct StepPtrs {
-    gcf s_in[6];
-    gcf v_in[3];
-    gcf u_in[3];
-    gcf rho, jj, lam, mu, nu, gam, bx;
-};
-
-template <int P>
-__device__ __forceinline__ void load_row(RowLoads<P> &p, const StepPtrs &q, int m, int nx, int nz, int jc,
-                                         int i0, int i1)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    const int ms = min(max(m, 0), nx - 1);
-    const int mr = min(max(m - 1, 0), nx - 1);
-    const int mq = min(max(m - 3, i0), i1 - 1);
-    const unsigned os = (unsigned)(ms * nz + jc), orr = (unsigned)(mr * nz + jc), oq = (unsigned)(mq * nz + jc);
-#pragma unroll
-    for (int f = 0; f < NS; f++) p.s[f] = q.s_in[f][os];
-#pragma unroll
-    for (int f = 0; f < NV; f++) {
-        p.v[f] = q.v_in[f][orr];
-        p.u[f] = q.u_in[f][orr];
-    }
-    p.rho = q.rho[orr];
-    p.bx = q.bx[mr];
-    p.jj = 1.f;
-    if constexpr (P == SP_SPIN) p.jj = q.jj[orr];
-    p.mu = q.mu[oq];
-    p.lam = p.nu = p.gam = 0.f;
-    if constexpr (P != SP_SH) p.lam = q.lam[oq];
-    if constexpr (P == SP_SPIN) {
-        p.nu = q.nu[oq];
-        p.gam = q.gam[oq];
-    }
-}
-
-#ifndef SP_STEP_WAVES
-#define SP_STEP_WAVES 1
-#endif
-template <int P, bool EX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SP_STEP_WAVES))) void k_step(StepArgs a)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    // Blocks are dispatched round-robin over the 8 XCDs; remap them so that
-    // each XCD (and its L2) walks a contiguous run of the work list.
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int q8 = nb >> 3, r8 = nb & 7, bx8 = b & 7, by8 = b >> 3;
-    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
-    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
-    if (wave >= a.nitems) return;  // whole wavefront leaves together
-    const int L = threadIdx.x & 63;
-    const int4 item = a.items[2 * wave];
-    if (item.x < 0) return;        // padding entry of the work list
-    const int4 item2 = a.items[2 * wave + 1];
-    const ShotDesc &d = a.shots[item.x];
-    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
-    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
-    const int j = item.y - 3 + L;
-    const int jc = min(max(j, 0), nz - 1);
-    const bool jin = j >= 0 && j < nz;
-    const bool outl = jin && L >= 3 && L < 61;
-    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
-    const long long NP = d.Npad;
-    const double dt = a.dt, dt2 = 2.0 * a.dt;
-    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
-    const int rd = a.rd, wr = a.rd ^ 1;
-    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
-
-    gf st = (gf)d.state;
-    gcf mat = (gcf)d.mat;
-    StepPtrs q;
-#pragma unroll
-    for (int f = 0; f < NS; f++) q.s_in[f] = st + slot_s(P, rd, f) * NP;
-#pragma unroll
-    for (int f = 0; f < NV; f++) {
-        q.v_in[f] = st + slot_v(P, rd, f) * NP;
-        q.u_in[f] = st + slot_u(P, f) * NP;
-    }
-    q.rho = mat + M_RHO * NP;
-    q.jj = mat + M_J * NP;
-    q.lam = mat + M_LAM * NP;
-    q.mu = mat + M_MU * NP;
-    q.nu = mat + M_NU * NP;
-    q.gam = mat + M_GAM * NP;
-    q.bx = (gcf)d.bx;
-    gf s_out[NS];
-    gf v_out[NV];
-    gf u_out[NV];
-#pragma unroll
-    for (int f = 0; f < NS; f++) s_out[f] = st + slot_s(P, wr, f) * NP;
-#pragma unroll
-    for (int f = 0; f < NV; f++) {
-        v_out[f] = st + slot_v(P, wr, f) * NP;
-        u_out[f] = st + slot_u(P, f) * NP;
-    }
-
-    // injection list: sources (forward) or receivers with adjoint sources
-    const int *ik = a.adjoint ? d.rec_k : d.src_k;
-    const int *irow = a.adjoint ? d.rec_row : d.src_row;
-    const int ni = a.adjoint ? d.nrec : d.nsrc;
-    const float *ival = a.adjoint ? d.adstf : d.stf;
-    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
-    // first list entries at or after the chunk's rows, found on the host
-    int pi = a.adjoint ? item2.y : item2.x;
-    int pr = item2.z;
-    const bool record = !a.adjoint;
-
-    float S0[NS], S1[NS], S2[NS], S3[NS];
-    float W0[NV], W1[NV], W2[NV], W3[NV];
-#pragma unroll
-    for (int f = 0; f < NS; f++) S0[f] = S1[f] = S2[f] = S3[f] = 0.f;
-#pragma unroll
-    for (int f = 0; f < NV; f++) W0[f] = W1[f] = W2[f] = W3[f] = 0.f;
-
-    // two rows in flight ahead of the one being computed
-    RowLoads<P> n1, n2;
-    load_row<P>(n1, q, i0 - 3, nx, nz, jc, i0, i1);
-    load_row<P>(n2, q, i0 - 2, nx, nz, jc, i0, i1);
-    for (int m = i0 - 3; m <= i1 + 2; ++m) {
-        const RowLoads<P> cur = n1;
-        n1 = n2;
-        if (m + 2 <= i1 + 2) load_row<P>(n2, q, m + 2, nx, nz, jc, i0, i1);
-#pragma unroll
-        for (int f = 0; f < NS; f++) {
-            S0[f] = S1[f];
-            S1[f] = S2[f];
-            S2[f] = S3[f];
-            S3[f] = cur.s[f];
-        }
-        if (m < i0) continue;
-
-        // ---- new velocities of row r = m-1 (stress window rows r-2..r+1)
-        const int r = m - 1;
-        float vn[NV];
-#pragma unroll
-        for (int f = 0; f < NV; f++) vn[f] = 0.f;
-        if (r >= 0 && r < nx) {
-            const bool xm_ok = r >= 2 && r < nx - 2;
-            float dsv[NV];
-            if constexpr (P == SP_SH) {
-                // div_sy (solver.py:54-58)
-                float zm1 = lane_m1(S2[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(S2[1]);
-                double ddx = xm_ok ? dmn<EX>(S0[0], S1[0], S2[0], S3[0], KX) : 0.0;
-                double ddz = zm_ok ? dmn<EX>(zm2, zm1, S2[1], zp1, KZ) : 0.0;
-                dsv[0] = (float)(ddx + ddz);
-            } else {
-                // div_sxz (solver.py:61-66)
-                float xzm1 = lane_m1(S2[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(S2[2]);
-                float zzm1 = lane_m1(S2[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(S2[1]);
-                double dxx = xm_ok ? dmn<EX>(S0[0], S1[0], S2[0], S3[0], KX) : 0.0;
-                double dxxz = xm_ok ? dmn<EX>(S0[2], S1[2], S2[2], S3[2], KX) : 0.0;
-                double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, S2[2], xzp1, KZ) : 0.0;
-                double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, S2[1], zzp1, KZ) : 0.0;
-                dsv[0] = (float)(dxx + dzxz);
-                dsv[1] = (float)(dxxz + dzzz);
-                if constexpr (P == SP_SPIN) {
-                    // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
-                    float yzm1 = lane_m1(S2[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(S2[5]);
-                    float yym1 = lane_m1(S2[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(S2[4]);
-                    double dxyx = xm_ok ? dmn<EX>(S0[3], S1[3], S2[3], S3[3], KX) : 0.0;
-                    double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, S2[5], yzp1, KZ) : 0.0;
-                    float dsy = (float)(dxyx + dzyz);
-                    double dzyy = zm_ok ? dmn<EX>(yym2, yym1, S2[4], yyp1, KZ) : 0.0;
-                    double dxyy = xm_ok ? dmn<EX>(S0[4], S1[4], S2[4], S3[4], KX) : 0.0;
-                    dsv[0] = (float)((double)dsv[0] + dzyy);
-                    dsv[1] = (float)((double)dsv[1] - dxyy);
-                    dsv[2] = (float)((double)dsy + 2.0 * (double)S2[4]);
-                }
-            }
-            // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list order
-            const int kend = (r + 1) * nz, kr = r * nz + j;
-            while (pi < ni && ik[pi] < kend) {
-                if (ik[pi] == kr) {
-                    const int row = irow[pi];
-                    dsv[0] = add_rn(dsv[0], ival[(long long)row * a.nt + itx]);
-                    if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], ival[(nrows + row) * a.nt + itx]);
-                }
-                ++pi;
-            }
-            // add_vy / add_vxz (solver.py:95-109) with the separable sponge of
-            // set_bound (fd2d.py:47-68): f32(f32(bx*bottom)*top)
-            const float b1 = (float)((double)cur.bx * bzb);
-            const double b = (double)(float)((double)b1 * bzt);
-            if constexpr (EX) {
-                const double rr = (double)cur.rho, irr = rcp_rn(rr);
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    double rf = rr, ir = irr;
-                    if constexpr (P == SP_SPIN) {
-                        if (f == 2) {
-                            rf = (double)cur.jj;
-                            ir = rcp_rn(rf);
-                        }
-                    }
-                    const float v = vel_x(cur.v[f], dsv[f], dt, rf, ir, b);
-                    vn[f] = jin ? v : 0.f;
-                }
-            } else {
-                const double dtr = dt * rcp64((double)cur.rho);
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    double dtm = dtr;
-                    if constexpr (P == SP_SPIN) {
-                        if (f == 2) dtm = dt * rcp64((double)cur.jj);
-                    }
-                    vn[f] = jin ? (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm)) : 0.f;
-                }
-            }
-            if (outl && r >= i0 && r < i1) {
-                const unsigned k = (unsigned)(r * nz + j);
-                float un[NV];
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    un[f] = upd_disp<EX>(cur.u[f], vn[f], dt);
-                    u_out[f][k] = un[f];
-                    v_out[f][k] = vn[f];
-                }
-                if (a.snap_idx >= 0) {
-                    gf sp = (gf)d.snap + (long long)a.snap_idx * n_snap(P) * NP;
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        sp[f * NP + k] = cur.u[f];
-                        sp[(NV + f) * NP + k] = vn[f];
-                    }
-                }
-                // save_obs (solver.py:151-156): displacement after the step
-                if (record) {
-                    for (int q2 = pr; q2 < d.nrec && d.rec_k[q2] < kend; ++q2) {
-                        if (d.rec_k[q2] == kr) {
-                            const int row = d.rec_row[q2];
-#pragma unroll
-                            for (int f = 0; f < NV; f++)
-                                d.rec[((long long)f * d.nrec + row) * a.nt + itx] = un[f];
-                        }
-                    }
-                }
-            }
-            if (record && r >= i0 && r < i1)
-                while (pr < d.nrec && d.rec_k[pr] < kend) ++pr;
-        }
-#pragma unroll
-        for (int f = 0; f < NV; f++) {
-            W0[f] = W1[f];
-            W1[f] = W2[f];
-            W2[f] = W3[f];
-            W3[f] = vn[f];
-        }
-
-        // ---- new stresses of row qr = m-3 (velocity window rows qr-1..qr+2)
-        const int qr = m - 3;
-        if (qr < i0) continue;
-        const bool xp_ok = qr >= 1 && qr < nx - 2;
-        const unsigned kq = (unsigned)(qr * nz + j);
-        if constexpr (P == SP_SH) {
-            // div_vy + add_sy (solver.py:112-117, 128-133)
-            float zm1 = lane_m1(W1[0]), zp1 = lane_p1(W1[0]), zp2 = lane_p1(zp1);
-            if (outl) {
-                const float dvdx = xp_ok ? (float)dpl<EX>(W0[0], W1[0], W2[0], W3[0], KX) : 0.f;
-                const float dvdz = zp_ok ? (float)dpl<EX>(zm1, W1[0], zp1, zp2, KZ) : 0.f;
-                const double dtmu = dt * (double)cur.mu;
-                s_out[0][kq] = upd_sadd<EX>(S0[0], dtmu, (double)dvdx);
-                s_out[1][kq] = upd_sadd<EX>(S0[1], dtmu, (double)dvdz);
-            }
-        } else {
-            // div_vxz + add_sxz (solver.py:119-126, 143-149)
-            float xm1 = lane_m1(W1[0]), xp1 = lane_p1(W1[0]), xp2 = lane_p1(xp1);
-            float zm1 = lane_m1(W1[1]), zp1 = lane_p1(W1[1]), zp2 = lane_p1(zp1);
-            float cm1 = 0.f, cp1 = 0.f, cp2 = 0.f;
-            if constexpr (P == SP_SPIN) {
-                cm1 = lane_m1(W1[2]);
-                cp1 = lane_p1(W1[2]);
-                cp2 = lane_p1(cp1);
-            }
-            if (outl) {
-                const float dvxdx = xp_ok ? (float)dpl<EX>(W0[0], W1[0], W2[0], W3[0], KX) : 0.f;
-                const float dvzdx = xp_ok ? (float)dpl<EX>(W0[1], W1[1], W2[1], W3[1], KX) : 0.f;
-                const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, W1[0], xp1, xp2, KZ) : 0.f;
-                const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, W1[1], zp1, zp2, KZ) : 0.f;
-                const float lam = cur.lam, mu = cur.mu;
-                const double l2m = (double)lam + 2.0 * (double)mu;
-                s_out[0][kq] = upd_snorm<EX>(S0[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
-                s_out[1][kq] = upd_snorm<EX>(S0[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
-                s_out[2][kq] = upd_sadd<EX>(S0[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
-                if constexpr (P == SP_SPIN) {
-                    // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
-                    const float dcdx = xp_ok ? (float)dpl<EX>(W0[2], W1[2], W2[2], W3[2], KX) : 0.f;
-                    const float dcdz = zp_ok ? (float)dpl<EX>(cm1, W1[2], cp1, cp2, KZ) : 0.f;
-                    const double dtg = dt * (double)cur.gam;
-                    s_out[4][kq] = upd_srot<EX>(S0[4], dt2, cur.nu, W1[2], sub_rn(dvzdx, dvxdz));
-                    s_out[3][kq] = upd_sadd<EX>(S0[3], dtg, (double)dcdx);
-                    s_out[5][kq] = upd_sadd<EX>(S0[5], dtg, (double)dcdz);
-                }
-            }
-        }
-    }
-}
-
-// ------------------------------------------------- step kernel, version 3
-// Same algorithm and rounding as k_step; the difference is memory-pipeline
-// behaviour.  On gfx9 loads and stores share one in-order vmcnt counter, so
-// the compiler can only wait for exactly the prefetched row (and not drain
-// every outstanding load and store) when each row iteration issues the same
-// sequence of vector memory operations.  Hence:
-//   * buffer loads/stores with one 32-bit byte offset per row, the field's
-//     slot offset in an SGPR and the base in a buffer resource;
-//   * loads always issue (rows and lanes clamped into the grid); stores
-//     always issue, with an offset past the end of the buffer for lanes or
-//     rows that must not be written -- the buffer range check drops them;
-//   * snapshots and receiver records are separate kernels (k_snap,
-//     k_record); source injection reads the time function with scalar
-//     loads only;
-//   * the row loop is unrolled by four with windows indexed by row & 3, so
-//     the register windows never move.
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-constexpr unsigned OOB = 0x40000000u;  // >= every per-shot buffer size (checked on the host)
-__device__ __forceinline__ rsrc_t make_rsrc(const void *p, long long bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, int soff)
-{
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
-}
-__device__ __forceinline__ void bst(rsrc_t r, unsigned voff, int soff, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, soff, 0);
-}
-
-template <int P>
-struct Row3 {
-    float s[n_s(P)];   // old stresses, row m
-    float v[n_v(P)];   // old velocities, row m-1
-    float u[n_v(P)];   // displacement, row m-1
-    float rho, jj, bx; // row m-1
-    float lam, mu, nu, gam;  // row m-3
-};
-
-template <int P>
-__device__ __forceinline__ void load3(Row3<P> &p, rsrc_t RS, rsrc_t RM, cflt_p bxp, int NP4, int rd, int mp, int nx,
-                                      int nz, int jc, int i0, int i1)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    // each field's row is clamped to the rows the chunk needs, so the extra
-    // prologue/epilogue iterations re-read cached rows instead of new ones:
-    // stresses i0-3..i1+2, velocities i0-1..i1+1, u and material i0..i1-1
-    const int ms = min(max(mp, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
-    const int mr = min(max(mp - 1, max(i0 - 1, 0)), min(i1 + 1, nx - 1));
-    const int mu_ = min(max(mp - 1, i0), i1 - 1);
-    const int mq = min(max(mp - 3, i0), i1 - 1);
-    const unsigned os = (unsigned)(ms * nz + jc) * 4u, orr = (unsigned)(mr * nz + jc) * 4u,
-                   ou = (unsigned)(mu_ * nz + jc) * 4u, oq = (unsigned)(mq * nz + jc) * 4u;
-#pragma unroll
-    for (int f = 0; f < NS; f++) p.s[f] = bld(RS, os, slot_s(P, rd, f) * NP4);
-#pragma unroll
-    for (int f = 0; f < NV; f++) {
-        p.v[f] = bld(RS, orr, slot_v(P, rd, f) * NP4);
-        p.u[f] = bld(RS, ou, slot_u(P, f) * NP4);
-    }
-    p.rho = bld(RM, orr, M_RHO * NP4);
-    p.bx = bxp[mr];
-    p.jj = 1.f;
-    if constexpr (P == SP_SPIN) p.jj = bld(RM, orr, M_J * NP4);
-    p.mu = bld(RM, oq, M_MU * NP4);
-    p.lam = p.nu = p.gam = 0.f;
-    if constexpr (P != SP_SH) p.lam = bld(RM, oq, M_LAM * NP4);
-    if constexpr (P == SP_SPIN) {
-        p.nu = bld(RM, oq, M_NU * NP4);
-        p.gam = bld(RM, oq, M_GAM * NP4);
-    }
-}
-
-template <int P, bool EX>
-__global__ __launch_bounds__(256) void k_step3(StepArgs a)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    const int nb = gridDim.x, bb = blockIdx.x;
-    const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
-    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
-    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
-    if (wave >= a.nitems) return;
-    const int L = threadIdx.x & 63;
-    const int4 item = a.items[2 * wave];
-    if (item.x < 0) return;
-    const int4 item2 = a.items[2 * wave + 1];
-    const ShotDesc &d = a.shots[item.x];
-    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
-    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
-    const int j = item.y - a.shift + L;
-    const int jc = min(max(j, 0), nz - 1);
-    const bool jin = j >= 0 && j < nz;
-    const bool outl = jin && L >= a.shift && L < a.shift + a.wout;
-    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
-    const int NP4 = (int)(d.Npad * 4);  // bytes per field
-    const double dt = a.dt, dt2 = 2.0 * a.dt;
-    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
-    const int rd = a.rd, wr = a.rd ^ 1;
-    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
-    const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
-    const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
-    const cflt_p bxp = (cflt_p)d.bx;
-
-    // injection list: sources (forward) or receivers with adjoint sources
-    const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
-    const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
-    const int ni = a.adjoint ? d.nrec : d.nsrc;
-    const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
-    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
-    int pi = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
-    while (pi < ni && ik[pi] < (i0 - 1) * nz) ++pi;  // the list pointer starts at row i0-4
-    int src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
-
-    float S[4][NS], W[4][NV];
-    Row3<P> PF[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-#pragma unroll
-        for (int f = 0; f < NS; f++) S[u][f] = 0.f;
-#pragma unroll
-        for (int f = 0; f < NV; f++) W[u][f] = 0.f;
-    }
-
-    // prologue (i0 is a multiple of 4): rows i0-3, i0-2, i0-1 fill stress
-    // slots 1, 2, 3; rows up to i0+1 are prefetched.
-    load3<P>(PF[1], RS, RM, bxp, NP4, rd, i0 - 3, nx, nz, jc, i0, i1);
-    load3<P>(PF[2], RS, RM, bxp, NP4, rd, i0 - 2, nx, nz, jc, i0, i1);
-    load3<P>(PF[3], RS, RM, bxp, NP4, rd, i0 - 1, nx, nz, jc, i0, i1);
-    load3<P>(PF[0], RS, RM, bxp, NP4, rd, i0, nx, nz, jc, i0, i1);
-#pragma unroll
-    for (int f = 0; f < NS; f++) {
-        S[1][f] = PF[1].s[f];
-        S[2][f] = PF[2].s[f];
-        S[3][f] = PF[3].s[f];
-    }
-    load3<P>(PF[1], RS, RM, bxp, NP4, rd, i0 + 1, nx, nz, jc, i0, i1);
-
-    const int mend = i1 + 2;
-    for (int mb = i0; mb <= mend; mb += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int m = mb + u;
-            if (m > mend) break;
-            const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
-            // ---- prefetch row m+2 into slot (u+2)&3 (always issued)
-            load3<P>(PF[B], RS, RM, bxp, NP4, rd, m + 2, nx, nz, jc, i0, i1);
-            const Row3<P> &cur = PF[D];
-#pragma unroll
-            for (int f = 0; f < NS; f++) S[D][f] = cur.s[f];
-            // stress window rows m-3, m-2, m-1, m = slots A, B, C, D
-            float(&Sa)[NS] = S[A];
-            float(&Sb)[NS] = S[B];
-            float(&Sc)[NS] = S[C];
-            float(&Sd)[NS] = S[D];
-
-            // ---- new velocities of row r = m-1
-            const int r = m - 1;
-            const bool rin = r >= 0 && r < nx;
-            const bool xm_ok = r >= 2 && r < nx - 2;
-            float dsv[NV];
-            if constexpr (P == SP_SH) {
-                float zm1 = lane_m1(Sc[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(Sc[1]);
-                double ddx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
-                double ddz = zm_ok ? dmn<EX>(zm2, zm1, Sc[1], zp1, KZ) : 0.0;
-                dsv[0] = (float)(ddx + ddz);
-            } else {
-                float xzm1 = lane_m1(Sc[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(Sc[2]);
-                float zzm1 = lane_m1(Sc[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(Sc[1]);
-                double dxx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
-                double dxxz = xm_ok ? dmn<EX>(Sa[2], Sb[2], Sc[2], Sd[2], KX) : 0.0;
-                double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, Sc[2], xzp1, KZ) : 0.0;
-                double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, Sc[1], zzp1, KZ) : 0.0;
-                dsv[0] = (float)(dxx + dzxz);
-                dsv[1] = (float)(dxxz + dzzz);
-                if constexpr (P == SP_SPIN) {
-                    float yzm1 = lane_m1(Sc[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(Sc[5]);
-                    float yym1 = lane_m1(Sc[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(Sc[4]);
-                    double dxyx = xm_ok ? dmn<EX>(Sa[3], Sb[3], Sc[3], Sd[3], KX) : 0.0;
-                    double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, Sc[5], yzp1, KZ) : 0.0;
-                    float dsy = (float)(dxyx + dzyz);
-                    double dzyy = zm_ok ? dmn<EX>(yym2, yym1, Sc[4], yyp1, KZ) : 0.0;
-                    double dxyy = xm_ok ? dmn<EX>(Sa[4], Sb[4], Sc[4], Sd[4], KX) : 0.0;
-                    dsv[0] = (float)((double)dsv[0] + dzyy);
-                    dsv[1] = (float)((double)dsv[1] - dxyy);
-                    dsv[2] = (float)((double)dsy + 2.0 * (double)Sc[4]);
-                }
-            }
-            if (r == src_row) {
-                // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list
-                // order; every read here is wave-uniform (scalar loads)
-                const int kend = (r + 1) * nz, kr = r * nz + j;
-                while (pi < ni && ik[pi] < kend) {
-                    const int kk = ik[pi];
-                    const int row = irow[pi];
-                    const float s0 = ival[(long long)row * a.nt + itx];
-                    float s1 = 0.f;
-                    if constexpr (P != SP_SH) s1 = ival[(nrows + row) * a.nt + itx];
-                    if (kk == kr) {
-                        dsv[0] = add_rn(dsv[0], s0);
-                        if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
-                    }
-                    ++pi;
-                }
-                src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
-            }
-            const float b1 = (float)((double)cur.bx * bzb);
-            const double b = (double)(float)((double)b1 * bzt);
-            float vn[NV], un[NV];
-            if constexpr (EX) {
-                const double rr = (double)cur.rho, irr = rcp_rn(rr);
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    double rf = rr, ir = irr;
-                    if constexpr (P == SP_SPIN) {
-                        if (f == 2) {
-                            rf = (double)cur.jj;
-                            ir = rcp_rn(rf);
-                        }
-                    }
-                    const float v = vel_x(cur.v[f], dsv[f], dt, rf, ir, b);
-                    vn[f] = (rin && jin) ? v : 0.f;
-                    un[f] = disp_x(cur.u[f], vn[f], dt);
-                }
-            } else {
-                const double dtr = dt * rcp64((double)cur.rho);
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    double dtm = dtr;
-                    if constexpr (P == SP_SPIN) {
-                        if (f == 2) dtm = dt * rcp64((double)cur.jj);
-                    }
-                    const float v = (float)(b * ((double)cur.v[f] + (double)dsv[f] * dtm));
-                    vn[f] = (rin && jin) ? v : 0.f;
-                    un[f] = (float)((double)cur.u[f] + (double)vn[f] * dt);
-                }
-            }
-            {
-                const unsigned ov = (outl && r >= i0 && r < i1) ? (unsigned)(r * nz + j) * 4u : OOB;
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    bst(RS, ov, slot_u(P, f) * NP4, un[f]);
-                    bst(RS, ov, slot_v(P, wr, f) * NP4, vn[f]);
-                }
-            }
-#pragma unroll
-            for (int f = 0; f < NV; f++) W[C][f] = vn[f];
-
-            // ---- new stresses of row q = m-3 (velocity rows q-1..q+2 = slots D, A, B, C)
-            const int qr = m - 3;
-            float(&Wa)[NV] = W[D];
-            float(&Wb)[NV] = W[A];
-            float(&Wc)[NV] = W[B];
-            float(&Wd)[NV] = W[C];
-            const bool xp_ok = qr >= 1 && qr < nx - 2;
-            const unsigned oq = (outl && qr >= i0 && qr < i1) ? (unsigned)(qr * nz + j) * 4u : OOB;
-            if constexpr (P == SP_SH) {
-                float zm1 = lane_m1(Wb[0]), zp1 = lane_p1(Wb[0]), zp2 = lane_p1(zp1);
-                const float dvdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
-                const float dvdz = zp_ok ? (float)dpl<EX>(zm1, Wb[0], zp1, zp2, KZ) : 0.f;
-                const double dtmu = dt * (double)cur.mu;
-                bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_sadd<EX>(Sa[0], dtmu, (double)dvdx));
-                bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_sadd<EX>(Sa[1], dtmu, (double)dvdz));
-            } else {
-                float xm1 = lane_m1(Wb[0]), xp1 = lane_p1(Wb[0]), xp2 = lane_p1(xp1);
-                float zm1 = lane_m1(Wb[1]), zp1 = lane_p1(Wb[1]), zp2 = lane_p1(zp1);
-                const float dvxdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
-                const float dvzdx = xp_ok ? (float)dpl<EX>(Wa[1], Wb[1], Wc[1], Wd[1], KX) : 0.f;
-                const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, Wb[0], xp1, xp2, KZ) : 0.f;
-                const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, Wb[1], zp1, zp2, KZ) : 0.f;
-                const float lam = cur.lam, mu = cur.mu;
-                const double l2m = (double)lam + 2.0 * (double)mu;
-                bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_snorm<EX>(Sa[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz)));
-                bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_snorm<EX>(Sa[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx)));
-                bst(RS, oq, slot_s(P, wr, 2) * NP4, upd_sadd<EX>(Sa[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx))));
-                if constexpr (P == SP_SPIN) {
-                    float cm1 = lane_m1(Wb[2]), cp1 = lane_p1(Wb[2]), cp2 = lane_p1(cp1);
-                    const float dcdx = xp_ok ? (float)dpl<EX>(Wa[2], Wb[2], Wc[2], Wd[2], KX) : 0.f;
-                    const float dcdz = zp_ok ? (float)dpl<EX>(cm1, Wb[2], cp1, cp2, KZ) : 0.f;
-                    const double dtg = dt * (double)cur.gam;
-                    bst(RS, oq, slot_s(P, wr, 4) * NP4, upd_srot<EX>(Sa[4], dt2, cur.nu, Wb[2], sub_rn(dvzdx, dvxdz)));
-                    bst(RS, oq, slot_s(P, wr, 3) * NP4, upd_sadd<EX>(Sa[3], dtg, (double)dcdx));
-                    bst(RS, oq, slot_s(P, wr, 5) * NP4, upd_sadd<EX>(Sa[5], dtg, (double)dcdz));
-                }
-            }
-        }
-    }
-}
-
-// ------------------------------------------------- two steps per launch
-// k_step2 advances a strip by two time steps (it and it+1) in one row march,
-// keeping the intermediate state in registers: the state is read once and
-// written once per two steps.  The intermediate stresses and velocities of a
-// chunk's halo rows are recomputed by each chunk that needs them; the strip
-// keeps 6 lanes of halo on each side (lanes [6, 58) are output), and the
-// chunk reads 6 rows before and 5 after its own.  Every value is computed
-// with the same operations, in the same order, as two launches of k_step3,
-// so results are identical bit for bit.  The result goes to the other
-// stress/velocity set (rd ^ 1), like one step, so no wavefront overwrites
-// rows another one still reads.
-//
-// Iteration m of the march (rows relative to m):
-//   A  v(n+1)  at row m-1 from s(n) rows m-3..m   (source at step it)
-//   B  s(n+1)  at row m-3 from v(n+1) rows m-4..m-1
-//   C  v(n+2)  at row m-4 from s(n+1) rows m-6..m-3 (source at step it+1);
-//      u(n+1), u(n+2) and the receiver records of both steps at row m-4
-//   D  s(n+2)  at row m-6 from v(n+2) rows m-7..m-4
-// Windows of four rows are indexed by row & 3 (loop unrolled by four).
-
-template <int P>
-struct Row2 {
-    float s[n_s(P)];     // s(n), row m
-    float v[n_v(P)];     // v(n), row m-1
-    float u[n_v(P)];     // u(n), row m-4
-    float rhoA, jA;      // row m-1 (j: micro-inertia, spin only)
-    float lamB, muB;     // row m-3
-    float nuB, gamB;     // row m-3 (spin only: nu, mu_c + nu_c)
-    // rows m-4 and m-6 of the same materials are the values loaded three
-    // iterations earlier; the march keeps them in register windows
-};
-
-// The rows are wave-uniform, so each load's row offset goes into the scalar
-// offset and the per-lane offset vj = 4 jc is the same for every load (no
-// vector address arithmetic per row).  Every row is clamped into the grid,
-// so every load stays inside its buffer.
-template <int P>
-__device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p bxp, int NP4, int rd, int mp, int nx,
-                                      int nz, unsigned vj, int i0, int i1)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    // rows clamped to what the chunk needs (and to the grid)
-    const int ms = min(max(mp, max(i0 - 6, 0)), min(i1 + 5, nx - 1));
-    const int ma = min(max(mp - 1, max(i0 - 4, 0)), min(i1 + 4, nx - 1));
-    const int mb = min(max(mp - 3, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
-    const int mu_ = min(max(mp - 4, i0), i1 - 1);
-    const int rb = nz * 4;
-    const int os = ms * rb, oa = ma * rb, ob = mb * rb, ou = mu_ * rb;
-#pragma unroll
-    for (int f = 0; f < NS; f++) p.s[f] = bld(RS, vj, slot_s(P, rd, f) * NP4 + os);
-#pragma unroll
-    for (int f = 0; f < NV; f++) {
-        p.v[f] = bld(RS, vj, slot_v(P, rd, f) * NP4 + oa);
-        p.u[f] = bld(RS, vj, slot_u(P, f) * NP4 + ou);
-    }
-    p.rhoA = bld(RM, vj, M_RHO * NP4 + oa);
-    p.muB = bld(RM, vj, M_MU * NP4 + ob);
-    p.lamB = 0.f;
-    if constexpr (P != SP_SH) p.lamB = bld(RM, vj, M_LAM * NP4 + ob);
-    p.jA = 1.f;
-    p.nuB = p.gamB = 0.f;
-    if constexpr (P == SP_SPIN) {
-        p.jA = bld(RM, vj, M_J * NP4 + oa);
-        p.nuB = bld(RM, vj, M_NU * NP4 + ob);
-        p.gamB = bld(RM, vj, M_GAM * NP4 + ob);
-    }
-}
-
-// stress divergence of one velocity row (div_sy / div_sxz, solver.py:54-66)
-// from the stress window rows r-2..r+1 (sa..sd)
-template <int P, bool EX>
-__device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[n_s(P)], const float (&sb)[n_s(P)],
-                                        const float (&sc)[n_s(P)], const float (&sd)[n_s(P)], bool xm_ok, bool zm_ok,
-                                        int L, const DCo &KX, const DCo &KZ)
-{
-    if constexpr (P == SP_SH) {
-        float zm1 = lane_m1(sc[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(sc[1]);
-        double ddx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
-        double ddz = zm_ok ? dmn<EX>(zm2, zm1, sc[1], zp1, KZ) : 0.0;
-        dsv[0] = (float)(ddx + ddz);
-    } else {
-        float xzm1 = lane_m1(sc[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(sc[2]);
-        float zzm1 = lane_m1(sc[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(sc[1]);
-        double dxx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
-        double dxxz = xm_ok ? dmn<EX>(sa[2], sb[2], sc[2], sd[2], KX) : 0.0;
-        double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, sc[2], xzp1, KZ) : 0.0;
-        double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, sc[1], zzp1, KZ) : 0.0;
-        dsv[0] = (float)(dxx + dzxz);
-        dsv[1] = (float)(dxxz + dzzz);
-        if constexpr (P == SP_SPIN) {
-            // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
-            float yzm1 = lane_m1(sc[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(sc[5]);
-            float yym1 = lane_m1(sc[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(sc[4]);
-            double dxyx = xm_ok ? dmn<EX>(sa[3], sb[3], sc[3], sd[3], KX) : 0.0;
-            double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, sc[5], yzp1, KZ) : 0.0;
-            float dsy = (float)(dxyx + dzyz);
-            double dzyy = zm_ok ? dmn<EX>(yym2, yym1, sc[4], yyp1, KZ) : 0.0;
-            double dxyy = xm_ok ? dmn<EX>(sa[4], sb[4], sc[4], sd[4], KX) : 0.0;
-            dsv[0] = (float)((double)dsv[0] + dzyy);
-            dsv[1] = (float)((double)dsv[1] - dxyy);
-            dsv[2] = (float)((double)dsy + 2.0 * (double)sc[4]);
-        }
-    }
-}
-
-// velocity update (add_vy / add_vxz, solver.py:96-117) from the divergence
-// plus sources, the old v, 1/rho and the sponge factors
-// `zint`: every lane of the strip lies outside the top/bottom sponges; with
-// bx = 1 the sponge factor is exactly 1 and its arithmetic is skipped
-// (the same float64 result, bit for bit).
-template <int P, bool EX>
-__device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&dsv)[n_v(P)], const float (&vo)[n_v(P)],
-                                           float rho, float jj, float bxr, double bzb, double bzt, bool zint, bool rin,
-                                           bool jin, double dt)
-{
-    if constexpr (EX) {
-        double b = 1.0;
-        if (!(zint && bxr == 1.f)) {
-            const float b1 = (float)((double)bxr * bzb);
-            b = (double)(float)((double)b1 * bzt);
-        }
-        const double rr = (double)rho, irr = rcp_rn(rr);
-#pragma unroll
-        for (int f = 0; f < n_v(P); f++) {
-            double rf = rr, ir = irr;
-            if constexpr (P == SP_SPIN) {
-                if (f == 2) {  // vy_c divides by the micro-inertia j (solver.py:226)
-                    rf = (double)jj;
-                    ir = rcp_rn(rf);
-                }
-            }
-            const float v = vel_x(vo[f], dsv[f], dt, rf, ir, b);
-            vn[f] = (rin && jin) ? v : 0.f;
-        }
-        return;
-    }
-    const double dtr = dt * rcp64((double)rho);
-    double dtj = dtr;
-    if constexpr (P == SP_SPIN) dtj = dt * rcp64((double)jj);
-    if (zint && bxr == 1.f) {
-#pragma unroll
-        for (int f = 0; f < n_v(P); f++) {
-            const float v = (float)((double)vo[f] + (double)dsv[f] * (f == 2 ? dtj : dtr));
-            vn[f] = (rin && jin) ? v : 0.f;
-        }
-        return;
-    }
-    const float b1 = (float)((double)bxr * bzb);
-    const double b = (double)(float)((double)b1 * bzt);
-#pragma unroll
-    for (int f = 0; f < n_v(P); f++) {
-        const float v = (float)(b * ((double)vo[f] + (double)dsv[f] * (f == 2 ? dtj : dtr)));
-        vn[f] = (rin && jin) ? v : 0.f;
-    }
-}
-
-// stress update of one row (add_sy / add_sxz, solver.py:120-148) from the
-// velocity window rows q-1..q+2 (wa..wd) and the old stresses
-template <int P, bool EX>
-__device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n_v(P)], const float (&wb)[n_v(P)],
-                                        const float (&wc)[n_v(P)], const float (&wd)[n_v(P)], const float (&so)[n_s(P)],
-                                        float lam, float mu, float nu, float gam, bool xp_ok, bool zp_ok, int L,
-                                        const DCo &KX, const DCo &KZ, double dt)
-{
-    if constexpr (P == SP_SH) {
-        float zm1 = lane_m1(wb[0]), zp1 = lane_p1(wb[0]), zp2 = lane_p1(zp1);
-        const float dvdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
-        const float dvdz = zp_ok ? (float)dpl<EX>(zm1, wb[0], zp1, zp2, KZ) : 0.f;
-        const double dtmu = dt * (double)mu;
-        sn[0] = upd_sadd<EX>(so[0], dtmu, (double)dvdx);
-        sn[1] = upd_sadd<EX>(so[1], dtmu, (double)dvdz);
-    } else {
-        float xm1 = lane_m1(wb[0]), xp1 = lane_p1(wb[0]), xp2 = lane_p1(xp1);
-        float zm1 = lane_m1(wb[1]), zp1 = lane_p1(wb[1]), zp2 = lane_p1(zp1);
-        const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
-        const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1], wb[1], wc[1], wd[1], KX) : 0.f;
-        const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, wb[0], xp1, xp2, KZ) : 0.f;
-        const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, wb[1], zp1, zp2, KZ) : 0.f;
-        const double l2m = (double)lam + 2.0 * (double)mu;
-        sn[0] = upd_snorm<EX>(so[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
-        sn[1] = upd_snorm<EX>(so[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
-        sn[2] = upd_sadd<EX>(so[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
-        if constexpr (P == SP_SPIN) {
-            // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
-            float cm1 = lane_m1(wb[2]), cp1 = lane_p1(wb[2]), cp2 = lane_p1(cp1);
-            const float dcdx = xp_ok ? (float)dpl<EX>(wa[2], wb[2], wc[2], wd[2], KX) : 0.f;
-            const float dcdz = zp_ok ? (float)dpl<EX>(cm1, wb[2], cp1, cp2, KZ) : 0.f;
-            const double dtg = dt * (double)gam;
-            sn[4] = upd_srot<EX>(so[4], 2.0 * dt, nu, wb[2], sub_rn(dvzdx, dvxdz));
-            sn[3] = upd_sadd<EX>(so[3], dtg, (double)dcdx);
-            sn[5] = upd_sadd<EX>(so[5], dtg, (double)dcdz);
-        }
-    }
-}
-
-// sources (forward) or adjoint sources (adjoint) of velocity row r at step
-// `it`: float32 adds in list order (stf_dsy / stf_dsxz, solver.py:78-93)
-template <int P>
-__device__ __forceinline__ void inject(float (&dsv)[n_v(P)], int r, int j, int nz, int &pi, int &src_row, int ni,
-                                       cint_p ik, cint_p irow, cflt_p ival, long long nrows, int nt, int it)
-{
-    if (r != src_row) return;
-    const int kend = (r + 1) * nz, kr = r * nz + j;
-    while (pi < ni && ik[pi] < kend) {
-        const int kk = ik[pi];
-        const int row = irow[pi];
-        const float s0 = ival[(long long)row * nt + it];
-        float s1 = 0.f;
-        if constexpr (P != SP_SH) s1 = ival[(nrows + row) * nt + it];
-        if (kk == kr) {
-            dsv[0] = add_rn(dsv[0], s0);
-            if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
-        }
-        ++pi;
-    }
-    src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
-}
-
-template <int P, bool EX>
-__global__ __launch_bounds__(256) void k_step2(StepArgs a)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    const int nb = gridDim.x, bb = blockIdx.x;
-    const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
-    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
-    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
-    if (wave >= a.nitems) return;
-    const int L = threadIdx.x & 63;
-    const int4 item = a.items[2 * wave];
-    if (item.x < 0) return;
-    const int4 item2 = a.items[2 * wave + 1];
-    const ShotDesc &d = a.shots[item.x];
-    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
-    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
-    const int j = item.y - a.shift + L;
-    const int jc = min(max(j, 0), nz - 1);
-    const bool jin = j >= 0 && j < nz;
-    const bool outl = jin && L >= a.shift && L < a.shift + a.wout;
-    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
-    const int NP4 = (int)(d.Npad * 4);
-    const double dt = a.dt;
-    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
-    const int rd = a.rd, wr = a.rd ^ 1;
-    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
-    const bool zint = __all(bzb == 1.0 && bzt == 1.0);
-    const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
-    const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
-    const cflt_p bxp = (cflt_p)d.bx;
-    const unsigned vj = (unsigned)jc * 4u;            // per-lane load offset
-    const unsigned vo = outl ? (unsigned)j * 4u : OOB;  // per-lane store offset (OOB: not written)
-
-    const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
-    const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
-    const int ni = a.adjoint ? d.nrec : d.nsrc;
-    const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
-    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
-    // injection pointers: step it from row i0-4, step it+1 from row i0-1
-    int p1 = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
-    int p2 = p1;
-    while (p2 < ni && ik[p2] < (i0 - 1) * nz) ++p2;
-    int row1 = p1 < ni ? ik[p1] / nz : 0x7fffffff;
-    int row2 = p2 < ni ? ik[p2] / nz : 0x7fffffff;
-    // receivers recorded by this chunk (forward only): rows i0..i1-1
-    const cint_p rk = (cint_p)d.rec_k;
-    const cint_p rrow = (cint_p)d.rec_row;
-    int pr = __builtin_amdgcn_readfirstlane(item2.z);
-    int rec_row = (!a.adjoint && pr < d.nrec) ? rk[pr] / nz : 0x7fffffff;
-    const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
-
-    float S0[4][NS], W1[4][NV], S1[4][NS], W2[4][NV];
-    // materials of rows m-1 (rho, j) and m-3 (lam, mu, nu, gam) by
-    // iteration: three iterations later they are rows m-4 and m-6
-    float RW[4], JW[4], LW[4], MW[4], NW[4], GW[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        RW[u] = JW[u] = 1.f;
-        LW[u] = MW[u] = NW[u] = GW[u] = 0.f;
-    }
-    Row2<P> PF[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-#pragma unroll
-        for (int f = 0; f < NS; f++) S0[u][f] = S1[u][f] = 0.f;
-#pragma unroll
-        for (int f = 0; f < NV; f++) W1[u][f] = W2[u][f] = 0.f;
-    }
-
-    // the march starts at m0 = i0 - 3: s(n) rows m0-3..m0-1 fill window slots 1..3
-    const int m0 = i0 - 3, mend = i1 + 5;
-    load2<P>(PF[1], RS, RM, bxp, NP4, rd, m0 - 3, nx, nz, vj, i0, i1);
-    load2<P>(PF[2], RS, RM, bxp, NP4, rd, m0 - 2, nx, nz, vj, i0, i1);
-    load2<P>(PF[3], RS, RM, bxp, NP4, rd, m0 - 1, nx, nz, vj, i0, i1);
-    load2<P>(PF[0], RS, RM, bxp, NP4, rd, m0, nx, nz, vj, i0, i1);
-#pragma unroll
-    for (int f = 0; f < NS; f++) {
-        S0[1][f] = PF[1].s[f];
-        S0[2][f] = PF[2].s[f];
-        S0[3][f] = PF[3].s[f];
-    }
-    load2<P>(PF[1], RS, RM, bxp, NP4, rd, m0 + 1, nx, nz, vj, i0, i1);
-
-    for (int mb = m0; mb <= mend; mb += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int m = mb + u;
-            if (m > mend) break;
-            const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
-            load2<P>(PF[B], RS, RM, bxp, NP4, rd, m + 2, nx, nz, vj, i0, i1);
-            const Row2<P> &cur = PF[D];
-#pragma unroll
-            for (int f = 0; f < NS; f++) S0[D][f] = cur.s[f];
-
-            // ---- A: v(n+1) at row m-1; s(n) rows m-3..m = slots A, B, C, D
-            {
-                const int r = m - 1;
-                const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
-                float dsv[NV], vn[NV];
-                vel_row<P, EX>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, KX, KZ);
-                inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
-                const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, cur.v, cur.rhoA, cur.jA, bxr, bzb, bzt, zint, rin, jin, dt);
-                RW[D] = cur.rhoA;
-                JW[D] = cur.jA;
-#pragma unroll
-                for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
-            }
-            // ---- B: s(n+1) at row m-3; v(n+1) rows m-4..m-1 = slots D, A, B, C
-            {
-                const int q = m - 3;
-                const bool xp_ok = q >= 1 && q < nx - 2;
-                float sn[NS];
-                str_row<P, EX>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB, xp_ok, zp_ok, L, KX, KZ, dt);
-                LW[D] = cur.lamB;
-                MW[D] = cur.muB;
-                NW[D] = cur.nuB;
-                GW[D] = cur.gamB;
-#pragma unroll
-                for (int f = 0; f < NS; f++) S1[A][f] = sn[f];
-            }
-            // ---- C: v(n+2) at row m-4; s(n+1) rows m-6..m-3 = slots B, C, D, A
-            {
-                const int r = m - 4;
-                const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
-                float dsv[NV], vn[NV];
-                vel_row<P, EX>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, KX, KZ);
-                inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
-                const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, W1[D], RW[A], JW[A], bxr, bzb, bzt, zint, rin, jin, dt);
-                float u1[NV], u2[NV];
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    u1[f] = upd_disp<EX>(cur.u[f], W1[D][f], dt);
-                    u2[f] = upd_disp<EX>(u1[f], vn[f], dt);
-                    W2[D][f] = vn[f];
-                }
-                // rows outside the chunk: every lane's offset is out of range
-                const bool rok = r >= i0 && r < i1;
-                const unsigned ov = rok ? vo : OOB;
-                const int orow = rok ? r * nz * 4 : 0;
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    bst(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
-                    bst(RS, ov, slot_v(P, wr, f) * NP4 + orow, vn[f]);
-                }
-                // adjoint snapshots of either step (solver.py:193-203, 236-242):
-                // [isa][u before the step | v after it][NV]
-                if (a.snap_v >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        bst(RP, ov, f * NP4 + orow, cur.u[f]);
-                        bst(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
-                    }
-                }
-                if (a.snap_u >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        bst(RP, ov, f * NP4 + orow, u1[f]);
-                        bst(RP, ov, (NV + f) * NP4 + orow, vn[f]);
-                    }
-                }
-                // save_obs of both steps (solver.py:151-156) for receivers in
-                // this row; rows past the chunk belong to the next chunk
-                if (r == rec_row && r < i1) {
-                    const int kend = (r + 1) * nz, kr = r * nz + j;
-                    while (pr < d.nrec && rk[pr] < kend) {
-                        const int row = rrow[pr];
-                        const unsigned orc = (outl && rk[pr] == kr) ? (unsigned)(row * a.nt + itx) * 4u : OOB;
-                        const int cs = d.nrec * a.nt * 4;
-#pragma unroll
-                        for (int f = 0; f < NV; f++) {
-                            bst(RR, orc, f * cs, u1[f]);
-                            bst(RR, orc, f * cs + 4, u2[f]);
-                        }
-                        ++pr;
-                    }
-                    rec_row = pr < d.nrec ? rk[pr] / nz : 0x7fffffff;
-                }
-            }
-            // ---- D: s(n+2) at row m-6; v(n+2) rows m-7..m-4 = slots A, B, C, D
-            {
-                const int q = m - 6;
-                const bool xp_ok = q >= 1 && q < nx - 2;
-                float sn[NS];
-                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok, L, KX, KZ, dt);
-                const bool qok = q >= i0 && q < i1;
-                const unsigned oq = qok ? vo : OOB;
-                const int orow = qok ? q * nz * 4 : 0;
-#pragma unroll
-                for (int f = 0; f < NS; f++) bst(RS, oq, slot_s(P, wr, f) * NP4 + orow, sn[f]);
-            }
-        }
-    }
-}
-
-// ------------------------------------- two steps per launch, two cells per lane
-// k_step2w is k_step2 with every lane holding two consecutive z cells
-// (j0 + 2L and j0 + 2L + 1): a wavefront covers 128 cells, 116 of them
-// output, and every load and store moves both cells at once (8 bytes per
-// lane), so a row issues half the memory instructions per cell.  The
-// arithmetic per cell is the same as in k_step2, operation for operation.
-// Needs even nz (a lane's two cells are then both inside or both outside
-// the grid and every access is 8-byte aligned); SH and P-SV.
-template <int P>
-struct Row2W {
-    float s[n_s(P)][2];  // s(n), row m
-    float v[n_v(P)][2];  // v(n), row m-1
-    float u[n_v(P)][2];  // u(n), row m-4
-    float rhoA[2];       // row m-1
-    float lamB[2], muB[2];  // row m-3
-};
-
-typedef unsigned v2u_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void bld2(float (&x)[2], rsrc_t r, unsigned voff, int soff)
-{
-    const v2u_t w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0);
-    x[0] = __uint_as_float(w.x);
-    x[1] = __uint_as_float(w.y);
-}
-__device__ __forceinline__ void bst2(rsrc_t r, unsigned voff, int soff, const float (&x)[2])
-{
-    v2u_t w;
-    w.x = __float_as_uint(x[0]);
-    w.y = __float_as_uint(x[1]);
-    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)voff, soff, 0);
-}
-
-template <int P>
-__device__ __forceinline__ void load2w(Row2W<P> &p, rsrc_t RS, rsrc_t RM, int NP4, int rd, int mp, int nx, int nz,
-                                       unsigned vj, int i0, int i1)
-{
-    constexpr int NS = n_s(P), NV = n_v(P);
-    // the same row clamps as load2
-    const int ms = min(max(mp, max(i0 - 6, 0)), min(i1 + 5, nx - 1));
-    const int ma = min(max(mp - 1, max(i0 - 4, 0)), min(i1 + 4, nx - 1));
-    const int mb = min(max(mp - 3, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
-    const int mu_ = min(max(mp - 4, i0), i1 - 1);
-    const int rb = nz * 4;
-    const int os = ms * rb, oa = ma * rb, ob = mb * rb, ou = mu_ * rb;
-#pragma unroll
-    for (int f = 0; f < NS; f++) bld2(p.s[f], RS, vj, slot_s(P, rd, f) * NP4 + os);
-#pragma unroll
-    for (int f = 0; f < NV; f++) {
-        bld2(p.v[f], RS, vj, slot_v(P, rd, f) * NP4 + oa);
-        bld2(p.u[f], RS, vj, slot_u(P, f) * NP4 + ou);
-    }
-    bld2(p.rhoA, RM, vj, M_RHO * NP4 + oa);
-    bld2(p.muB, RM, vj, M_MU * NP4 + ob);
-    p.lamB[0] = p.lamB[1] = 0.f;
-    if constexpr (P != SP_SH) bld2(p.lamB, RM, vj, M_LAM * NP4 + ob);
-}
-
-// z neighbours of a two-cell field x: cell 0 sits at z = j0 + 2L, cell 1 at
-// z + 1; the previous lane holds z - 2, z - 1 and the next z + 2, z + 3
-__device__ __forceinline__ void zminus_w(float (&m2)[2], float (&m1)[2], float (&p1)[2], const float (&x)[2])
-{
-    const float a_m = lane_m1(x[0]), b_m = lane_m1(x[1]), a_p = lane_p1(x[0]);
-    m2[0] = a_m;  m1[0] = b_m;  p1[0] = x[1];
-    m2[1] = b_m;  m1[1] = x[0]; p1[1] = a_p;
-}
-__device__ __forceinline__ void zplus_w(float (&m1)[2], float (&p1)[2], float (&p2)[2], const float (&x)[2])
-{
-    const float b_m = lane_m1(x[1]), a_p = lane_p1(x[0]), b_p = lane_p1(x[1]);
-    m1[0] = b_m;  p1[0] = x[1]; p2[0] = a_p;
-    m1[1] = x[0]; p1[1] = a_p;  p2[1] = b_p;
-}
-
-// vel_row for two cells per lane (div_sy / div_sxz, solver.py:54-66)
-template <int P, bool EX>
-__device__ __forceinline__ void vel_row_w(float (&dsv)[n_v(P)][2], const float (&sa)[n_s(P)][2],
-                                          const float (&sb)[n_s(P)][2], const float (&sc)[n_s(P)][2],
-                                          const float (&sd)[n_s(P)][2], bool xm_ok, const bool (&zm_ok)[2],
-                                          const DCo &KX, const DCo &KZ)
-{
-    if constexpr (P == SP_SH) {
-        float zm2[2], zm1[2], zp1[2];
-        zminus_w(zm2, zm1, zp1, sc[1]);
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            double ddx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
-            double ddz = zm_ok[c] ? dmn<EX>(zm2[c], zm1[c], sc[1][c], zp1[c], KZ) : 0.0;
-            dsv[0][c] = (float)(ddx + ddz);
-        }
-    } else {
-        float xzm2[2], xzm1[2], xzp1[2], zzm2[2], zzm1[2], zzp1[2];
-        zminus_w(xzm2, xzm1, xzp1, sc[2]);
-        zminus_w(zzm2, zzm1, zzp1, sc[1]);
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            double dxx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
-            double dxxz = xm_ok ? dmn<EX>(sa[2][c], sb[2][c], sc[2][c], sd[2][c], KX) : 0.0;
-            double dzxz = zm_ok[c] ? dmn<EX>(xzm2[c], xzm1[c], sc[2][c], xzp1[c], KZ) : 0.0;
-            double dzzz = zm_ok[c] ? dmn<EX>(zzm2[c], zzm1[c], sc[1][c], zzp1[c], KZ) : 0.0;
-            dsv[0][c] = (float)(dxx + dzxz);
-            dsv[1][c] = (float)(dxxz + dzzz);
-        }
-    }
-}
-
-// str_row for two cells per lane (add_sy / add_sxz, solver.py:120-148)
-template <int P, bool EX>
-__device__ __forceinline__ void str_row_w(float (&sn)[n_s(P)][2], const float (&wa)[n_v(P)][2],
-                                          const float (&wb)[n_v(P)][2], const float (&wc)[n_v(P)][2],
-                                          const float (&wd)[n_v(P)][2], const float (&so)[n_s(P)][2],
-                                          const float (&lam)[2], const float (&mu)[2], bool xp_ok,
-                                          const bool (&zp_ok)[2], const DCo &KX, const DCo &KZ, double dt)
-{
-    if constexpr (P == SP_SH) {
-        float zm1[2], zp1[2], zp2[2];
-        zplus_w(zm1, zp1, zp2, wb[0]);
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const float dvdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
-            const float dvdz = zp_ok[c] ? (float)dpl<EX>(zm1[c], wb[0][c], zp1[c], zp2[c], KZ) : 0.f;
-            const double dtmu = dt * (double)mu[c];
-            sn[0][c] = upd_sadd<EX>(so[0][c], dtmu, (double)dvdx);
-            sn[1][c] = upd_sadd<EX>(so[1][c], dtmu, (double)dvdz);
-        }
-    } else {
-        float xm1[2], xp1[2], xp2[2], zm1[2], zp1[2], zp2[2];
-        zplus_w(xm1, xp1, xp2, wb[0]);
-        zplus_w(zm1, zp1, zp2, wb[1]);
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
-            const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1][c], wb[1][c], wc[1][c], wd[1][c], KX) : 0.f;
-            const float dvxdz = zp_ok[c] ? (float)dpl<EX>(xm1[c], wb[0][c], xp1[c], xp2[c], KZ) : 0.f;
-            const float dvzdz = zp_ok[c] ? (float)dpl<EX>(zm1[c], wb[1][c], zp1[c], zp2[c], KZ) : 0.f;
-            const double l2m = (double)lam[c] + 2.0 * (double)mu[c];
-            sn[0][c] = upd_snorm<EX>(so[0][c], dt, l2m, dvxdx, mul_rn(lam[c], dvzdz));
-            sn[1][c] = upd_snorm<EX>(so[1][c], dt, l2m, dvzdz, mul_rn(lam[c], dvxdx));
-            sn[2][c] = upd_sadd<EX>(so[2][c], dt, (double)mul_rn(mu[c], add_rn(dvxdz, dvzdx)));
-        }
-    }
-}
-
-// sources (forward) or adjoint sources (adjoint) of velocity row r, two cells
-// per lane: float32 adds in list order (stf_dsy / stf_dsxz, solver.py:78-93)
-template <int P>
-__device__ __forceinline__ void inject_w(float (&dsv)[n_v(P)][2], int r, int jl, int nz, int &pi, int &src_row,
-                                         int ni, cint_p ik, cint_p irow, cflt_p ival, long long nrows, int nt, int it)
-{
-    if (r != src_row) return;
-    const int kend = (r + 1) * nz, kl = r * nz + jl;  // kl: this lane's cell 0
-    while (pi < ni && ik[pi] < kend) {
-        const int kk = ik[pi];
-        const int row = irow[pi];
-        const float s0 = ival[(long long)row * nt + it];
-        float s1 = 0.f;
-        if constexpr (P != SP_SH) s1 = ival[(nrows + row) * nt + it];
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            if (kk == kl + c) {
-                dsv[0][c] = add_rn(dsv[0][c], s0);
-                if constexpr (P != SP_SH) dsv[1][c] = add_rn(dsv[1][c], s1);
-            }
-        }
-        ++pi;
-    }
-    src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
-}
-
-template <int P, bool EX>
-__global__ __launch_bounds__(256) void k_step2w(StepArgs a)
-{
-    static_assert(P == SP_SH || P == SP_PSV, "two cells per lane: SH and P-SV");
-    constexpr int NS = n_s(P), NV = n_v(P);
-    const int nb = gridDim.x, bb = blockIdx.x;
-    const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
-    const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
-    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
-    if (wave >= a.nitems) return;
-    const int L = threadIdx.x & 63;
-    const int4 item = a.items[2 * wave];
-    if (item.x < 0) return;
-    const int4 item2 = a.items[2 * wave + 1];
-    const ShotDesc &d = a.shots[item.x];
-    const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
-    const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
-    const int jl = item.y - a.shift + 2 * L;          // cell 0 of this lane (even)
-    const bool jin = jl >= 0 && jl < nz;               // both cells (nz even)
-    const bool outl = jin && 2 * L >= a.shift && 2 * L < a.shift + a.wout;
-    bool zm_ok[2], zp_ok[2];
-    double bzb[2], bzt[2];
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-        const int j = jl + c;
-        zm_ok[c] = j >= 2 && j < nz - 2;
-        zp_ok[c] = j >= 1 && j < nz - 2;
-        const int jc = min(max(j, 0), nz - 1);
-        bzb[c] = d.bzb[jc];
-        bzt[c] = d.bzt[jc];
-    }
-    const int jp = min(max(jl, 0), nz - 2);            // clamped pair start
-    const int NP4 = (int)(d.Npad * 4);
-    const double dt = a.dt;
-    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
-    const int rd = a.rd, wr = a.rd ^ 1;
-    const bool zint = __all(bzb[0] == 1.0 && bzt[0] == 1.0 && bzb[1] == 1.0 && bzt[1] == 1.0);
-    const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
-    const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
-    const cflt_p bxp = (cflt_p)d.bx;
-    const unsigned vj = (unsigned)jp * 4u;
-    const unsigned vo = outl ? (unsigned)jl * 4u : OOB;
-
-    const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
-    const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
-    const int ni = a.adjoint ? d.nrec : d.nsrc;
-    const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
-    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
-    int p1 = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
-    int p2 = p1;
-    while (p2 < ni && ik[p2] < (i0 - 1) * nz) ++p2;
-    int row1 = p1 < ni ? ik[p1] / nz : 0x7fffffff;
-    int row2 = p2 < ni ? ik[p2] / nz : 0x7fffffff;
-    const cint_p rk = (cint_p)d.rec_k;
-    const cint_p rrow = (cint_p)d.rec_row;
-    int pr = __builtin_amdgcn_readfirstlane(item2.z);
-    int rec_row = (!a.adjoint && pr < d.nrec) ? rk[pr] / nz : 0x7fffffff;
-    const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
-
-    float S0[4][NS][2], W1[4][NV][2], S1[4][NS][2], W2[4][NV][2];
-    float RW[4][2], LW[4][2], MW[4][2];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            RW[u][c] = 1.f;
-            LW[u][c] = MW[u][c] = 0.f;
-#pragma unroll
-            for (int f = 0; f < NS; f++) S0[u][f][c] = S1[u][f][c] = 0.f;
-#pragma unroll
-            for (int f = 0; f < NV; f++) W1[u][f][c] = W2[u][f][c] = 0.f;
-        }
-    }
-    Row2W<P> PF[4];
-
-    const int m0 = i0 - 3, mend = i1 + 5;
-    load2w<P>(PF[1], RS, RM, NP4, rd, m0 - 3, nx, nz, vj, i0, i1);
-    load2w<P>(PF[2], RS, RM, NP4, rd, m0 - 2, nx, nz, vj, i0, i1);
-    load2w<P>(PF[3], RS, RM, NP4, rd, m0 - 1, nx, nz, vj, i0, i1);
-    load2w<P>(PF[0], RS, RM, NP4, rd, m0, nx, nz, vj, i0, i1);
-#pragma unroll
-    for (int f = 0; f < NS; f++)
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            S0[1][f][c] = PF[1].s[f][c];
-            S0[2][f][c] = PF[2].s[f][c];
-            S0[3][f][c] = PF[3].s[f][c];
-        }
-    load2w<P>(PF[1], RS, RM, NP4, rd, m0 + 1, nx, nz, vj, i0, i1);
-
-    for (int mb = m0; mb <= mend; mb += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int m = mb + u;
-            if (m > mend) break;
-            const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
-            load2w<P>(PF[B], RS, RM, NP4, rd, m + 2, nx, nz, vj, i0, i1);
-            const Row2W<P> &cur = PF[D];
-#pragma unroll
-            for (int f = 0; f < NS; f++)
-#pragma unroll
-                for (int c = 0; c < 2; c++) S0[D][f][c] = cur.s[f][c];
-
-            // ---- A: v(n+1) at row m-1
-            {
-                const int r = m - 1;
-                const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
-                float dsv[NV][2], vn[NV][2];
-                vel_row_w<P, EX>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ);
-                inject_w<P>(dsv, r, jl, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
-                const float bxr = bxp[min(max(r, 0), nx - 1)];
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    float dc[NV], vc[NV], vo_[NV];
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        dc[f] = dsv[f][c];
-                        vo_[f] = cur.v[f][c];
-                    }
-                    vel_finish<P, EX>(vc, dc, vo_, cur.rhoA[c], 1.f, bxr, bzb[c], bzt[c], zint, rin, jin, dt);
-#pragma unroll
-                    for (int f = 0; f < NV; f++) W1[C][f][c] = vc[f];
-                    RW[D][c] = cur.rhoA[c];
-                }
-            }
-            // ---- B: s(n+1) at row m-3
-            {
-                const int q = m - 3;
-                const bool xp_ok = q >= 1 && q < nx - 2;
-                str_row_w<P, EX>(S1[A], W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, xp_ok, zp_ok, KX, KZ, dt);
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    LW[D][c] = cur.lamB[c];
-                    MW[D][c] = cur.muB[c];
-                }
-            }
-            // ---- C: v(n+2) at row m-4
-            {
-                const int r = m - 4;
-                const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
-                float dsv[NV][2], u1[NV][2], u2[NV][2];
-                vel_row_w<P, EX>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ);
-                inject_w<P>(dsv, r, jl, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
-                const float bxr = bxp[min(max(r, 0), nx - 1)];
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    float dc[NV], vc[NV], vo_[NV];
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        dc[f] = dsv[f][c];
-                        vo_[f] = W1[D][f][c];
-                    }
-                    vel_finish<P, EX>(vc, dc, vo_, RW[A][c], 1.f, bxr, bzb[c], bzt[c], zint, rin, jin, dt);
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        u1[f][c] = upd_disp<EX>(cur.u[f][c], W1[D][f][c], dt);
-                        u2[f][c] = upd_disp<EX>(u1[f][c], vc[f], dt);
-                        W2[D][f][c] = vc[f];
-                    }
-                }
-                const bool rok = r >= i0 && r < i1;
-                const unsigned ov = rok ? vo : OOB;
-                const int orow = rok ? r * nz * 4 : 0;
-#pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    bst2(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
-                    bst2(RS, ov, slot_v(P, wr, f) * NP4 + orow, W2[D][f]);
-                }
-                if (a.snap_v >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        bst2(RP, ov, f * NP4 + orow, cur.u[f]);
-                        bst2(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
-                    }
-                }
-                if (a.snap_u >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        bst2(RP, ov, f * NP4 + orow, u1[f]);
-                        bst2(RP, ov, (NV + f) * NP4 + orow, W2[D][f]);
-                    }
-                }
-                // save_obs of both steps for receivers in this row
-                if (r == rec_row && r < i1) {
-                    const int kend = (r + 1) * nz, kl = r * nz + jl;
-                    while (pr < d.nrec && rk[pr] < kend) {
-                        const int row = rrow[pr];
-                        const int c = rk[pr] - kl;  // 0 or 1 in the lane holding the receiver
-                        const unsigned orc = (outl && (c == 0 || c == 1)) ? (unsigned)(row * a.nt + itx) * 4u : OOB;
-                        const int cs = d.nrec * a.nt * 4;
-#pragma unroll
-                        for (int f = 0; f < NV; f++) {
-                            bst(RR, orc, f * cs, c == 1 ? u1[f][1] : u1[f][0]);
-                            bst(RR, orc, f * cs + 4, c == 1 ? u2[f][1] : u2[f][0]);
-                        }
-                        ++pr;
-                    }
-                    rec_row = pr < d.nrec ? rk[pr] / nz : 0x7fffffff;
-                }
-            }
-            // ---- D: s(n+2) at row m-6
-            {
-                const int q = m - 6;
-                const bool xp_ok = q >= 1 && q < nx - 2;
-                float sn[NS][2];
-                str_row_w<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], xp_ok, zp_ok, KX, KZ, dt);
-                const bool qok = q >= i0 && q < i1;
-                const unsigned oq = qok ? vo : OOB;
-                const int orow = qok ? q * nz * 4 : 0;
-#pragma unroll
-                for (int f = 0; f < NS; f++) bst2(RS, oq, slot_s(P, wr, f) * NP4 + orow, sn[f]);
-            }
-        }
-    }
-}
-
-// save_obs (solver.py:151-156) for every receiver of a batch, after a step:
-// one thread per (shot, receiver) of the batch.
-__global__ __launch_bounds__(256) void k_record(const ShotDesc *shots, int nshots, int maxrec, int it, int nt, int P,
-                                                const int *itp)
-{
-    if (itp) it += *itp;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int s = t / maxrec, q = t - s * maxrec;
-    if (s >= nshots) return;
-    const ShotDesc &d = shots[s];
-    if (q >= d.nrec) return;
-    const int k = d.rec_k[q], row = d.rec_row[q];
-    const int nv = n_v(P);
-    for (int f = 0; f < nv; f++)
-        d.rec[((long long)f * d.nrec + row) * nt + it] = d.state[(long long)(2 * n_s(P) + 2 * nv + f) * d.Npad + k];
-}
-
-// advances the step counter of a replayed graph (one thread, vector store)
-__global__ void k_advance(int *counter, int steps)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += steps;
-}
-
-// Adjoint snapshots (solver.py:193-203, 236-242): u before a step (part 0)
-// or the new v after it (part 1) into snapshot isa, for every shot.
-__global__ __launch_bounds__(256) void k_snap(const ShotDesc *shots, int nshots, long long maxN, int isa, int part,
-                                              int vset, int P)
-{
-    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-    const int s = blockIdx.y;
-    if (s >= nshots) return;
-    const ShotDesc &d = shots[s];
-    if (k >= (long long)d.nx * d.nz) return;
-    const int nv = n_v(P), ns = n_s(P);
-    const long long NP = d.Npad;
-    float *dst = d.snap + ((long long)isa * 2 * nv + part * nv) * NP + k;
-    const float *src = d.state + (part == 0 ? (long long)(2 * ns + 2 * nv) : (long long)(2 * ns + vset * nv)) * NP + k;
-    for (int f = 0; f < nv; f++) dst[f * NP] = src[f * NP];
-}
-
-// ------------------------------------------------------ interaction kernel
-struct InteractArgs {
-    const ShotDesc *shots;
-    const int *list;  // shots of this grid in the batch
-    int nlist;
-    int nx, nz;
-    long long Npad;
-    float *kern;      // [SP_NKERN][Npad]
-    int cur;          // velocity set holding the adjoint velocity
-    int isae;
-    double ndt;
-    DCo KX, KZ;  // derivative constants (the exact split form is used)
-};
-
-// D+ in x and z in the reference's operation order (runs every sae steps,
-// so always in the exact form)
-__device__ __forceinline__ float dpx(const float *u, int i, long long k, int nx, int nz, const DCo &K)
-{
-    if (i >= 1 && i < nx - 2)
-        return (float)dpl<true>(u[k - nz], u[k], u[k + nz], u[k + 2 * (long long)nz], K);
-    return 0.f;
-}
-__device__ __forceinline__ float dpz(const float *u, int j, long long k, int nz, const DCo &K)
-{
-    if (j >= 1 && j < nz - 2) return (float)dpl<true>(u[k - 1], u[k], u[k + 1], u[k + 2], K);
-    return 0.f;
-}
-
-// SH: interaction_muy on D+ of the adjoint and the forward uy
-// (solver.py:159-163, 315-321).  P-SV and spin: this library's kernels,
-// accumulated in float64 from the float32 derivative fields:
-//   k_lam += div_a div ndt;  k_mu += (2 exx_a exx + 2 ezz_a ezz + g_a g) ndt
-//   k_rho += (vx_a vx + vz_a vz) ndt;  spin: k_nu += 4 E_a E ndt,
-//   k_j += vyc_a vyc ndt, k_muc += grad phi_a . grad phi ndt,
-//   with g = dz ux + dx uz and E = -uy_c - (dx uz - dz ux)/2.
-template <int P>
-__global__ __launch_bounds__(256) void k_interact(InteractArgs a)
-{
-#pragma clang fp contract(off)
-    const long long N = (long long)a.nx * a.nz;
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const int i = (int)(k / a.nz), j = (int)(k - (long long)i * a.nz);
-    const int nx = a.nx, nz = a.nz;
-    const long long NP = a.Npad;
-    float *K = a.kern;
-    if constexpr (P == SP_SH) {
-        float km = K[SP_K_MU * NP + k];
-        double kh = K[SP_K_HESS * NP + k], kr = K[SP_K_RHO * NP + k];
-        for (int s = 0; s < a.nlist; s++) {
-            const ShotDesc &d = a.shots[a.list[s]];
-            const float *ua = d.state + slot_u(P, 0) * NP;
-            const float *uf = d.snap + (long long)a.isae * n_snap(P) * NP;
-            const float ax = dpx(ua, i, k, nx, nz, a.KX), fx = dpx(uf, i, k, nx, nz, a.KX);
-            const float az = dpz(ua, j, k, nz, a.KZ), fz = dpz(uf, j, k, nz, a.KZ);
-            km = (float)((double)km + (double)add_rn(mul_rn(ax, fx), mul_rn(az, fz)) * a.ndt);
-            const double vf = uf[NP + k], va = d.state[slot_v(P, a.cur, 0) * NP + k];
-            kr = (double)(float)(kr + va * vf * a.ndt);
-            kh = (double)(float)(kh + vf * vf * a.ndt);
-        }
-        K[SP_K_MU * NP + k] = km;
-        K[SP_K_RHO * NP + k] = (float)kr;
-        K[SP_K_HESS * NP + k] = (float)kh;
-        return;
-    }
-    double kl = K[SP_K_LAM * NP + k], kmu = K[SP_K_MU * NP + k], kr = K[SP_K_RHO * NP + k];
-    double kh = K[SP_K_HESS * NP + k];
-    double knu = 0, kj = 0, kc = 0;
-    if constexpr (P == SP_SPIN) {
-        knu = K[SP_K_NU * NP + k];
-        kj = K[SP_K_J * NP + k];
-        kc = K[SP_K_MUC * NP + k];
-    }
-    for (int s = 0; s < a.nlist; s++) {
-        const ShotDesc &d = a.shots[a.list[s]];
-        const float *uxa = d.state + slot_u(P, 0) * NP, *uza = d.state + slot_u(P, 1) * NP;
-        const float *sn = d.snap + (long long)a.isae * n_snap(P) * NP;
-        const float *uxf = sn, *uzf = sn + NP;
-        const double axx = dpx(uxa, i, k, nx, nz, a.KX), axz = dpz(uxa, j, k, nz, a.KZ);
-        const double azx = dpx(uza, i, k, nx, nz, a.KX), azz = dpz(uza, j, k, nz, a.KZ);
-        const double fxx = dpx(uxf, i, k, nx, nz, a.KX), fxz = dpz(uxf, j, k, nz, a.KZ);
-        const double fzx = dpx(uzf, i, k, nx, nz, a.KX), fzz = dpz(uzf, j, k, nz, a.KZ);
-        const double vxa = d.state[slot_v(P, a.cur, 0) * NP + k], vza = d.state[slot_v(P, a.cur, 1) * NP + k];
-        const double vxf = sn[(n_v(P) + 0) * NP + k], vzf = sn[(n_v(P) + 1) * NP + k];
-        // the same expressions, in the same order, as the oracle
-        const double diva = axx + azz, divf = fxx + fzz, sha = axz + azx, shf = fxz + fzx;
-        kl = (double)(float)(kl + diva * divf * a.ndt);
-        kmu = (double)(float)(kmu + (2.0 * axx * fxx + 2.0 * azz * fzz + sha * shf) * a.ndt);
-        kr = (double)(float)(kr + (vxa * vxf + vza * vzf) * a.ndt);
-        kh = (double)(float)(kh + (vxf * vxf + vzf * vzf) * a.ndt);
-        if constexpr (P == SP_SPIN) {
-            const float *uca = d.state + slot_u(P, 2) * NP, *ucf = sn + 2 * NP;
-            const double ea = -(double)uca[k] - 0.5 * (azx - axz);
-            const double ef = -(double)ucf[k] - 0.5 * (fzx - fxz);
-            knu = (double)(float)(knu + 4.0 * ea * ef * a.ndt);
-            const double vca = d.state[slot_v(P, a.cur, 2) * NP + k], vcf = sn[5 * NP + k];
-            kj = (double)(float)(kj + vca * vcf * a.ndt);
-            const double cax = dpx(uca, i, k, nx, nz, a.KX), caz = dpz(uca, j, k, nz, a.KZ);
-            const double cfx = dpx(ucf, i, k, nx, nz, a.KX), cfz = dpz(ucf, j, k, nz, a.KZ);
-            kc = (double)(float)(kc + (cax * cfx + caz * cfz) * a.ndt);
-        }
-    }
-    K[SP_K_LAM * NP + k] = (float)kl;
-    K[SP_K_MU * NP + k] = (float)kmu;
-    K[SP_K_RHO * NP + k] = (float)kr;
-    K[SP_K_HESS * NP + k] = (float)kh;
-    if constexpr (P == SP_SPIN) {
-        K[SP_K_NU * NP + k] = (float)knu;
-        K[SP_K_J * NP + k] = (float)kj;
-        K[SP_K_MUC * NP + k] = (float)kc;
-    }
-}
-
-// ---------------------------------------------------------- misfit kernel
-// diff kernel of misfit/waveform.py:5-25 plus the norm's partial sums.
-// One block per (shot, component, receiver) triple of the batch.
-struct MisfitArgs {
-    const ShotDesc *shots;
-    const int4 *triples;  // (shot, comp, rec, -)
-    const float *const *data;  // per shot, [n_rec][nrec][nt]
-    double *partial;
-    const double *taper;  // [nt], computed on the host with the C library's cos
-    int n, nt;
-};
-
-__global__ __launch_bounds__(256) void k_misfit(MisfitArgs a)
-{
-    const int b = blockIdx.x;
-    if (b >= a.n) return;
-    const int4 t = a.triples[b];
-    const ShotDesc &d = a.shots[t.x];
-    const int nt = a.nt;
-    const long long base = ((long long)t.y * d.nrec + t.z) * nt;
-    const float *syn = a.data[t.x] + base;  // observed data ("syn" in fd2d.py:305)
-    const float *obs = d.rec + base;         // current synthetics
-    float *ad = d.adstf + base;
-    double acc = 0.0;
-    for (int it = threadIdx.x; it < nt; it += blockDim.x) {
-        const double taper = a.taper[it];
-        const float res = (float)((double)sub_rn(syn[it], obs[it]) * taper);
-        ad[nt - it - 1] = (float)((double)res * taper * 2.0);
-        acc += (double)res * (double)res;
-    }
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partial[b] = red[0];
-}
-
-// ------------------------------------------------------- smoothing kernels
-// apply_gauxxian_x / _z with init_gausian's normaliser (fd2d.py:71-108).
-// g[d] = exp(-d^2/(2 s^2)) / (sqrt(2 pi) s) for |i-n| = d, in float64.
-__global__ __launch_bounds__(256) void k_smooth_x(const float *data, float *gtmp, const double *g, int nx, int nz)
-{
-#pragma clang fp contract(off)
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long long)nx * nz) return;
-    const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
-    double s = 0.0;
-    for (int n = 0; n < nx; n++) s += g[i > n ? i - n : n - i] * (double)data[(long long)n * nz + j];
-    gtmp[k] = (float)s;
-}
-
-__global__ __launch_bounds__(256) void k_smooth_z(float *data, const float *gtmp, const double *g,
-                                                  const double *sumx, const double *sumz, int nx, int nz)
-{
-#pragma clang fp contract(off)
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long long)nx * nz) return;
-    const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
-    const float *row = gtmp + (long long)i * nz;
-    double s = 0.0;
-    for (int n = 0; n < nz; n++) s += g[j > n ? j - n : n - j] * (double)row[n];
-    const float gsum = (float)(sumx[i] * sumz[j]);
-    data[k] = (float)(s / (double)gsum);
-}
-
-// ---------------------------------------------------------- small kernels
-__global__ void k_fill(float *p, long long n, float v)
-{
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = v;
-}
-
-// vps2lm: fd2d.py:21-33 (lam holds vp and mu holds vs on entry)
-__global__ void k_vps2lm(float *lam, float *mu, const float *rho, long long n)
-{
-#pragma clang fp contract(off)
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const float vp = lam[k], vs = mu[k];
-    if (vp > vs) lam[k] = (float)((double)rho[k] * ((double)(vp * vp) - 2.0 * (double)vs * (double)vs));
-    else lam[k] = 0.f;
-    mu[k] = rho[k] * vs * vs;
-}
-
-__global__ void k_gamma(float *gam, const float *muc, const float *nuc, long long n)
-{
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) gam[k] = add_rn(muc[k], nuc[k]);
-}
-
-// vp/vs/rho kernels from lam/mu/rho kernels: lam = rho (vp^2 - 2 vs^2),
-// mu = rho vs^2 (the parameterisation of vps2lm).
-__global__ void k_kern_vps(float *K, const float *mat, long long n, long long NP)
-{
-#pragma clang fp contract(off)
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double rho = mat[M_RHO * NP + k], lam = mat[M_LAM * NP + k], mu = mat[M_MU * NP + k];
-    const double vp = sqrt((lam + 2.0 * mu) / rho), vs = sqrt(mu / rho);
-    const double kl = K[SP_K_LAM * NP + k], km = K[SP_K_MU * NP + k], kr = K[SP_K_RHO * NP + k];
-    K[SP_K_VP * NP + k] = (float)(2.0 * rho * vp * kl);
-    K[SP_K_VS * NP + k] = (float)(2.0 * rho * vs * (km - 2.0 * kl));
-    K[SP_K_RHOP * NP + k] = (float)((vp * vp - 2.0 * vs * vs) * kl + vs * vs * km + kr);
-}
+#include "sp_layout.h"
+#include "sp_arith.h"
+#include "k_step.h"
+#include "k_step3.h"
+#include "k_step2.h"
+#include "k_step2w.h"
+#include "sp_kernels.h"
 
 // ===================================================================== host
 struct Grid {

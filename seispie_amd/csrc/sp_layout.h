@@ -1,0 +1,85 @@
+// seispie_hip -- device data layout: state slots, shot descriptors, step
+// arguments, cross-lane rotations.
+// Part of seispie_hip.hip: included there, inside its anonymous namespace,
+// after <hip/hip_runtime.h> and the headers before this one.
+#pragma once
+
+// ------------------------------------------------------------------ layout
+// Stress, velocity and displacement fields per physics.
+//   SH:   s = {sxy, szy}                         v = {vy}          u = {uy}
+//   P-SV: s = {sxx, szz, sxz}                    v = {vx, vz}      u = {ux, uz}
+//   spin: s = {sxx, szz, sxz, syx_c, syy_c, syz_c} v = {vx, vz, vy_c} u = {ux, uz, uy_c}
+// State slots of a shot: two stress sets, two velocity sets, one u set.
+__host__ __device__ constexpr int n_s(int p) { return p == SP_SH ? 2 : (p == SP_PSV ? 3 : 6); }
+__host__ __device__ constexpr int n_v(int p) { return p == SP_SH ? 1 : (p == SP_PSV ? 2 : 3); }
+__host__ __device__ constexpr int n_slots(int p) { return 2 * n_s(p) + 3 * n_v(p); }
+__host__ __device__ constexpr int slot_s(int p, int set, int f) { return set * n_s(p) + f; }
+__host__ __device__ constexpr int slot_v(int p, int set, int f) { return 2 * n_s(p) + set * n_v(p) + f; }
+__host__ __device__ constexpr int slot_u(int p, int f) { return 2 * n_s(p) + 2 * n_v(p) + f; }
+// snapshot fields: u (before the step) then v (after it), solver.py:193-242
+__host__ __device__ constexpr int n_snap(int p) { return 2 * n_v(p); }
+// components of the records (SH y; P-SV x, z; spin x, z, y_c), of the
+// injected sources (SH y; P-SV/spin x, z) and of the misfit.
+__host__ __device__ constexpr int n_rec(int p) { return n_v(p); }
+__host__ __device__ constexpr int n_inj(int p) { return p == SP_SH ? 1 : 2; }
+__host__ __device__ constexpr int n_mis(int p) { return p == SP_SH ? 1 : 2; }
+// material slots of a grid
+constexpr int M_RHO = SP_F_RHO, M_LAM = SP_F_LAM, M_MU = SP_F_MU, M_NU = SP_F_NU, M_J = SP_F_J;
+constexpr int M_GAM = 8;  // float32(mu_c + nu_c), as the reference adds them (solver.py:140)
+constexpr int N_MAT = 9;
+
+struct ShotDesc {
+    const float *mat;   // [N_MAT][Npad]
+    const float *bx;    // [nx] x-side sponge factors (already float32-rounded)
+    const double *bzb;  // [nz] bottom factor or 1
+    const double *bzt;  // [nz] top factor or 1
+    float *state;       // [n_slots][Npad]
+    float *rec;         // [n_rec][nrec][nt]
+    float *snap;        // [nsa][n_snap][Npad] or null
+    const float *stf;   // [n_inj][nsrc][nt]
+    float *adstf;       // [n_mis][nrec][nt]
+    const int *src_k, *src_row, *rec_k, *rec_row;  // sorted by k
+    int nsrc, nrec, nx, nz;
+    long long Npad;
+    double c1x, c2x, c1z, c2z;  // 9/(8h), 1/(24h)
+    // exact mode: 9/(8h) and 1/(24h) as unevaluated sums hi + lo of two doubles
+    double k9hx, k9lx, k1hx, k1lx, k9hz, k9lz, k1hz, k1lz;
+};
+
+// read-only, wave-uniform data through the constant address space: scalar
+// loads, counted by lgkmcnt and never by the vector memory counter
+typedef const int __attribute__((address_space(4))) *cint_p;
+typedef const float __attribute__((address_space(4))) *cflt_p;
+
+struct StepArgs {
+    const ShotDesc *shots;
+    const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-4) nz,
+                        // the first receiver with k >= (i0-4) nz and the first with k >= i0 nz
+    int nitems;
+    int it, nt;         // step of this launch (plus *itp when itp is set)
+    const int *itp;     // device step counter of a replayed graph, or null
+    double dt;
+    int rd;        // stress/velocity set read this step
+    int snap_idx;  // -1: no snapshot this step (one-step kernel k_step)
+    // two-step kernel: the snapshot of the pair's first step (snap_v) and
+    // of its second step (snap_u), -1: none; the kernel writes both halves
+    // of each (u before the step, v after it)
+    int snap_v, snap_u;
+    int adjoint;   // inject adjoint sources at the receivers, record nothing
+    int shift;     // cells of a strip before its first output cell (>= 3)
+    int wout;      // output cells per strip (shift + wout <= 61; k_step2w: 6 + 116)
+};
+
+// z neighbours by whole-wavefront DPP rotations: lane L receives the value
+// of lane L-1 (lane_m1) or L+1 (lane_p1), lane 0 / 63 wrapping to 63 / 0 as
+// a 64-wide shuffle does (checked on gfx950).  One VALU move each, where a
+// shuffle through the LDS crossbar waits on the LDS queue.  Every caller
+// runs them with all 64 lanes active.
+__device__ __forceinline__ float lane_m1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C /* wave_ror:1 */, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_p1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134 /* wave_rol:1 */, 0xf, 0xf, false));
+}
