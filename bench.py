@@ -204,7 +204,7 @@ def main():
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
     ap.add_argument("--fuse", type=int, default=-1,
                     help="steps per launch option: 0 one step, 1 two steps (k_step2), 2 two steps with two "
-                         "cells per lane (k_step2w; SH and P-SV with even nz)")
+                         "cells per lane (k_step2 with W = 2; SH and P-SV with even nz)")
     ap.add_argument("--fast", action="store_true",
                     help="step kernels in fast mode (reciprocal multiplications and fused multiply-adds) instead "
                          "of the default exact mode, whose records equal the oracle bit for bit")
@@ -297,8 +297,8 @@ def main():
     steps_per_launch = bytes_per_launch / init.alg_bytes_per_step()
     wide = ctx.get_option(_lib.OPT_FUSE) == 2 and init.physics != _lib.SPIN and all(g.nz % 2 == 0 for g in init.grids)
     if steps_per_launch > 1.5:
-        kname = ("k_step2w<%s> (two time steps per launch, two cells per lane)" if wide
-                 else "k_step2<%s> (two time steps per launch)")
+        kname = ("k_step2<%s, W=2> (two time steps per launch, two cells per lane)" if wide
+                 else "k_step2<%s, W=1> (two time steps per launch)")
     else:
         kname = "k_step3<%s>"
     kname = kname % {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]

@@ -1,4 +1,5 @@
-// seispie_hip -- k_step2, two time steps per launch.
+// seispie_hip -- k_step2<P, EX, W>, two time steps per launch with W = 1 or 2
+// z cells per lane.
 // Part of seispie_hip.hip: included there, inside its anonymous namespace,
 // after <hip/hip_runtime.h> and the headers before this one.
 #pragma once
@@ -8,12 +9,20 @@
 // keeping the intermediate state in registers: the state is read once and
 // written once per two steps.  The intermediate stresses and velocities of a
 // chunk's halo rows are recomputed by each chunk that needs them; the strip
-// keeps 6 lanes of halo on each side (lanes [6, 58) are output), and the
-// chunk reads 6 rows before and 5 after its own.  Every value is computed
-// with the same operations, in the same order, as two launches of k_step3,
-// so results are identical bit for bit.  The result goes to the other
-// stress/velocity set (rd ^ 1), like one step, so no wavefront overwrites
-// rows another one still reads.
+// keeps 6 cells of halo on each side, and the chunk reads 6 rows before and
+// 5 after its own.  Every value is computed with the same operations, in the
+// same order, as two launches of k_step3, so results are identical bit for
+// bit.  The result goes to the other stress/velocity set (rd ^ 1), like one
+// step, so no wavefront overwrites rows another one still reads.
+//
+// W cells per lane: lane L holds z cells j0 + W L .. j0 + W L + W - 1 of
+// every field.
+//   W = 1: a wavefront covers 64 cells, lanes [6, 58) are output.
+//   W = 2: 128 cells, 116 of them output, and every load and
+//     store moves both cells at once (8 bytes per lane), so a row issues
+//     half the memory instructions per cell.  Needs even nz (a lane's two
+//     cells are then both inside or both outside the grid and every access
+//     is 8-byte aligned); SH and P-SV (spin would need more than 256 VGPRs).
 //
 // Iteration m of the march (rows relative to m):
 //   A  v(n+1)  at row m-1 from s(n) rows m-3..m   (source at step it)
@@ -23,14 +32,40 @@
 //   D  s(n+2)  at row m-6 from v(n+2) rows m-7..m-4
 // Windows of four rows are indexed by row & 3 (loop unrolled by four).
 
-template <int P>
+// W consecutive floats of a lane at its offset: one 4- or 8-byte access
+typedef unsigned v2u_t __attribute__((ext_vector_type(2)));
+template <int W>
+__device__ __forceinline__ void bldw(float (&x)[W], rsrc_t r, unsigned voff, int soff)
+{
+    if constexpr (W == 1) {
+        x[0] = bld(r, voff, soff);
+    } else {
+        const v2u_t w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0);
+        x[0] = __uint_as_float(w.x);
+        x[1] = __uint_as_float(w.y);
+    }
+}
+template <int W>
+__device__ __forceinline__ void bstw(rsrc_t r, unsigned voff, int soff, const float (&x)[W])
+{
+    if constexpr (W == 1) {
+        bst(r, voff, soff, x[0]);
+    } else {
+        v2u_t w;
+        w.x = __float_as_uint(x[0]);
+        w.y = __float_as_uint(x[1]);
+        __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)voff, soff, 0);
+    }
+}
+
+template <int P, int W>
 struct Row2 {
-    float s[n_s(P)];     // s(n), row m
-    float v[n_v(P)];     // v(n), row m-1
-    float u[n_v(P)];     // u(n), row m-4
-    float rhoA, jA;      // row m-1 (j: micro-inertia, spin only)
-    float lamB, muB;     // row m-3
-    float nuB, gamB;     // row m-3 (spin only: nu, mu_c + nu_c)
+    float s[n_s(P)][W];     // s(n), row m
+    float v[n_v(P)][W];     // v(n), row m-1
+    float u[n_v(P)][W];     // u(n), row m-4
+    float rhoA[W], jA[W];   // row m-1 (j: micro-inertia, spin only)
+    float lamB[W], muB[W];  // row m-3
+    float nuB[W], gamB[W];  // row m-3 (spin only: nu, mu_c + nu_c)
     // rows m-4 and m-6 of the same materials are the values loaded three
     // iterations earlier; the march keeps them in register windows
 };
@@ -39,9 +74,9 @@ struct Row2 {
 // offset and the per-lane offset vj = 4 jc is the same for every load (no
 // vector address arithmetic per row).  Every row is clamped into the grid,
 // so every load stays inside its buffer.
-template <int P>
-__device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p bxp, int NP4, int rd, int mp, int nx,
-                                      int nz, unsigned vj, int i0, int i1)
+template <int P, int W>
+__device__ __forceinline__ void load2(Row2<P, W> &p, rsrc_t RS, rsrc_t RM, int NP4, int rd, int mp, int nx, int nz,
+                                      unsigned vj, int i0, int i1)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
     // rows clamped to what the chunk needs (and to the grid)
@@ -52,64 +87,110 @@ __device__ __forceinline__ void load2(Row2<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
     const int rb = nz * 4;
     const int os = ms * rb, oa = ma * rb, ob = mb * rb, ou = mu_ * rb;
 #pragma unroll
-    for (int f = 0; f < NS; f++) p.s[f] = bld(RS, vj, slot_s(P, rd, f) * NP4 + os);
+    for (int f = 0; f < NS; f++) bldw<W>(p.s[f], RS, vj, slot_s(P, rd, f) * NP4 + os);
 #pragma unroll
     for (int f = 0; f < NV; f++) {
-        p.v[f] = bld(RS, vj, slot_v(P, rd, f) * NP4 + oa);
-        p.u[f] = bld(RS, vj, slot_u(P, f) * NP4 + ou);
+        bldw<W>(p.v[f], RS, vj, slot_v(P, rd, f) * NP4 + oa);
+        bldw<W>(p.u[f], RS, vj, slot_u(P, f) * NP4 + ou);
     }
-    p.rhoA = bld(RM, vj, M_RHO * NP4 + oa);
-    p.muB = bld(RM, vj, M_MU * NP4 + ob);
-    p.lamB = 0.f;
-    if constexpr (P != SP_SH) p.lamB = bld(RM, vj, M_LAM * NP4 + ob);
-    p.jA = 1.f;
-    p.nuB = p.gamB = 0.f;
+    bldw<W>(p.rhoA, RM, vj, M_RHO * NP4 + oa);
+    bldw<W>(p.muB, RM, vj, M_MU * NP4 + ob);
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+        p.lamB[c] = 0.f;
+        p.jA[c] = 1.f;
+        p.nuB[c] = p.gamB[c] = 0.f;
+    }
+    if constexpr (P != SP_SH) bldw<W>(p.lamB, RM, vj, M_LAM * NP4 + ob);
     if constexpr (P == SP_SPIN) {
-        p.jA = bld(RM, vj, M_J * NP4 + oa);
-        p.nuB = bld(RM, vj, M_NU * NP4 + ob);
-        p.gamB = bld(RM, vj, M_GAM * NP4 + ob);
+        bldw<W>(p.jA, RM, vj, M_J * NP4 + oa);
+        bldw<W>(p.nuB, RM, vj, M_NU * NP4 + ob);
+        bldw<W>(p.gamB, RM, vj, M_GAM * NP4 + ob);
+    }
+}
+
+// z neighbours of a field held W cells per lane.  For D- (at z: z-2, z-1,
+// z+1) and D+ (at z: z-1, z+1, z+2); with two cells, the lane's own pair
+// plus one rotated cell from each side covers both.
+template <int W>
+__device__ __forceinline__ void zminus(float (&m2)[W], float (&m1)[W], float (&p1)[W], const float (&x)[W])
+{
+    if constexpr (W == 1) {
+        m1[0] = lane_m1(x[0]);
+        m2[0] = lane_m1(m1[0]);
+        p1[0] = lane_p1(x[0]);
+    } else {
+        const float a_m = lane_m1(x[0]), b_m = lane_m1(x[1]), a_p = lane_p1(x[0]);
+        m2[0] = a_m;  m1[0] = b_m;  p1[0] = x[1];
+        m2[1] = b_m;  m1[1] = x[0]; p1[1] = a_p;
+    }
+}
+template <int W>
+__device__ __forceinline__ void zplus(float (&m1)[W], float (&p1)[W], float (&p2)[W], const float (&x)[W])
+{
+    if constexpr (W == 1) {
+        m1[0] = lane_m1(x[0]);
+        p1[0] = lane_p1(x[0]);
+        p2[0] = lane_p1(p1[0]);
+    } else {
+        const float b_m = lane_m1(x[1]), a_p = lane_p1(x[0]), b_p = lane_p1(x[1]);
+        m1[0] = b_m;  p1[0] = x[1]; p2[0] = a_p;
+        m1[1] = x[0]; p1[1] = a_p;  p2[1] = b_p;
     }
 }
 
 // stress divergence of one velocity row (div_sy / div_sxz, solver.py:54-66)
 // from the stress window rows r-2..r+1 (sa..sd)
-template <int P, bool EX>
-__device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)], const float (&sa)[n_s(P)], const float (&sb)[n_s(P)],
-                                        const float (&sc)[n_s(P)], const float (&sd)[n_s(P)], bool xm_ok, bool zm_ok,
-                                        int L, const DCo &KX, const DCo &KZ)
+template <int P, bool EX, int W>
+__device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&sa)[n_s(P)][W],
+                                        const float (&sb)[n_s(P)][W], const float (&sc)[n_s(P)][W],
+                                        const float (&sd)[n_s(P)][W], bool xm_ok, const bool (&zm_ok)[W],
+                                        const DCo &KX, const DCo &KZ)
 {
     if constexpr (P == SP_SH) {
-        float zm1 = lane_m1(sc[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(sc[1]);
-        double ddx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
-        double ddz = zm_ok ? dmn<EX>(zm2, zm1, sc[1], zp1, KZ) : 0.0;
-        dsv[0] = (float)(ddx + ddz);
+        float zm2[W], zm1[W], zp1[W];
+        zminus<W>(zm2, zm1, zp1, sc[1]);
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            const double ddx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
+            const double ddz = zm_ok[c] ? dmn<EX>(zm2[c], zm1[c], sc[1][c], zp1[c], KZ) : 0.0;
+            dsv[0][c] = (float)(ddx + ddz);
+        }
     } else {
-        float xzm1 = lane_m1(sc[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(sc[2]);
-        float zzm1 = lane_m1(sc[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(sc[1]);
-        double dxx = xm_ok ? dmn<EX>(sa[0], sb[0], sc[0], sd[0], KX) : 0.0;
-        double dxxz = xm_ok ? dmn<EX>(sa[2], sb[2], sc[2], sd[2], KX) : 0.0;
-        double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, sc[2], xzp1, KZ) : 0.0;
-        double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, sc[1], zzp1, KZ) : 0.0;
-        dsv[0] = (float)(dxx + dzxz);
-        dsv[1] = (float)(dxxz + dzzz);
+        float xzm2[W], xzm1[W], xzp1[W], zzm2[W], zzm1[W], zzp1[W];
+        zminus<W>(xzm2, xzm1, xzp1, sc[2]);
+        zminus<W>(zzm2, zzm1, zzp1, sc[1]);
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            const double dxx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
+            const double dxxz = xm_ok ? dmn<EX>(sa[2][c], sb[2][c], sc[2][c], sd[2][c], KX) : 0.0;
+            const double dzxz = zm_ok[c] ? dmn<EX>(xzm2[c], xzm1[c], sc[2][c], xzp1[c], KZ) : 0.0;
+            const double dzzz = zm_ok[c] ? dmn<EX>(zzm2[c], zzm1[c], sc[1][c], zzp1[c], KZ) : 0.0;
+            dsv[0][c] = (float)(dxx + dzxz);
+            dsv[1][c] = (float)(dxxz + dzzz);
+        }
         if constexpr (P == SP_SPIN) {
             // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
-            float yzm1 = lane_m1(sc[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(sc[5]);
-            float yym1 = lane_m1(sc[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(sc[4]);
-            double dxyx = xm_ok ? dmn<EX>(sa[3], sb[3], sc[3], sd[3], KX) : 0.0;
-            double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, sc[5], yzp1, KZ) : 0.0;
-            float dsy = (float)(dxyx + dzyz);
-            double dzyy = zm_ok ? dmn<EX>(yym2, yym1, sc[4], yyp1, KZ) : 0.0;
-            double dxyy = xm_ok ? dmn<EX>(sa[4], sb[4], sc[4], sd[4], KX) : 0.0;
-            dsv[0] = (float)((double)dsv[0] + dzyy);
-            dsv[1] = (float)((double)dsv[1] - dxyy);
-            dsv[2] = (float)((double)dsy + 2.0 * (double)sc[4]);
+            float yzm2[W], yzm1[W], yzp1[W], yym2[W], yym1[W], yyp1[W];
+            zminus<W>(yzm2, yzm1, yzp1, sc[5]);
+            zminus<W>(yym2, yym1, yyp1, sc[4]);
+#pragma unroll
+            for (int c = 0; c < W; c++) {
+                const double dxyx = xm_ok ? dmn<EX>(sa[3][c], sb[3][c], sc[3][c], sd[3][c], KX) : 0.0;
+                const double dzyz = zm_ok[c] ? dmn<EX>(yzm2[c], yzm1[c], sc[5][c], yzp1[c], KZ) : 0.0;
+                const float dsy = (float)(dxyx + dzyz);
+                const double dzyy = zm_ok[c] ? dmn<EX>(yym2[c], yym1[c], sc[4][c], yyp1[c], KZ) : 0.0;
+                const double dxyy = xm_ok ? dmn<EX>(sa[4][c], sb[4][c], sc[4][c], sd[4][c], KX) : 0.0;
+                dsv[0][c] = (float)((double)dsv[0][c] + dzyy);
+                dsv[1][c] = (float)((double)dsv[1][c] - dxyy);
+                dsv[2][c] = (float)((double)dsy + 2.0 * (double)sc[4][c]);
+            }
         }
     }
 }
 
-// velocity update (add_vy / add_vxz, solver.py:96-117) from the divergence
-// plus sources, the old v, 1/rho and the sponge factors
+// velocity update (add_vy / add_vxz, solver.py:96-117) of one cell from the
+// divergence plus sources, the old v, 1/rho and the sponge factors
 // `zint`: every lane of the strip lies outside the top/bottom sponges; with
 // bx = 1 the sponge factor is exactly 1 and its arithmetic is skipped
 // (the same float64 result, bit for bit).
@@ -161,47 +242,59 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
 
 // stress update of one row (add_sy / add_sxz, solver.py:120-148) from the
 // velocity window rows q-1..q+2 (wa..wd) and the old stresses
-template <int P, bool EX>
-__device__ __forceinline__ void str_row(float (&sn)[n_s(P)], const float (&wa)[n_v(P)], const float (&wb)[n_v(P)],
-                                        const float (&wc)[n_v(P)], const float (&wd)[n_v(P)], const float (&so)[n_s(P)],
-                                        float lam, float mu, float nu, float gam, bool xp_ok, bool zp_ok, int L,
-                                        const DCo &KX, const DCo &KZ, double dt)
+template <int P, bool EX, int W>
+__device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa)[n_v(P)][W],
+                                        const float (&wb)[n_v(P)][W], const float (&wc)[n_v(P)][W],
+                                        const float (&wd)[n_v(P)][W], const float (&so)[n_s(P)][W],
+                                        const float (&lam)[W], const float (&mu)[W], const float (&nu)[W],
+                                        const float (&gam)[W], bool xp_ok, const bool (&zp_ok)[W], const DCo &KX,
+                                        const DCo &KZ, double dt)
 {
     if constexpr (P == SP_SH) {
-        float zm1 = lane_m1(wb[0]), zp1 = lane_p1(wb[0]), zp2 = lane_p1(zp1);
-        const float dvdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
-        const float dvdz = zp_ok ? (float)dpl<EX>(zm1, wb[0], zp1, zp2, KZ) : 0.f;
-        const double dtmu = dt * (double)mu;
-        sn[0] = upd_sadd<EX>(so[0], dtmu, (double)dvdx);
-        sn[1] = upd_sadd<EX>(so[1], dtmu, (double)dvdz);
+        float zm1[W], zp1[W], zp2[W];
+        zplus<W>(zm1, zp1, zp2, wb[0]);
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            const float dvdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
+            const float dvdz = zp_ok[c] ? (float)dpl<EX>(zm1[c], wb[0][c], zp1[c], zp2[c], KZ) : 0.f;
+            const double dtmu = dt * (double)mu[c];
+            sn[0][c] = upd_sadd<EX>(so[0][c], dtmu, (double)dvdx);
+            sn[1][c] = upd_sadd<EX>(so[1][c], dtmu, (double)dvdz);
+        }
     } else {
-        float xm1 = lane_m1(wb[0]), xp1 = lane_p1(wb[0]), xp2 = lane_p1(xp1);
-        float zm1 = lane_m1(wb[1]), zp1 = lane_p1(wb[1]), zp2 = lane_p1(zp1);
-        const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0], wb[0], wc[0], wd[0], KX) : 0.f;
-        const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1], wb[1], wc[1], wd[1], KX) : 0.f;
-        const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, wb[0], xp1, xp2, KZ) : 0.f;
-        const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, wb[1], zp1, zp2, KZ) : 0.f;
-        const double l2m = (double)lam + 2.0 * (double)mu;
-        sn[0] = upd_snorm<EX>(so[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
-        sn[1] = upd_snorm<EX>(so[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
-        sn[2] = upd_sadd<EX>(so[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
-        if constexpr (P == SP_SPIN) {
-            // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
-            float cm1 = lane_m1(wb[2]), cp1 = lane_p1(wb[2]), cp2 = lane_p1(cp1);
-            const float dcdx = xp_ok ? (float)dpl<EX>(wa[2], wb[2], wc[2], wd[2], KX) : 0.f;
-            const float dcdz = zp_ok ? (float)dpl<EX>(cm1, wb[2], cp1, cp2, KZ) : 0.f;
-            const double dtg = dt * (double)gam;
-            sn[4] = upd_srot<EX>(so[4], 2.0 * dt, nu, wb[2], sub_rn(dvzdx, dvxdz));
-            sn[3] = upd_sadd<EX>(so[3], dtg, (double)dcdx);
-            sn[5] = upd_sadd<EX>(so[5], dtg, (double)dcdz);
+        float xm1[W], xp1[W], xp2[W], zm1[W], zp1[W], zp2[W];
+        zplus<W>(xm1, xp1, xp2, wb[0]);
+        zplus<W>(zm1, zp1, zp2, wb[1]);
+        float cm1[W], cp1[W], cp2[W];
+        if constexpr (P == SP_SPIN) zplus<W>(cm1, cp1, cp2, wb[2]);
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
+            const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1][c], wb[1][c], wc[1][c], wd[1][c], KX) : 0.f;
+            const float dvxdz = zp_ok[c] ? (float)dpl<EX>(xm1[c], wb[0][c], xp1[c], xp2[c], KZ) : 0.f;
+            const float dvzdz = zp_ok[c] ? (float)dpl<EX>(zm1[c], wb[1][c], zp1[c], zp2[c], KZ) : 0.f;
+            const double l2m = (double)lam[c] + 2.0 * (double)mu[c];
+            sn[0][c] = upd_snorm<EX>(so[0][c], dt, l2m, dvxdx, mul_rn(lam[c], dvzdz));
+            sn[1][c] = upd_snorm<EX>(so[1][c], dt, l2m, dvzdz, mul_rn(lam[c], dvxdx));
+            sn[2][c] = upd_sadd<EX>(so[2][c], dt, (double)mul_rn(mu[c], add_rn(dvxdz, dvzdx)));
+            if constexpr (P == SP_SPIN) {
+                // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
+                const float dcdx = xp_ok ? (float)dpl<EX>(wa[2][c], wb[2][c], wc[2][c], wd[2][c], KX) : 0.f;
+                const float dcdz = zp_ok[c] ? (float)dpl<EX>(cm1[c], wb[2][c], cp1[c], cp2[c], KZ) : 0.f;
+                const double dtg = dt * (double)gam[c];
+                sn[4][c] = upd_srot<EX>(so[4][c], 2.0 * dt, nu[c], wb[2][c], sub_rn(dvzdx, dvxdz));
+                sn[3][c] = upd_sadd<EX>(so[3][c], dtg, (double)dcdx);
+                sn[5][c] = upd_sadd<EX>(so[5][c], dtg, (double)dcdz);
+            }
         }
     }
 }
 
 // sources (forward) or adjoint sources (adjoint) of velocity row r at step
-// `it`: float32 adds in list order (stf_dsy / stf_dsxz, solver.py:78-93)
-template <int P>
-__device__ __forceinline__ void inject(float (&dsv)[n_v(P)], int r, int j, int nz, int &pi, int &src_row, int ni,
+// `it`: float32 adds in list order (stf_dsy / stf_dsxz, solver.py:78-93);
+// j is the lane's first cell
+template <int P, int W>
+__device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, int nz, int &pi, int &src_row, int ni,
                                        cint_p ik, cint_p irow, cflt_p ival, long long nrows, int nt, int it)
 {
     if (r != src_row) return;
@@ -212,18 +305,22 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)], int r, int j, int n
         const float s0 = ival[(long long)row * nt + it];
         float s1 = 0.f;
         if constexpr (P != SP_SH) s1 = ival[(nrows + row) * nt + it];
-        if (kk == kr) {
-            dsv[0] = add_rn(dsv[0], s0);
-            if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            if (kk == kr + c) {
+                dsv[0][c] = add_rn(dsv[0][c], s0);
+                if constexpr (P != SP_SH) dsv[1][c] = add_rn(dsv[1][c], s1);
+            }
         }
         ++pi;
     }
     src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
 }
 
-template <int P, bool EX>
+template <int P, bool EX, int W>
 __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 {
+    static_assert(W == 1 || (W == 2 && P != SP_SPIN), "two cells per lane: SH and P-SV");
     constexpr int NS = n_s(P), NV = n_v(P);
     const int nb = gridDim.x, bb = blockIdx.x;
     const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
@@ -237,17 +334,27 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     const ShotDesc &d = a.shots[item.x];
     const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
-    const int j = item.y - a.shift + L;
-    const int jc = min(max(j, 0), nz - 1);
-    const bool jin = j >= 0 && j < nz;
-    const bool outl = jin && L >= a.shift && L < a.shift + a.wout;
-    const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
+    const int j = item.y - a.shift + W * L;  // the lane's first cell
+    const int jc = min(max(j, 0), nz - W);   // the first cell it loads
+    const bool jin = j >= 0 && j < nz;       // every cell of the lane (nz is a multiple of W)
+    const bool outl = jin && W * L >= a.shift && W * L < a.shift + a.wout;
+    bool zm_ok[W], zp_ok[W];
+    double bzb[W], bzt[W];
+    bool bz_one = true;
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+        const int jj = j + c, jb = min(max(jj, 0), nz - 1);
+        zm_ok[c] = jj >= 2 && jj < nz - 2;
+        zp_ok[c] = jj >= 1 && jj < nz - 2;
+        bzb[c] = d.bzb[jb];
+        bzt[c] = d.bzt[jb];
+        bz_one = bz_one && bzb[c] == 1.0 && bzt[c] == 1.0;
+    }
     const int NP4 = (int)(d.Npad * 4);
     const double dt = a.dt;
     const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
     const int rd = a.rd, wr = a.rd ^ 1;
-    const double bzb = d.bzb[jc], bzt = d.bzt[jc];
-    const bool zint = __all(bzb == 1.0 && bzt == 1.0);
+    const bool zint = __all(bz_one);
     const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
     const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
     const cflt_p bxp = (cflt_p)d.bx;
@@ -272,37 +379,39 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     int rec_row = (!a.adjoint && pr < d.nrec) ? rk[pr] / nz : 0x7fffffff;
     const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
 
-    float S0[4][NS], W1[4][NV], S1[4][NS], W2[4][NV];
+    float S0[4][NS][W], W1[4][NV][W], S1[4][NS][W], W2[4][NV][W];
     // materials of rows m-1 (rho, j) and m-3 (lam, mu, nu, gam) by
     // iteration: three iterations later they are rows m-4 and m-6
-    float RW[4], JW[4], LW[4], MW[4], NW[4], GW[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        RW[u] = JW[u] = 1.f;
-        LW[u] = MW[u] = NW[u] = GW[u] = 0.f;
-    }
-    Row2<P> PF[4];
+    float RW[4][W], JW[4][W], LW[4][W], MW[4][W], NW[4][W], GW[4][W];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
 #pragma unroll
-        for (int f = 0; f < NS; f++) S0[u][f] = S1[u][f] = 0.f;
+        for (int c = 0; c < W; c++) {
+            RW[u][c] = JW[u][c] = 1.f;
+            LW[u][c] = MW[u][c] = NW[u][c] = GW[u][c] = 0.f;
 #pragma unroll
-        for (int f = 0; f < NV; f++) W1[u][f] = W2[u][f] = 0.f;
+            for (int f = 0; f < NS; f++) S0[u][f][c] = S1[u][f][c] = 0.f;
+#pragma unroll
+            for (int f = 0; f < NV; f++) W1[u][f][c] = W2[u][f][c] = 0.f;
+        }
     }
+    Row2<P, W> PF[4];
 
     // the march starts at m0 = i0 - 3: s(n) rows m0-3..m0-1 fill window slots 1..3
     const int m0 = i0 - 3, mend = i1 + 5;
-    load2<P>(PF[1], RS, RM, bxp, NP4, rd, m0 - 3, nx, nz, vj, i0, i1);
-    load2<P>(PF[2], RS, RM, bxp, NP4, rd, m0 - 2, nx, nz, vj, i0, i1);
-    load2<P>(PF[3], RS, RM, bxp, NP4, rd, m0 - 1, nx, nz, vj, i0, i1);
-    load2<P>(PF[0], RS, RM, bxp, NP4, rd, m0, nx, nz, vj, i0, i1);
+    load2<P, W>(PF[1], RS, RM, NP4, rd, m0 - 3, nx, nz, vj, i0, i1);
+    load2<P, W>(PF[2], RS, RM, NP4, rd, m0 - 2, nx, nz, vj, i0, i1);
+    load2<P, W>(PF[3], RS, RM, NP4, rd, m0 - 1, nx, nz, vj, i0, i1);
+    load2<P, W>(PF[0], RS, RM, NP4, rd, m0, nx, nz, vj, i0, i1);
 #pragma unroll
-    for (int f = 0; f < NS; f++) {
-        S0[1][f] = PF[1].s[f];
-        S0[2][f] = PF[2].s[f];
-        S0[3][f] = PF[3].s[f];
-    }
-    load2<P>(PF[1], RS, RM, bxp, NP4, rd, m0 + 1, nx, nz, vj, i0, i1);
+    for (int f = 0; f < NS; f++)
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            S0[1][f][c] = PF[1].s[f][c];
+            S0[2][f][c] = PF[2].s[f][c];
+            S0[3][f][c] = PF[3].s[f][c];
+        }
+    load2<P, W>(PF[1], RS, RM, NP4, rd, m0 + 1, nx, nz, vj, i0, i1);
 
     for (int mb = m0; mb <= mend; mb += 4) {
 #pragma unroll
@@ -310,53 +419,78 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             const int m = mb + u;
             if (m > mend) break;
             const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
-            load2<P>(PF[B], RS, RM, bxp, NP4, rd, m + 2, nx, nz, vj, i0, i1);
-            const Row2<P> &cur = PF[D];
+            load2<P, W>(PF[B], RS, RM, NP4, rd, m + 2, nx, nz, vj, i0, i1);
+            const Row2<P, W> &cur = PF[D];
 #pragma unroll
-            for (int f = 0; f < NS; f++) S0[D][f] = cur.s[f];
+            for (int f = 0; f < NS; f++)
+#pragma unroll
+                for (int c = 0; c < W; c++) S0[D][f][c] = cur.s[f][c];
 
             // ---- A: v(n+1) at row m-1; s(n) rows m-3..m = slots A, B, C, D
             {
                 const int r = m - 1;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
-                float dsv[NV], vn[NV];
-                vel_row<P, EX>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, L, KX, KZ);
-                inject<P>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
+                float dsv[NV][W];
+                vel_row<P, EX, W>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ);
+                inject<P, W>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, cur.v, cur.rhoA, cur.jA, bxr, bzb, bzt, zint, rin, jin, dt);
-                RW[D] = cur.rhoA;
-                JW[D] = cur.jA;
 #pragma unroll
-                for (int f = 0; f < NV; f++) W1[C][f] = vn[f];
+                for (int c = 0; c < W; c++) {
+                    float dc[NV], vc[NV], vn[NV];
+#pragma unroll
+                    for (int f = 0; f < NV; f++) {
+                        dc[f] = dsv[f][c];
+                        vc[f] = cur.v[f][c];
+                    }
+                    vel_finish<P, EX>(vn, dc, vc, cur.rhoA[c], cur.jA[c], bxr, bzb[c], bzt[c], zint, rin, jin, dt);
+                    RW[D][c] = cur.rhoA[c];
+                    JW[D][c] = cur.jA[c];
+#pragma unroll
+                    for (int f = 0; f < NV; f++) W1[C][f][c] = vn[f];
+                }
             }
             // ---- B: s(n+1) at row m-3; v(n+1) rows m-4..m-1 = slots D, A, B, C
             {
                 const int q = m - 3;
                 const bool xp_ok = q >= 1 && q < nx - 2;
-                float sn[NS];
-                str_row<P, EX>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB, xp_ok, zp_ok, L, KX, KZ, dt);
-                LW[D] = cur.lamB;
-                MW[D] = cur.muB;
-                NW[D] = cur.nuB;
-                GW[D] = cur.gamB;
+                float sn[NS][W];
+                str_row<P, EX, W>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB, xp_ok,
+                                  zp_ok, KX, KZ, dt);
 #pragma unroll
-                for (int f = 0; f < NS; f++) S1[A][f] = sn[f];
+                for (int f = 0; f < NS; f++)
+#pragma unroll
+                    for (int c = 0; c < W; c++) S1[A][f][c] = sn[f][c];
+#pragma unroll
+                for (int c = 0; c < W; c++) {
+                    LW[D][c] = cur.lamB[c];
+                    MW[D][c] = cur.muB[c];
+                    NW[D][c] = cur.nuB[c];
+                    GW[D][c] = cur.gamB[c];
+                }
             }
             // ---- C: v(n+2) at row m-4; s(n+1) rows m-6..m-3 = slots B, C, D, A
             {
                 const int r = m - 4;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
-                float dsv[NV], vn[NV];
-                vel_row<P, EX>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, L, KX, KZ);
-                inject<P>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
+                float dsv[NV][W], u1[NV][W], u2[NV][W];
+                vel_row<P, EX, W>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ);
+                inject<P, W>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
-                vel_finish<P, EX>(vn, dsv, W1[D], RW[A], JW[A], bxr, bzb, bzt, zint, rin, jin, dt);
-                float u1[NV], u2[NV];
 #pragma unroll
-                for (int f = 0; f < NV; f++) {
-                    u1[f] = upd_disp<EX>(cur.u[f], W1[D][f], dt);
-                    u2[f] = upd_disp<EX>(u1[f], vn[f], dt);
-                    W2[D][f] = vn[f];
+                for (int c = 0; c < W; c++) {
+                    float dc[NV], vc[NV], vn[NV];
+#pragma unroll
+                    for (int f = 0; f < NV; f++) {
+                        dc[f] = dsv[f][c];
+                        vc[f] = W1[D][f][c];
+                    }
+                    vel_finish<P, EX>(vn, dc, vc, RW[A][c], JW[A][c], bxr, bzb[c], bzt[c], zint, rin, jin, dt);
+#pragma unroll
+                    for (int f = 0; f < NV; f++) {
+                        u1[f][c] = upd_disp<EX>(cur.u[f][c], W1[D][f][c], dt);
+                        u2[f][c] = upd_disp<EX>(u1[f][c], vn[f], dt);
+                        W2[D][f][c] = vn[f];
+                    }
                 }
                 // rows outside the chunk: every lane's offset is out of range
                 const bool rok = r >= i0 && r < i1;
@@ -364,8 +498,8 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int orow = rok ? r * nz * 4 : 0;
 #pragma unroll
                 for (int f = 0; f < NV; f++) {
-                    bst(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
-                    bst(RS, ov, slot_v(P, wr, f) * NP4 + orow, vn[f]);
+                    bstw<W>(RS, ov, slot_u(P, f) * NP4 + orow, u2[f]);
+                    bstw<W>(RS, ov, slot_v(P, wr, f) * NP4 + orow, W2[D][f]);
                 }
                 // adjoint snapshots of either step (solver.py:193-203, 236-242):
                 // [isa][u before the step | v after it][NV]
@@ -373,16 +507,16 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                     const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
-                        bst(RP, ov, f * NP4 + orow, cur.u[f]);
-                        bst(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
+                        bstw<W>(RP, ov, f * NP4 + orow, cur.u[f]);
+                        bstw<W>(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
                     }
                 }
                 if (a.snap_u >= 0) {
                     const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
-                        bst(RP, ov, f * NP4 + orow, u1[f]);
-                        bst(RP, ov, (NV + f) * NP4 + orow, vn[f]);
+                        bstw<W>(RP, ov, f * NP4 + orow, u1[f]);
+                        bstw<W>(RP, ov, (NV + f) * NP4 + orow, W2[D][f]);
                     }
                 }
                 // save_obs of both steps (solver.py:151-156) for receivers in
@@ -391,12 +525,13 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                     const int kend = (r + 1) * nz, kr = r * nz + j;
                     while (pr < d.nrec && rk[pr] < kend) {
                         const int row = rrow[pr];
-                        const unsigned orc = (outl && rk[pr] == kr) ? (unsigned)(row * a.nt + itx) * 4u : OOB;
+                        const int c = rk[pr] - kr;  // 0 .. W-1 in the lane holding the receiver
+                        const unsigned orc = (outl && c >= 0 && c < W) ? (unsigned)(row * a.nt + itx) * 4u : OOB;
                         const int cs = d.nrec * a.nt * 4;
 #pragma unroll
                         for (int f = 0; f < NV; f++) {
-                            bst(RR, orc, f * cs, u1[f]);
-                            bst(RR, orc, f * cs + 4, u2[f]);
+                            bst(RR, orc, f * cs, (W == 2 && c == 1) ? u1[f][W - 1] : u1[f][0]);
+                            bst(RR, orc, f * cs + 4, (W == 2 && c == 1) ? u2[f][W - 1] : u2[f][0]);
                         }
                         ++pr;
                     }
@@ -407,13 +542,14 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             {
                 const int q = m - 6;
                 const bool xp_ok = q >= 1 && q < nx - 2;
-                float sn[NS];
-                str_row<P, EX>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok, L, KX, KZ, dt);
+                float sn[NS][W];
+                str_row<P, EX, W>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok, KX,
+                                  KZ, dt);
                 const bool qok = q >= i0 && q < i1;
                 const unsigned oq = qok ? vo : OOB;
                 const int orow = qok ? q * nz * 4 : 0;
 #pragma unroll
-                for (int f = 0; f < NS; f++) bst(RS, oq, slot_s(P, wr, f) * NP4 + orow, sn[f]);
+                for (int f = 0; f < NS; f++) bstw<W>(RS, oq, slot_s(P, wr, f) * NP4 + orow, sn[f]);
             }
         }
     }

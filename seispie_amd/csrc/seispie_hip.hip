@@ -5,9 +5,9 @@
 // one thread per cell, each kernel streaming whole arrays through HBM.
 // Here one launch advances every shot of a batch by one or two time steps:
 //
-//   * k_step2<PHYS> (two steps per launch, the default wherever the state
-//     between the steps is not needed), k_step2w<SH|PSV> (the same with two
-//     z cells per lane: 8-byte accesses, 116 outputs per wavefront),
+//   * k_step2<PHYS, EX, W> (two steps per launch, the default wherever the
+//     state between the steps is not needed; W = 2 holds two z cells per
+//     lane, with 8-byte accesses and 116 outputs per wavefront),
 //     k_step3<PHYS> (one step; the same march with buffer loads and stores)
 //     and k_step<PHYS> (plain one-step fallback for grids too large for
 //     32-bit buffer offsets).  In each,
@@ -40,7 +40,7 @@
 // This file holds the host side (contexts, batches, runs, autotuner) and the
 // C ABI.  The device code is in the headers included below, one per part:
 // sp_layout.h (state slots, descriptors), sp_arith.h (the difference
-// arithmetic), k_step.h, k_step3.h, k_step2.h, k_step2w.h (step kernels) and
+// arithmetic), k_step.h, k_step3.h, k_step2.h (step kernels) and
 // sp_kernels.h (records, snapshots, interaction, misfit, smoothing).
 #include <hip/hip_runtime.h>
 
@@ -64,7 +64,6 @@ namespace {
 #include "k_step.h"
 #include "k_step3.h"
 #include "k_step2.h"
-#include "k_step2w.h"
 #include "sp_kernels.h"
 
 // ===================================================================== host
@@ -321,9 +320,9 @@ int batch_kernel(const sp_ctx *c, const std::vector<int> &batch)
 int strip_shift(const sp_ctx *c) { return c->cur_fuse ? 6 : (c->cur_kernel == 1 ? c->shift : 3); }
 int strip_width(const sp_ctx *c) { return c->cur_fuse ? 52 : (c->cur_kernel == 1 ? c->wout : 58); }
 
-// 0: one step per launch; 1: two steps per launch (k_step2); 2: two steps
-// per launch with two cells per lane (k_step2w: SH and P-SV, every grid of
-// the batch with even nz; single steps keep the k_step2 strips)
+// 0: one step per launch; 1: two steps per launch (k_step2, one cell per
+// lane); 2: two steps per launch with two cells per lane (SH and P-SV, every
+// grid of the batch with even nz; single steps keep the one-cell strips)
 int fuse_ok(const sp_ctx *c, const std::vector<int> &batch)
 {
     if (!c->fuse || batch_kernel(c, batch) != 1) return 0;
@@ -497,26 +496,26 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
         a.wout = 116;
         grid = dim3((unsigned)((c->n_items_w + 3) / 4));
         if (c->physics == SP_SH) {
-            if (ex) hipLaunchKernelGGL((k_step2w<SP_SH, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2w<SP_SH, false>), grid, block, 0, c->stream, a);
+            if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true, 2>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_SH, false, 2>), grid, block, 0, c->stream, a);
         } else {
-            if (ex) hipLaunchKernelGGL((k_step2w<SP_PSV, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2w<SP_PSV, false>), grid, block, 0, c->stream, a);
+            if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true, 2>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_PSV, false, 2>), grid, block, 0, c->stream, a);
         }
     } else if (nsteps == 2) {
         if (!c->cur_fuse) return fail(c, "internal: two-step launch without the two-step geometry");
         switch (c->physics) {
         case SP_SH:
-            if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_SH, false>), grid, block, 0, c->stream, a);
+            if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true, 1>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_SH, false, 1>), grid, block, 0, c->stream, a);
             break;
         case SP_PSV:
-            if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_PSV, false>), grid, block, 0, c->stream, a);
+            if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true, 1>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_PSV, false, 1>), grid, block, 0, c->stream, a);
             break;
         default:
-            if (ex) hipLaunchKernelGGL((k_step2<SP_SPIN, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_SPIN, false>), grid, block, 0, c->stream, a);
+            if (ex) hipLaunchKernelGGL((k_step2<SP_SPIN, true, 1>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_step2<SP_SPIN, false, 1>), grid, block, 0, c->stream, a);
             break;
         }
     } else if (c->cur_kernel == 1) {

@@ -67,7 +67,7 @@ struct StepArgs {
     int snap_v, snap_u;
     int adjoint;   // inject adjoint sources at the receivers, record nothing
     int shift;     // cells of a strip before its first output cell (>= 3)
-    int wout;      // output cells per strip (shift + wout <= 61; k_step2w: 6 + 116)
+    int wout;      // output cells per strip (shift + wout <= 61; two cells per lane: 6 + 116)
 };
 
 // z neighbours by whole-wavefront DPP rotations: lane L receives the value

@@ -54,7 +54,7 @@ def test_checkpointed_adjoint_equals_stored_snapshots(ctx_factory, physics, sae)
 
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV])
 def test_checkpointed_adjoint_with_two_cells_per_lane(ctx_factory, physics):
-    """The checkpointed forward and adjoint with SP_OPT_FUSE 2 (k_step2w for
+    """The checkpointed forward and adjoint with SP_OPT_FUSE 2 (two cells per lane for
     the two-step launches, including the recomputed segments) give the
     stored-snapshot gradient of one cell per lane bit for bit."""
     prob = _small(physics, 60, 130, 200, 2)

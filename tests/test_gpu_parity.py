@@ -407,7 +407,7 @@ def test_two_step_launches_equal_single_steps(ctx_factory, physics, chunk, order
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV])
 @pytest.mark.parametrize("chunk,order,sae", [(16, 0, 10), (4, 1, 5), (32, 3, 1)])
 def test_two_cells_per_lane_equals_one(ctx_factory, physics, chunk, order, sae):
-    """SP_OPT_FUSE 2 (k_step2w, two z cells per lane) gives the records,
+    """SP_OPT_FUSE 2 (k_step2 with two z cells per lane) gives the records,
     misfit and kernels of k_step2 bit for bit, and the records equal the
     oracle: even-nz ragged grids with receivers on every row and on the
     strip edges (z 114..117 and 0..3), sources on the grid corners."""
