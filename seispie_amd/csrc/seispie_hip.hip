@@ -10,13 +10,13 @@
 //     lane, with 8-byte accesses and 116 outputs per wavefront),
 //     k_step3<PHYS> (one step; the same march with buffer loads and stores)
 //     and k_step<PHYS> (plain one-step fallback for grids too large for
-//     32-bit buffer offsets).  In each,
-//     a wavefront owns a strip of 64 consecutive z cells
-//     (z is the contiguous axis, k = i*nz + j) and marches along x through
-//     a chunk of rows.  It keeps a 4-row register window of the old
-//     stresses and of the new velocities, takes z neighbours with
-//     wavefront (DPP) rotations, and writes new velocity, displacement and stress
-//     for the 58 interior lanes.  Stress divergence, source injection,
+//     32-bit buffer offsets).  In each, a wavefront owns a strip of 64
+//     (or 128) consecutive z cells (z is the contiguous axis,
+//     k = i*nz + j) and marches along x through a chunk of rows.  It keeps
+//     4-row register windows of the stresses and velocities, takes z
+//     neighbours with wavefront (DPP) rotations, and writes new velocity,
+//     displacement and stress for the cells inside its halo (3 cells a side
+//     for one step, 6 for two).  Stress divergence, source injection,
 //     sponge, velocity/displacement update, velocity gradient, stress
 //     update, receiver recording and adjoint snapshots are all fused, so a
 //     cell's state is read and written once per launch.  Stresses and
@@ -315,8 +315,9 @@ int batch_kernel(const sp_ctx *c, const std::vector<int> &batch)
     return 1;
 }
 
-// strip geometry of the prepared batch: the two-step kernel needs 6 halo
-// lanes (and the single-step kernel runs on the same strips in that mode)
+// strip geometry of the prepared batch's one-cell strips: the two-step
+// kernel needs 6 halo cells (and the single-step kernel runs on the same
+// strips in that mode); launches with two cells per lane use 6 + 116
 int strip_shift(const sp_ctx *c) { return c->cur_fuse ? 6 : (c->cur_kernel == 1 ? c->shift : 3); }
 int strip_width(const sp_ctx *c) { return c->cur_fuse ? 52 : (c->cur_kernel == 1 ? c->wout : 58); }
 
