@@ -333,6 +333,14 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     const int4 item2 = a.items[2 * wave + 1];
     const ShotDesc &d = a.shots[item.x];
     const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
+    // snapshot slots of the pair's two steps (solver.py:193-203): given, or
+    // in a replayed graph derived from the step
+    int snap_v = a.snap_v, snap_u = a.snap_u;
+    if (a.snap_every > 0) {
+        const int e = a.snap_every;
+        snap_v = (itx + 1) % e == 0 ? a.snap_count - (itx + 1) / e : -1;
+        snap_u = (itx + 2) % e == 0 ? a.snap_count - (itx + 2) / e : -1;
+    }
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - a.shift + W * L;  // the lane's first cell
     const int jc = min(max(j, 0), nz - W);   // the first cell it loads
@@ -503,16 +511,16 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 }
                 // adjoint snapshots of either step (solver.py:193-203, 236-242):
                 // [isa][u before the step | v after it][NV]
-                if (a.snap_v >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
+                if (snap_v >= 0) {
+                    const rsrc_t RP = make_rsrc(d.snap + (long long)snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
                         bstw<W>(RP, ov, f * NP4 + orow, cur.u[f]);
                         bstw<W>(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
                     }
                 }
-                if (a.snap_u >= 0) {
-                    const rsrc_t RP = make_rsrc(d.snap + (long long)a.snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
+                if (snap_u >= 0) {
+                    const rsrc_t RP = make_rsrc(d.snap + (long long)snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
                         bstw<W>(RP, ov, f * NP4 + orow, u1[f]);

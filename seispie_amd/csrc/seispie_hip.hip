@@ -108,6 +108,7 @@ struct sp_ctx {
     int exact = 1;             // step kernels follow the reference's float64 operations exactly
     int *d_counter = nullptr;  // step counter read by the kernels of a replayed graph
     const int *itp = nullptr;  // set while a graph is captured
+    int snap_every = 0, snap_count = 0;  // snapshots of the forward run being captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
     int graph = 0;
     long long snap_mb = 0;
@@ -477,6 +478,8 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     StepArgs a;
     a.snap_v = snap_v;
     a.snap_u = snap_u;
+    a.snap_every = c->itp ? c->snap_every : 0;
+    a.snap_count = c->snap_count;
     a.shots = desc ? desc : c->d_desc;
     a.items = c->d_items;
     a.nitems = nitems;
@@ -580,15 +583,20 @@ int clear_state(sp_ctx *c, const std::vector<int> &batch)
     return 0;
 }
 
-// Forward steps [0, nt) of a prepared batch without snapshots, as replays of
-// one captured graph: a block of `block` steps (their step launches and
-// receiver records, step numbers relative to a device counter) and a
-// one-thread kernel that advances the counter; the steps past the last
-// whole block run eagerly.  The block returns the stress/velocity set to
-// where it started (an even number of launches), so every replay starts
-// from the same set.  Needs a stream other than the null stream.
-int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
+// Forward steps [0, nblk * block) of a prepared batch as replays of one
+// captured graph: a block of `block` steps (their step launches and receiver
+// records, step numbers relative to a device counter) and a one-thread
+// kernel that advances the counter.  Snapshots (sae > 0) are taken by the
+// two-step launches themselves, at slots they derive from the counter, so a
+// run with snapshots needs two steps per launch.  The block returns the
+// stress/velocity set to where it started (an even number of launches), so
+// every replay starts from the same set.  Needs a stream other than the null
+// stream.  Returns 1 (nothing run) where a graph does not apply; *it_done is
+// the first step left to the caller.
+int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int sae, int nsa, int &set, int &it_done)
 {
+    it_done = 0;
+    if (sae > 0 && !c->cur_fuse) return 1;
     const int nt = c->nt;
     int block = 64;
     while (block > 4 && block > nt) block /= 2;
@@ -611,6 +619,8 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
     hipGraphExec_t exec = nullptr;
     HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->itp = c->d_counter;
+    c->snap_every = sae;
+    c->snap_count = nsa;
     int s = set, rc = 0, nl = 0;
     for (int it = 0; it < block && !rc;) {
         const int ns = steps_next(c, it, block, 0, 0);
@@ -624,6 +634,7 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
     }
     if (!rc) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_counter, block);
     c->itp = nullptr;
+    c->snap_every = c->snap_count = 0;
     const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -642,18 +653,8 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int &set)
     }
     HIPCHECK(c, hipGraphExecDestroy(exec));
     c->t_launches += (long long)nl * (nblk - 1);  // the capture counted one block
-    // the remaining steps, eagerly
-    for (int it = nblk * block; it < nt;) {
-        const int ns = steps_next(c, it, nt, 0, 0);
-        if (launch_step(c, nitems, it, -1, 0, s, ns)) return 2;
-        s ^= 1;
-        if (ns == 1)
-            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
-                               nullptr);
-        HIPCHECK(c, hipGetLastError());
-        it += ns;
-    }
     set = s;
+    it_done = nblk * block;
     return 0;
 }
 
@@ -673,17 +674,13 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     }
     const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256)), sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
     const bool split = c->cur_kernel == 1;  // snapshots and records as kernels of their own
-    int set = 0;
-    if (c->graph && sae == 0 && split && c->stream) {
-        const int rc = run_forward_graph(c, nitems, nb, maxrec, set);
-        if (rc == 0) {
-            c->cur_set = set;
-            c->t_bytes += step_alg_bytes(c, batch) * c->nt;
-            return 0;
-        }
-        if (rc != 1) return rc;
+    int set = 0, it0 = 0;
+    if (c->graph && split && c->stream) {
+        const int rc = run_forward_graph(c, nitems, nb, maxrec, sae, nsa, set, it0);
+        if (rc != 0 && rc != 1) return rc;
     }
-    for (int it = 0; it < c->nt;) {
+    // the steps no graph replay covered, eagerly
+    for (int it = it0; it < c->nt;) {
         const int ns = steps_next(c, it, c->nt, 0, 0);
         // snapshots after step it and (two-step launches) after step it+1,
         // stored in reverse order (solver.py:193-203)

@@ -478,6 +478,43 @@ def test_graph_replayed_forward_equals_eager(ctx_factory, physics, fuse):
 
 
 @pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
+@pytest.mark.parametrize("sae", [10, 5])
+def test_graph_replayed_gradient_equals_eager(ctx_factory, physics, fuse, sae):
+    """A gradient whose forward pass replays a captured graph (the two-step
+    launches take the snapshots, at slots derived from the device step
+    counter) equals the eager gradient bit for bit: records, misfits and
+    kernels.  nt = 150 leaves 22 steps past the last 64-step block, which
+    run eagerly; sae 10 puts every snapshot on the second step of a pair,
+    sae 5 on both."""
+    prob = _small(physics, nt=150, nshots=2) if fuse == 1 else _small(physics, 70, 130, 150, 2)
+    prob.sae = sae
+    print("case: graph gradient", physics, "fuse", fuse, "sae", sae, flush=True)
+    out, data = [], None
+    for graph in (0, 1):
+        ctx = ctx_factory()
+        ctx.set_option(_lib.OPT_FUSE, fuse)
+        ctx.set_option(_lib.OPT_GRAPH, graph)
+        ids = configs.load_problem(ctx, prob)
+        if data is None:
+            ctx.forward(0)
+            data = [(ctx.records(i) * np.float32(1.01)).astype(np.float32) for i in ids]
+        for i, d in zip(ids, data):
+            ctx.set_data(i, d)
+        norms = ctx.gradient(sae)
+        recs = [ctx.records(i) for i in ids]
+        kern = [ctx.kernel(g, w) for g in range(len(prob.grids)) for w in (_lib.K_LAM, _lib.K_MU, _lib.K_RHO, _lib.K_HESS)]
+        out.append((recs, norms, kern))
+    (r0, n0, k0), (r1, n1, k1) = out
+    assert np.all(n0 > 0)
+    for a, b in zip(r0, r1):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(n0, n1)
+    assert all(np.abs(a).max() > 0 for a in k0[1:])  # mu, rho, Hessian (SH has no lambda kernel)
+    for a, b in zip(k0, k1):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
 def test_stepwise_forward_and_wavefields(ctx_factory, physics, fuse):
     """A forward run in pieces records what one run records (bit for bit),
     and the velocities read back after step k equal the oracle's velocity
