@@ -199,7 +199,7 @@ def main():
     ap.add_argument("--autograd", action="store_true",
                     help="time loss.backward() through the torch.autograd.Function (checkpointed adjoint) instead")
     ap.add_argument("--graph", action="store_true",
-                    help="replay the forward runs (with their adjoint snapshots) from a captured hipGraph")
+                    help="replay the forward and adjoint runs from captured hipGraphs")
     ap.add_argument("--no-fuse", action="store_true",
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
     ap.add_argument("--fuse", type=int, default=-1,

@@ -60,9 +60,9 @@ typedef struct sp_ctx sp_ctx;
 /* tuning options (sp_set_option) */
 #define SP_OPT_CHUNK 1      /* rows per wavefront item in the step kernel */
 #define SP_OPT_ORDER 2      /* work order: 0 shot-outer (default), 1 shot-inner, 2 tile-major, 3 strip-group-major */
-#define SP_OPT_GRAPH 3      /* 1: forward runs replay blocks of time steps from a captured hipGraph
-                               (with adjoint snapshots when steps run two per launch); the adjoint
-                               launches eagerly */
+#define SP_OPT_GRAPH 3      /* 1: forward and adjoint runs replay blocks of time steps from captured
+                               hipGraphs (forward runs with adjoint snapshots need two steps per
+                               launch); checkpointed runs launch eagerly */
 #define SP_OPT_SNAP_MB 4    /* snapshot memory budget in MiB (0 = auto) */
 #define SP_OPT_KERNEL 5     /* step kernel: 1 buffer-addressed unrolled march (default), 0 plain */
 #define SP_OPT_STRIP 6      /* strip geometry (shift << 16) | width: output lanes [shift, shift + width) */

@@ -109,6 +109,7 @@ struct sp_ctx {
     int *d_counter = nullptr;  // step counter read by the kernels of a replayed graph
     const int *itp = nullptr;  // set while a graph is captured
     int snap_every = 0, snap_count = 0;  // snapshots of the forward run being captured
+    int inter_sae = 0, inter_nsa = 0;    // interactions of the adjoint run being captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
     int graph = 0;
     long long snap_mb = 0;
@@ -813,6 +814,9 @@ int launch_interact(sp_ctx *c, const InteractPlan &pl, int cur, int isae, double
         a.kern = G.kern;
         a.cur = cur;
         a.isae = isae;
+        a.itp = c->itp;
+        a.sae = c->inter_sae;
+        a.nsa = c->itp ? c->inter_nsa : 0x7fffffff;
         a.ndt = ndt;
         a.KX.c1 = 9.0 / (8.0 * (double)G.dx);
         a.KX.c2 = 1.0 / (24.0 * (double)G.dx);
@@ -833,6 +837,77 @@ int launch_interact(sp_ctx *c, const InteractPlan &pl, int cur, int isae, double
     return 0;
 }
 
+// Adjoint steps [1, 1 + nblk * block) of a prepared batch as replays of one
+// captured graph (SP_OPT_GRAPH).  From step 1 on the launches repeat every
+// sae steps (a kernel interaction after each launch that ends on a multiple
+// of sae), so a block of m sae steps serves every replay: the step kernels
+// take the step, and the interaction kernel its snapshot slot, from the
+// device counter that a one-thread kernel advances by `block`.  m makes the
+// block's launch count even, so each replay starts from the same velocity
+// set.  Returns 1 (nothing run) where a graph does not apply; *it_done is the
+// first step left to the caller.
+int run_adjoint_graph(sp_ctx *c, int nitems, const InteractPlan &pl, int sae, double ndt, int &set, int &it_done)
+{
+    it_done = 1;
+    const int nt = c->nt;
+    auto launches_in = [&](int steps) {
+        int n = 0;
+        for (int it = 1; it < 1 + steps;) {
+            it += steps_next(c, it, 1 + steps, sae, 0);
+            n++;
+        }
+        return n;
+    };
+    int m = std::max(1, 64 / sae);
+    if (launches_in(m * sae) % 2) m *= 2;
+    const int block = m * sae;
+    const int nblk = (nt - 1) / block;
+    if (nblk < 2) return 1;  // not worth a graph: the caller runs eagerly
+    if (!c->d_counter) HIPCHECK(c, hipMalloc(&c->d_counter, sizeof(int)));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->d_counter, 1LL, 0.f);  // 0.f = int 0
+    HIPCHECK(c, hipGetLastError());
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->itp = c->d_counter;
+    c->inter_sae = sae;
+    c->inter_nsa = nt / sae;
+    int s = set, rc = 0, nl = 0;
+    for (int it = 1; it < 1 + block && !rc;) {
+        const int ns = steps_next(c, it, 1 + block, sae, 0);
+        rc = launch_step(c, nitems, it, -1, 1, s, ns);
+        s ^= 1;
+        nl++;
+        const int last = it + ns - 1;
+        if (!rc && last % sae == 0) rc = launch_interact(c, pl, s, last / sae, ndt);
+        it += ns;
+    }
+    if (!rc) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_counter, block);
+    c->itp = nullptr;
+    c->inter_sae = c->inter_nsa = 0;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    HIPCHECK(c, ec);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(c, ei);
+    for (int b = 0; b < nblk; b++) {
+        const hipError_t el = hipGraphLaunch(exec, c->stream);
+        if (el != hipSuccess) {
+            (void)hipGraphExecDestroy(exec);
+            HIPCHECK(c, el);
+        }
+    }
+    HIPCHECK(c, hipGraphExecDestroy(exec));
+    c->t_launches += (long long)nl * (nblk - 1);  // the capture counted one block
+    set = s;
+    it_done = 1 + nblk * block;
+    return 0;
+}
+
 int run_adjoint_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
 {
     int nitems = 0;
@@ -842,13 +917,30 @@ int run_adjoint_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     if (interact_lists(c, batch, pl)) return 2;
     const double ndt = (double)sae * c->dt;
     int set = 0;
-    for (int it = 0; it < c->nt;) {
+    // one launch (and its interaction) from step it
+    auto step = [&](int it) {
         const int ns = steps_next(c, it, c->nt, sae, 0);
-        if (launch_step(c, nitems, it, -1, 1, set, ns)) return 2;
+        if (launch_step(c, nitems, it, -1, 1, set, ns)) return -1;
         set ^= 1;
         const int last = it + ns - 1;
         if (last % sae == 0)
-            if (launch_interact(c, pl, set, last / sae, ndt)) return 2;
+            if (launch_interact(c, pl, set, last / sae, ndt)) return -1;
+        return ns;
+    };
+    int it = 0;
+    if (c->graph && c->cur_kernel == 1 && c->stream && c->nt > 1) {
+        const int ns = step(0);
+        if (ns < 0) return 2;
+        it = ns;
+        if (it == 1) {
+            const int rc = run_adjoint_graph(c, nitems, pl, sae, ndt, set, it);
+            if (rc != 0 && rc != 1) return rc;
+        }
+    }
+    // the steps no graph replay covered, eagerly
+    while (it < c->nt) {
+        const int ns = step(it);
+        if (ns < 0) return 2;
         it += ns;
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;

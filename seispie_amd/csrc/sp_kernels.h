@@ -53,7 +53,11 @@ struct InteractArgs {
     long long Npad;
     float *kern;      // [SP_NKERN][Npad]
     int cur;          // velocity set holding the adjoint velocity
-    int isae;
+    int isae;         // snapshot slot
+    // replayed graphs: the slot is isae + (device step counter) / sae; a slot
+    // outside [0, nsa) writes nothing
+    const int *itp;
+    int sae, nsa;
     double ndt;
     DCo KX, KZ;  // derivative constants (the exact split form is used)
 };
@@ -86,6 +90,8 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
     const long long N = (long long)a.nx * a.nz;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
+    const int isae = a.itp ? a.isae + *(cint_p)a.itp / a.sae : a.isae;
+    if (isae < 0 || isae >= a.nsa) return;
     const int i = (int)(k / a.nz), j = (int)(k - (long long)i * a.nz);
     const int nx = a.nx, nz = a.nz;
     const long long NP = a.Npad;
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
         for (int s = 0; s < a.nlist; s++) {
             const ShotDesc &d = a.shots[a.list[s]];
             const float *ua = d.state + slot_u(P, 0) * NP;
-            const float *uf = d.snap + (long long)a.isae * n_snap(P) * NP;
+            const float *uf = d.snap + (long long)isae * n_snap(P) * NP;
             const float ax = dpx(ua, i, k, nx, nz, a.KX), fx = dpx(uf, i, k, nx, nz, a.KX);
             const float az = dpz(ua, j, k, nz, a.KZ), fz = dpz(uf, j, k, nz, a.KZ);
             km = (float)((double)km + (double)add_rn(mul_rn(ax, fx), mul_rn(az, fz)) * a.ndt);
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(256) void k_interact(InteractArgs a)
     for (int s = 0; s < a.nlist; s++) {
         const ShotDesc &d = a.shots[a.list[s]];
         const float *uxa = d.state + slot_u(P, 0) * NP, *uza = d.state + slot_u(P, 1) * NP;
-        const float *sn = d.snap + (long long)a.isae * n_snap(P) * NP;
+        const float *sn = d.snap + (long long)isae * n_snap(P) * NP;
         const float *uxf = sn, *uzf = sn + NP;
         const double axx = dpx(uxa, i, k, nx, nz, a.KX), axz = dpz(uxa, j, k, nz, a.KZ);
         const double azx = dpx(uza, i, k, nx, nz, a.KX), azz = dpz(uza, j, k, nz, a.KZ);

@@ -480,12 +480,15 @@ def test_graph_replayed_forward_equals_eager(ctx_factory, physics, fuse):
 @pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
 @pytest.mark.parametrize("sae", [10, 5])
 def test_graph_replayed_gradient_equals_eager(ctx_factory, physics, fuse, sae):
-    """A gradient whose forward pass replays a captured graph (the two-step
-    launches take the snapshots, at slots derived from the device step
-    counter) equals the eager gradient bit for bit: records, misfits and
-    kernels.  nt = 150 leaves 22 steps past the last 64-step block, which
-    run eagerly; sae 10 puts every snapshot on the second step of a pair,
-    sae 5 on both."""
+    """A gradient whose forward and adjoint runs replay captured graphs
+    equals the eager gradient bit for bit: records, misfits and kernels.  In
+    the forward graph the two-step launches take the snapshots at slots
+    derived from the device step counter; in the adjoint graph the
+    interaction kernel reads its snapshot slot from it.  nt = 150 leaves 22
+    (forward) and 29 (adjoint) steps past the last block, which run
+    eagerly; sae 10 puts every snapshot on the second step of a pair and
+    the adjoint's launches all in pairs, sae 5 puts snapshots on both steps
+    and single adjoint steps between the pairs."""
     prob = _small(physics, nt=150, nshots=2) if fuse == 1 else _small(physics, 70, 130, 150, 2)
     prob.sae = sae
     print("case: graph gradient", physics, "fuse", fuse, "sae", sae, flush=True)
