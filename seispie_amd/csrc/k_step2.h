@@ -336,10 +336,9 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     // snapshot slots of the pair's two steps (solver.py:193-203): given, or
     // in a replayed graph derived from the step
     int snap_v = a.snap_v, snap_u = a.snap_u;
-    if (a.snap_every > 0) {
-        const int e = a.snap_every;
-        snap_v = (itx + 1) % e == 0 ? a.snap_count - (itx + 1) / e : -1;
-        snap_u = (itx + 2) % e == 0 ? a.snap_count - (itx + 2) / e : -1;
+    if (a.snap_tab) {
+        snap_v = ((cint_p)a.snap_tab)[itx];
+        snap_u = ((cint_p)a.snap_tab)[itx + 1];
     }
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - a.shift + W * L;  // the lane's first cell

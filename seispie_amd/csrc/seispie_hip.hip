@@ -108,7 +108,9 @@ struct sp_ctx {
     int exact = 1;             // step kernels follow the reference's float64 operations exactly
     int *d_counter = nullptr;  // step counter read by the kernels of a replayed graph
     const int *itp = nullptr;  // set while a graph is captured
-    int snap_every = 0, snap_count = 0;  // snapshots of the forward run being captured
+    const int *snap_tab = nullptr;       // snapshot slots of the forward run being captured
+    int *d_snaptab = nullptr;            // [nt + 1]
+    size_t d_snaptab_cap = 0;
     int inter_sae = 0, inter_nsa = 0;    // interactions of the adjoint run being captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
     int graph = 0;
@@ -479,8 +481,7 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     StepArgs a;
     a.snap_v = snap_v;
     a.snap_u = snap_u;
-    a.snap_every = c->itp ? c->snap_every : 0;
-    a.snap_count = c->snap_count;
+    a.snap_tab = c->itp ? c->snap_tab : nullptr;
     a.shots = desc ? desc : c->d_desc;
     a.items = c->d_items;
     a.nitems = nitems;
@@ -615,13 +616,22 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int sae, int ns
     if (!c->d_counter) HIPCHECK(c, hipMalloc(&c->d_counter, sizeof(int)));
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->d_counter, 1LL, 0.f);  // 0.f = int 0
     HIPCHECK(c, hipGetLastError());
+    if (sae > 0) {
+        // the slot of the snapshot after each step (solver.py:193-203), -1: none
+        std::vector<int> tab(nt + 1, -1);
+        for (int it = 0; it < nt; it++)
+            if ((it + 1) % sae == 0) tab[it] = nsa - (it + 1) / sae;
+        if (ensure(c, c->d_snaptab, c->d_snaptab_cap, tab.size())) return 2;
+        HIPCHECK(c, hipMemcpyAsync(c->d_snaptab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice,
+                                   c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
     const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->itp = c->d_counter;
-    c->snap_every = sae;
-    c->snap_count = nsa;
+    c->snap_tab = sae > 0 ? c->d_snaptab : nullptr;
     int s = set, rc = 0, nl = 0;
     for (int it = 0; it < block && !rc;) {
         const int ns = steps_next(c, it, block, 0, 0);
@@ -635,7 +645,7 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int sae, int ns
     }
     if (!rc) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_counter, block);
     c->itp = nullptr;
-    c->snap_every = c->snap_count = 0;
+    c->snap_tab = nullptr;
     const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -1272,6 +1282,7 @@ int sp_destroy(sp_ctx *c)
     if (c->seg_pool) (void)hipFree(c->seg_pool);
     if (c->d_items) (void)hipFree(c->d_items);
     if (c->d_items_w) (void)hipFree(c->d_items_w);
+    if (c->d_snaptab) (void)hipFree(c->d_snaptab);
     if (c->d_list) (void)hipFree(c->d_list);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -65,10 +65,9 @@ struct StepArgs {
     // of its second step (snap_u), -1: none; the kernel writes both halves
     // of each (u before the step, v after it)
     int snap_v, snap_u;
-    // replayed graphs: snapshots every snap_every steps (0: none or given
-    // by snap_v / snap_u), slot snap_count - (step + 1) / snap_every, taken
-    // from the device step counter
-    int snap_every, snap_count;
+    // replayed graphs: per step of the run, the snapshot slot taken after
+    // it or -1 ([nt + 1]; null: snap_v / snap_u as given)
+    const int *snap_tab;
     int adjoint;   // inject adjoint sources at the receivers, record nothing
     int shift;     // cells of a strip before its first output cell (>= 3)
     int wout;      // output cells per strip (shift + wout <= 61; two cells per lane: 6 + 116)
