@@ -482,9 +482,9 @@ def test_graph_replayed_forward_equals_eager(ctx_factory, physics, fuse):
 def test_graph_replayed_gradient_equals_eager(ctx_factory, physics, fuse, sae):
     """A gradient whose forward and adjoint runs replay captured graphs
     equals the eager gradient bit for bit: records, misfits and kernels.  In
-    the forward graph the two-step launches take the snapshots at slots
-    derived from the device step counter; in the adjoint graph the
-    interaction kernel reads its snapshot slot from it.  nt = 150 leaves 22
+    the forward graph the two-step launches take the snapshots at the slots
+    a per-step table gives for the device step counter; in the adjoint graph
+    the interaction kernel derives its snapshot slot from the counter.  nt = 150 leaves 22
     (forward) and 29 (adjoint) steps past the last block, which run
     eagerly; sae 10 puts every snapshot on the second step of a pair and
     the adjoint's launches all in pairs, sae 5 puts snapshots on both steps
