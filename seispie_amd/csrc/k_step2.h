@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     const ShotDesc &d = a.shots[item.x];
     const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     // snapshot slots of the pair's two steps (solver.py:193-203): given, or
-    // in a replayed graph derived from the step
+    // in a replayed graph looked up for the step
     int snap_v = a.snap_v, snap_u = a.snap_u;
     if (a.snap_tab) {
         snap_v = ((cint_p)a.snap_tab)[itx];
