@@ -477,7 +477,8 @@ def test_graph_replayed_forward_equals_eager(ctx_factory, physics, fuse):
             ctx.gradient(prob.sae)
 
 
-@pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
+@pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2),
+                                          (_lib.PSV, 0)])
 @pytest.mark.parametrize("sae", [10, 5])
 def test_graph_replayed_gradient_equals_eager(ctx_factory, physics, fuse, sae):
     """A gradient whose forward and adjoint runs replay captured graphs
@@ -488,8 +489,10 @@ def test_graph_replayed_gradient_equals_eager(ctx_factory, physics, fuse, sae):
     (forward) and 29 (adjoint) steps past the last block, which run
     eagerly; sae 10 puts every snapshot on the second step of a pair and
     the adjoint's launches all in pairs, sae 5 puts snapshots on both steps
-    and single adjoint steps between the pairs."""
-    prob = _small(physics, nt=150, nshots=2) if fuse == 1 else _small(physics, 70, 130, 150, 2)
+    and single adjoint steps between the pairs.  With one step per launch
+    (fuse 0) the forward run stays eager and the adjoint graph replays
+    single steps."""
+    prob = _small(physics, nt=150, nshots=2) if fuse != 2 else _small(physics, 70, 130, 150, 2)
     prob.sae = sae
     print("case: graph gradient", physics, "fuse", fuse, "sae", sae, flush=True)
     out, data = [], None
