@@ -81,6 +81,29 @@ def test_checkpointed_adjoint_with_two_cells_per_lane(ctx_factory, physics):
             np.testing.assert_array_equal(x, y, err_msg="kernel %d interval %d" % (w, C))
 
 
+def test_checkpointed_adjoint_under_the_graph_option(ctx_factory):
+    """SP_OPT_GRAPH 1 leaves the checkpointed forward and adjoint eager: the
+    same records and kernels as without it."""
+    prob = _small(_lib.PSV, nt=200, nshots=2)
+    prob.sae = 10
+    data = _data_for(prob)
+    print("case: ckpt with the graph option", flush=True)
+    out = []
+    for graph in (0, 1):
+        ctx, ids = _ctx_with(ctx_factory, prob, data)
+        ctx.set_option(_lib.OPT_GRAPH, graph)
+        ctx.forward_ckpt(10)
+        recs = [ctx.records(i) for i in ids]
+        ctx.misfit()
+        ctx.adjoint_ckpt(prob.sae)
+        out.append((recs, [ctx.kernel(0, w) for w in (_lib.K_LAM, _lib.K_MU, _lib.K_RHO, _lib.K_HESS)]))
+    for a, b in zip(out[0][0], out[1][0]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b)
+
+
 def test_checkpoint_errors(ctx_factory):
     prob = _small(_lib.PSV, nt=40, nshots=1)
     ctx = ctx_factory()

@@ -453,6 +453,21 @@ def test_two_cells_per_lane_equals_one(ctx_factory, physics, chunk, order, sae):
     _check_forward(prob, r2, exact=True)
 
 
+@pytest.mark.parametrize("physics,nz", [(_lib.SPIN, 0), (_lib.SH, 61), (_lib.PSV, 61)])
+def test_two_cells_option_falls_back(ctx_factory, physics, nz):
+    """SP_OPT_FUSE 2 on a batch that cannot hold two cells per lane (spin,
+    or a grid with odd nz) runs one cell per lane: the same records as
+    SP_OPT_FUSE 1, the option itself unchanged."""
+    prob = _small(physics, nt=120, nshots=2) if physics == _lib.SPIN else _small(physics, 50, nz, 120, 2)
+    print("case: fuse 2 fallback", physics, flush=True)
+    _, one = _run_forward(ctx_factory, prob, fuse=1)
+    ctx, two = _run_forward(ctx_factory, prob, fuse=2)
+    assert ctx.get_option(_lib.OPT_FUSE) == 2
+    for a, b in zip(one, two):
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("physics,fuse", [(_lib.SH, 1), (_lib.PSV, 1), (_lib.SPIN, 1), (_lib.SH, 2), (_lib.PSV, 2)])
 def test_graph_replayed_forward_equals_eager(ctx_factory, physics, fuse):
     """Forward runs replayed from a captured graph (SP_OPT_GRAPH) record
