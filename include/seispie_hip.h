@@ -82,7 +82,10 @@ int sp_create(int device, sp_ctx **out);
 int sp_destroy(sp_ctx *ctx);
 const char *sp_last_error(const sp_ctx *ctx);
 /* Work of the context goes to its own non-blocking stream until
- * sp_set_stream names another hipStream_t (NULL: the null stream). */
+ * sp_set_stream names another hipStream_t (NULL: the null stream).  The
+ * new stream is ordered after the work already queued on the old one (an
+ * event recorded there and waited for on the new stream), so a switch
+ * needs no synchronisation by the caller. */
 int sp_set_stream(sp_ctx *ctx, void *hip_stream);
 int sp_sync(sp_ctx *ctx);
 int sp_set_option(sp_ctx *ctx, int key, long long value);

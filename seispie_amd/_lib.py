@@ -127,10 +127,16 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-def _devptr(t):
-    """Device pointer of a torch tensor (float32, contiguous, on a GPU)."""
+def _devptr(t, n, device):
+    """Device pointer of a torch tensor that must be float32, contiguous,
+    hold exactly n values and live on GPU `device`.  The C entry points take
+    no length, so this check is what keeps a copy inside the allocation."""
     if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch.float32":
         raise ValueError("expected a contiguous float32 tensor on the GPU")
+    if t.device.index != device:
+        raise ValueError("tensor is on %s, the context on cuda:%d" % (t.device, device))
+    if t.numel() != n:
+        raise ValueError("expected a tensor of %d values, got %d" % (n, t.numel()))
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -149,6 +155,7 @@ class Context:
             raise SeispieError("sp_create(device=%d) failed with code %d (no HIP device visible?)" % (device, rc))
         self._p = p
         self.L = L
+        self.device = int(device)
         self.physics = None
         self.nt = None
         self.grids = []   # (nx, nz)
@@ -212,7 +219,7 @@ class Context:
     def set_field(self, grid, field, values):
         nx, nz = self.grids[grid]
         if _is_tensor(values):
-            self._chk(self.L.sp_set_field(self._p, grid, field, _devptr(values), 1))
+            self._chk(self.L.sp_set_field(self._p, grid, field, _devptr(values, nx * nz, self.device), 1))
         else:
             a = _f32(values, nx * nz)
             self._chk(self.L.sp_set_field(self._p, grid, field, _ptr(a), 0))
@@ -220,7 +227,7 @@ class Context:
     def get_field(self, grid, field, out=None):
         nx, nz = self.grids[grid]
         if out is not None and _is_tensor(out):
-            self._chk(self.L.sp_get_field(self._p, grid, field, _devptr(out), 1))
+            self._chk(self.L.sp_get_field(self._p, grid, field, _devptr(out, nx * nz, self.device), 1))
             return out
         a = np.empty(nx * nz, dtype=np.float32)
         self._chk(self.L.sp_get_field(self._p, grid, field, _ptr(a), 0))
@@ -256,7 +263,7 @@ class Context:
         nc = {WF_V: N_REC[self.physics], WF_U: N_REC[self.physics],
               WF_S: {SH: 2, PSV: 3, SPIN: 6}[self.physics]}[kind]
         if out is not None and _is_tensor(out):
-            self._chk(self.L.sp_get_wavefield(self._p, shot, kind, _devptr(out), 1))
+            self._chk(self.L.sp_get_wavefield(self._p, shot, kind, _devptr(out, nc * nx * nz, self.device), 1))
             return out
         a = np.empty((nc, nx * nz), dtype=np.float32)
         self._chk(self.L.sp_get_wavefield(self._p, shot, kind, _ptr(a), 0))
@@ -265,7 +272,7 @@ class Context:
     def records(self, shot, out=None):
         nrec = self.shots[shot][2]
         if out is not None and _is_tensor(out):
-            self._chk(self.L.sp_get_records(self._p, shot, _devptr(out), 1))
+            self._chk(self.L.sp_get_records(self._p, shot, _devptr(out, N_REC[self.physics] * nrec * self.nt, self.device), 1))
             return out
         a = np.empty((N_REC[self.physics], nrec, self.nt), dtype=np.float32)
         self._chk(self.L.sp_get_records(self._p, shot, _ptr(a), 0))
@@ -274,7 +281,7 @@ class Context:
     def set_data(self, shot, data):
         nrec = self.shots[shot][2]
         if _is_tensor(data):
-            self._chk(self.L.sp_set_data(self._p, shot, _devptr(data), 1))
+            self._chk(self.L.sp_set_data(self._p, shot, _devptr(data, N_REC[self.physics] * nrec * self.nt, self.device), 1))
         else:
             a = _f32(data, N_REC[self.physics] * nrec * self.nt)
             self._chk(self.L.sp_set_data(self._p, shot, _ptr(a), 0))
@@ -296,16 +303,17 @@ class Context:
         """[n_mis, nrec, nt], time-reversed; numpy or a device tensor."""
         nrec = self.shots[shot][2]
         if _is_tensor(src):
-            self._chk(self.L.sp_set_adjoint_source(self._p, shot, _devptr(src), 1))
+            self._chk(self.L.sp_set_adjoint_source(self._p, shot, _devptr(src, N_MIS[self.physics] * nrec * self.nt, self.device), 1))
         else:
             a = _f32(src, N_MIS[self.physics] * nrec * self.nt)
             self._chk(self.L.sp_set_adjoint_source(self._p, shot, _ptr(a), 0))
 
     def set_model_vps(self, grid, vp, vs, rho):
+        nx, nz = self.grids[grid]
         if _is_tensor(vp):
-            self._chk(self.L.sp_set_model_vps(self._p, grid, _devptr(vp), _devptr(vs), _devptr(rho), 1))
+            p = [_devptr(x, nx * nz, self.device) for x in (vp, vs, rho)]
+            self._chk(self.L.sp_set_model_vps(self._p, grid, p[0], p[1], p[2], 1))
         else:
-            nx, nz = self.grids[grid]
             a = [_f32(x, nx * nz) for x in (vp, vs, rho)]
             self._chk(self.L.sp_set_model_vps(self._p, grid, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), 0))
 
@@ -320,7 +328,7 @@ class Context:
     def kernel(self, grid, which, out=None):
         nx, nz = self.grids[grid]
         if out is not None and _is_tensor(out):
-            self._chk(self.L.sp_get_kernel(self._p, grid, which, _devptr(out), 1))
+            self._chk(self.L.sp_get_kernel(self._p, grid, which, _devptr(out, nx * nz, self.device), 1))
             return out
         a = np.empty(nx * nz, dtype=np.float32)
         self._chk(self.L.sp_get_kernel(self._p, grid, which, _ptr(a), 0))
@@ -336,7 +344,7 @@ class Context:
         """Gaussian smoothing of a host array (returns a new array) or, for a
         torch tensor on the device, in place."""
         if _is_tensor(field):
-            self._chk(self.L.sp_smooth(self._p, _devptr(field), int(nx), int(nz), int(sigma), 1))
+            self._chk(self.L.sp_smooth(self._p, _devptr(field, int(nx) * int(nz), self.device), int(nx), int(nz), int(sigma), 1))
             return field
         a = _f32(field, nx * nz).copy()
         self._chk(self.L.sp_smooth(self._p, _ptr(a), int(nx), int(nz), int(sigma), 0))

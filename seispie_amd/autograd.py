@@ -182,6 +182,7 @@ class SeismicRecords(torch.autograd.Function):
         ctx.sim = sim
         ctx.gen = gen
         ctx.shape = vp.shape
+        ctx.like = [(t.device, t.dtype) for t in (vp, vs, rho)]
         ctx.save_for_backward(*sim._last)
         return out
 
@@ -200,7 +201,11 @@ class SeismicRecords(torch.autograd.Function):
         gvs = 2.0 * rho64 * vs64 * (km - 2.0 * kl_eff)
         grho = torch.where(live, vp64 * vp64 - 2.0 * vs64 * vs64, torch.zeros_like(vp64)) * kl + vs64 * vs64 * km + kr
         shape = ctx.shape
-        return (gvp.float().reshape(shape), gvs.float().reshape(shape), grho.float().reshape(shape), None)
+        g = [x.float().reshape(shape) for x in (gvp, gvs, grho)]
+        # each gradient on its input's device and in its dtype (inputs may be
+        # host or float64 tensors; the solver itself runs on the GPU in float32)
+        g = [x.to(device=d, dtype=t) for x, (d, t) in zip(g, ctx.like)]
+        return (g[0], g[1], g[2], None)
 
 
 class SpinRecords(torch.autograd.Function):
@@ -213,6 +218,7 @@ class SpinRecords(torch.autograd.Function):
         ctx.sim = sim
         ctx.gen = gen
         ctx.shape = fields[0].shape
+        ctx.like = [(t.device, t.dtype) for t in fields]
         ctx.save_for_backward(*sim._last)
         return out
 
@@ -230,7 +236,8 @@ class SpinRecords(torch.autograd.Function):
         g = [(k.double() * inv).float().reshape(ctx.shape) for k in (kl, km, knu, kj)]
         gc = (kc.double() * inv).float().reshape(ctx.shape)
         gr = (kr.double() * inv).float().reshape(ctx.shape)
-        return (None, g[0], g[1], g[2], g[3], torch.zeros_like(gc), gc, gc.clone(), gr)
+        out = [g[0], g[1], g[2], g[3], torch.zeros_like(gc), gc, gc.clone(), gr]
+        return (None,) + tuple(x.to(device=d, dtype=t) for x, (d, t) in zip(out, ctx.like))
 
 
 def waveform_loss(records, data, dt):

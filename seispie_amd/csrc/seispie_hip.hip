@@ -138,6 +138,7 @@ struct sp_ctx {
     size_t d_list_cap = 0;
     // timing of the last run
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t ev_switch = nullptr;  // orders a new stream after the old one (sp_set_stream)
     long long t_launches = 0;
     double t_step_ms = 0, t_total_ms = 0, t_bytes = 0;
 };
@@ -1287,6 +1288,7 @@ int sp_destroy(sp_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev_switch) (void)hipEventDestroy(c->ev_switch);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return 0;
@@ -1297,7 +1299,15 @@ const char *sp_last_error(const sp_ctx *c) { return c ? c->err.c_str() : "null c
 int sp_set_stream(sp_ctx *c, void *stream)
 {
     if (!c) return 3;
-    c->stream = (hipStream_t)stream;  // NULL is the device's null stream, as in HIP
+    hipStream_t next = (hipStream_t)stream;  // NULL is the device's null stream, as in HIP
+    if (next == c->stream) return 0;
+    // work already queued on the old stream (fills, sponge factors, a run)
+    // completes before anything the new stream receives from now on
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (!c->ev_switch) HIPCHECK(c, hipEventCreateWithFlags(&c->ev_switch, hipEventDisableTiming));
+    HIPCHECK(c, hipEventRecord(c->ev_switch, c->stream));
+    HIPCHECK(c, hipStreamWaitEvent(next, c->ev_switch, 0));
+    c->stream = next;
     return 0;
 }
 

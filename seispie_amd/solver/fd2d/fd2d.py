@@ -374,7 +374,13 @@ class fd2d(base):
                 norms = ctx.misfit()
             misfit += float(norms.sum())
             if adj:
-                ctx.smooth_kernel(self._grid, _lib.K_MU, self.sigma)
+                if physics != _lib.SH:
+                    # vp / vs / rho' kernels from the raw lam / mu / rho
+                    # kernels and the model of this gradient, before k_mu is
+                    # smoothed in place as solver.py:395-396 does
+                    ctx.kernels_vps(self._grid)
+                if self.sigma > 0:
+                    ctx.smooth_kernel(self._grid, _lib.K_MU, self.sigma)
         misfit = np.float32(misfit)
         print('  misfit = %.2f' % misfit)
         if adj:
@@ -391,23 +397,25 @@ class fd2d(base):
         return kernel.copy_to_host(), misfit, model.copy_to_host()
 
     def compute_kernels(self, smooth=True):
-        """Every kernel of the last gradient (P-SV / spin: lam, mu, rho, vp,
-        vs, rho at fixed vp/vs; spin adds nu, j, muc), host float32; k_mu is
-        returned as run_kernel left it."""
+        """Every kernel of the last gradient as host float32 arrays (P-SV /
+        spin: lam, mu, rho, vp, vs, rho at fixed vp/vs; spin adds nu, j,
+        muc).  'mu' is k_mu as run_kernel left it (smoothed once); the
+        others are derived from the unsmoothed kernels and each smoothed
+        once into a new array, so repeated calls return the same values."""
         out = {}
         for physics, ctx in self._contexts():
             if physics == _lib.SH:
                 out['mu'] = ctx.kernel(self._grid, _lib.K_MU)
                 continue
-            ctx.kernels_vps(self._grid)
             names = {'lam': _lib.K_LAM, 'mu': _lib.K_MU, 'rho': _lib.K_RHO, 'vp': _lib.K_VP,
                      'vs': _lib.K_VS, 'rhop': _lib.K_RHOP}
             if physics == _lib.SPIN:
                 names.update(nu=_lib.K_NU, j=_lib.K_J, muc=_lib.K_MUC)
             for nm, w in names.items():
+                k = ctx.kernel(self._grid, w)
                 if smooth and nm != 'mu' and self.sigma > 0:
-                    ctx.smooth_kernel(self._grid, w, self.sigma)
-                out[nm] = ctx.kernel(self._grid, w)
+                    k = ctx.smooth(k, self.nx, self.nz, self.sigma)
+                out[nm] = k
         return out
 
     def update_model(self, mu):

@@ -278,3 +278,27 @@ def test_autograd_backward_after_another_forward():
     rb.square().sum().backward()
     assert all(torch.isfinite(t.grad).all() for t in b)
     sim.close()
+
+
+@pytest.mark.parametrize("physics", [_lib.PSV, _lib.SPIN])
+def test_autograd_accepts_host_and_float64_leaves(physics):
+    """Model leaves on the host (float64 here) get gradients of their own
+    device and dtype, equal to the float32 GPU leaves' gradients."""
+    import torch
+    prob = _small(physics, nt=100, nshots=1)
+    sim = _sim_from(prob, 5, interval=10)
+    g = prob.grids[0]
+    gpu = [torch.tensor(f, device=sim.device, requires_grad=True) for f in g.fields]
+    host = [torch.tensor(f.astype(np.float64), requires_grad=True) for f in g.fields]
+    w = None
+    for leaves in (gpu, host):
+        rec = sim(*leaves)
+        if w is None:
+            w = torch.randn(rec.shape, generator=torch.Generator().manual_seed(3)).to(rec.device)
+            w[:, sim.nmis:] = 0
+        (rec * w).sum().backward()
+    for a, b in zip(gpu, host):
+        assert b.grad.device.type == "cpu" and b.grad.dtype == torch.float64
+        torch.testing.assert_close(b.grad, a.grad.cpu().double(), rtol=0, atol=0)
+    assert any(float(b.grad.abs().max()) > 0 for b in host)
+    sim.close()

@@ -69,7 +69,7 @@ def _write_model(d, nx, nz, h, vp, vs, rho):
             np.asarray(v, np.float32).tofile(f)
 
 
-def make_examples(root, nx=60, nz=50, h=10.0, nt=300, dt=1e-3):
+def make_examples(root, nx=60, nz=50, h=10.0, nt=300, dt=1e-3, psv=False):
     fw = os.path.join(root, "forward")
     ad = os.path.join(root, "adjoint")
     os.makedirs(fw)
@@ -86,7 +86,10 @@ def make_examples(root, nx=60, nz=50, h=10.0, nt=300, dt=1e-3):
     with open(os.path.join(fw, "stations.dat"), "w") as f:
         for i in range(14, 46, 4):
             f.write("%g %g\n" % (i * h, 6 * h))
-    open(os.path.join(fw, "config.ini"), "w").write(FORWARD_INI.format(nt=nt, dt=dt))
+    ini = FORWARD_INI.format(nt=nt, dt=dt)
+    if psv:
+        ini = ini.replace("sh = yes\npsv = no", "sh = no\npsv = yes")
+    open(os.path.join(fw, "config.ini"), "w").write(ini)
     open(os.path.join(ad, "config.ini"), "w").write(ADJOINT_INI)
     return fw, ad
 
@@ -149,3 +152,51 @@ def test_forward_and_adjoint_workflows_match_the_oracle(tmp_path):
     mis, K = O.run_kernel(sim0, stf, data, 10, adj=True, sigma=3)
     assert np.abs(K["mu"]).max() > 0
     assert np.linalg.norm(kmu - K["mu"]) <= 1e-4 * np.linalg.norm(K["mu"])
+
+
+@pytest.mark.gpu
+def test_psv_compute_kernels_are_smoothed_once_and_repeatable(tmp_path):
+    """After a P-SV compute_gradient, compute_kernels() returns k_mu as the
+    gradient left it (smoothed once) and the lam, rho, vp, vs, rho' kernels
+    derived from the unsmoothed kernels and the gradient's model, each
+    smoothed once; a second call returns the same arrays."""
+    from oracle import oracle as O
+    fw, ad = make_examples(str(tmp_path), psv=True)
+    print("case: sprun P-SV adjoint + compute_kernels", flush=True)
+    config, wd = sprun.parse_config(ad, "config.ini")
+    wf = sprun.setup(config, wd)
+    s = wf.solver
+    k1 = s.compute_kernels()
+    k2 = s.compute_kernels()
+    for name in ("lam", "mu", "rho", "vp", "vs", "rhop"):
+        np.testing.assert_array_equal(k1[name], k2[name], err_msg=name)
+
+    def read(d, k):
+        return np.fromfile(os.path.join(fw, d, "proc000000_%s.bin" % k), np.float32)[1:]
+
+    bound = O.set_bound(s.nx, s.nz, 12, 0.015, 1, 1, 1, 0)
+    stf = O.make_stf(np.loadtxt(os.path.join(fw, "sources.dat"), ndmin=2), s.nt, s.dt)
+    lam_t, mu_t = O.vps2lm(read("model_true", "vp"), read("model_true", "vs"), read("model_true", "rho"))
+    sim = O.Sim(O.PSV, s.nx, s.nz, s.dx, s.dz, s.dt, s.nt,
+                {"rho": read("model_true", "rho"), "lam": lam_t, "mu": mu_t}, bound, s.src_id, s.rec_id, 0)
+    data = []
+    for i in range(2):
+        sim.isrc = i
+        obs = sim.forward(stf)[0]
+        data.append({"x": obs["x"], "z": obs["z"]})
+    rho0 = read("model_init", "rho")
+    lam0, mu0 = O.vps2lm(read("model_init", "vp"), read("model_init", "vs"), rho0)
+    sim0 = O.Sim(O.PSV, s.nx, s.nz, s.dx, s.dz, s.dt, s.nt, {"rho": rho0, "lam": lam0, "mu": mu0}, bound,
+                 s.src_id, s.rec_id, 0)
+    _, K = O.run_kernel(sim0, stf, data, 10, adj=True, sigma=0)
+    r, l, m = (x.astype(np.float64) for x in (rho0, lam0, mu0))
+    vp, vs = np.sqrt((l + 2 * m) / r), np.sqrt(m / r)
+    kl, km, kr = (K[k].astype(np.float64) for k in ("lam", "mu", "rho"))
+    want = {"lam": kl, "mu": km, "rho": kr, "vp": 2 * r * vp * kl, "vs": 2 * r * vs * (km - 2 * kl),
+            "rhop": (vp * vp - 2 * vs * vs) * kl + vs * vs * km + kr}
+    for name, w in want.items():
+        w = O.smooth(w.astype(np.float32), s.nx, s.nz, 3)
+        assert np.abs(w).max() > 0
+        err = np.linalg.norm(k1[name] - w) / np.linalg.norm(w)
+        print("%s kernel rel L2 %.2e" % (name, err))
+        assert err <= 1e-4, name
