@@ -90,21 +90,84 @@ def observed_data(true, chunk=0, order=0, fast=False):
     return data
 
 
+def host_cpu():
+    """This host's CPUs: nproc, the CPUs this process may run on (affinity
+    and the cgroup's CPU quota), and the lscpu model name."""
+    import math
+    import subprocess
+    info = {"nproc": os.cpu_count() or 1}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = info["nproc"]
+    usable = info["affinity"]
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_cpus"] = round(int(quota) / int(period), 2)
+            usable = min(usable, max(1, math.floor(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    info["usable"] = usable
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), None)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            model = next(ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name"))
+        except (OSError, StopIteration):
+            model = "unknown"
+    info["model"] = model
+    return info
+
+
+def baseline_build():
+    """The CPU baseline build: oracle.c with gcc -O3 -march=native -fopenmp,
+    compiled for this host into a temporary directory (a few seconds); the
+    prebuilt x86-64-v3 build if that fails."""
+    import subprocess
+    import tempfile
+    out = tempfile.mkdtemp(prefix="sp_baseline_")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "baseline", "BASELINE_OUT=" + out],
+                       check=True, capture_output=True, timeout=300)
+        return os.path.join(out, "libbaseline_native.so"), "gcc -O3 -march=native -fopenmp, built on this host"
+    except (OSError, subprocess.SubprocessError):
+        path = os.path.join(ROOT, "oracle", "_build", "libbaseline_v3.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "portable"], check=True)
+        return path, "gcc -O3 -march=x86-64-v3 -fopenmp (prebuilt; the native build failed)"
+
+
 def cpu_baseline(prob, warm=20, nsteps=200):
-    """Oracle (port of the reference kernels) on the host, one shot."""
+    """The reference's unfused kernel sequence (oracle.c, solver.py:205-234
+    and :290-321) on this host's cores, one shot of the workload's first
+    grid: a forward window and an adjoint window of nsteps steps after warm
+    steps each, on every usable core and on one core.  Gcell-upd/s counts
+    both windows (P = 2), as the GPU figure does."""
     from oracle import oracle as O
     from tests.helpers import oracle_sim, oracle_stf
     g = prob.grids[0]
     shot = g.shots[0]
     sim = oracle_sim(prob, g, shot)
     stf = oracle_stf(prob, shot)
-    ncores = min(os.cpu_count() or 1, 16)
+    path, how = baseline_build()
+    L = O.baseline_lib(path)
+    cpu = host_cpu()
+    # adjoint sources: the shot's source time function at every receiver
+    nrec = shot.rec_k.size
+    ad = [np.tile(np.asarray(c).reshape(-1, prob.nt)[:1], (nrec, 1)).reshape(-1) for c in stf]
     out = {}
-    for nth in (ncores, 1):
-        steps = nsteps if nth > 1 else max(nsteps // 8, 10)
-        sec = sim.cpu_bench(stf, warm if nth > 1 else 2, steps, nth)
-        out[nth] = (g.nx * g.nz * steps / sec / 1e9, steps)
-    return ncores, out
+    for nth in sorted({cpu["usable"], 1}, reverse=True):
+        tf = sim.cpu_bench(stf, warm, nsteps, nth, L)
+        ta = sim.cpu_bench_adjoint(ad, prob.sae, warm, nsteps, nth, L)
+        cells = g.nx * g.nz * nsteps
+        out[nth] = {"fwd_adj": 2 * cells / (tf + ta) / 1e9, "fwd": cells / tf / 1e9, "adj": cells / ta / 1e9,
+                    "seconds": tf + ta}
+    return cpu, how, out
 
 
 def autograd_bench(true, init, args):
@@ -182,6 +245,106 @@ def fwi_time_to_misfit(niter):
     return out
 
 
+CONFIG_BLOCK = ("1", "2", "3b", "4", "5")
+SMOOTH_SIGMA = 5  # smooth of the reference's examples/adjoint config
+
+
+def _context(args):
+    from seispie_amd import _lib
+    ctx = _lib.Context(0)
+    if args.no_fuse:
+        ctx.set_option(_lib.OPT_FUSE, 0)
+    if args.fuse >= 0:
+        ctx.set_option(_lib.OPT_FUSE, args.fuse)
+    if args.fast:
+        ctx.set_option(_lib.OPT_EXACT, 0)
+    if args.graph:
+        ctx.set_option(_lib.OPT_GRAPH, 1)
+    if args.chunk:
+        ctx.set_option(_lib.OPT_CHUNK, args.chunk)
+    ctx.set_option(_lib.OPT_ORDER, args.order)
+    return ctx
+
+
+def forward_only(true, init, args):
+    """Forward passes only (the step kernel), for counter runs."""
+    from seispie_amd import configs
+    ctx = _context(args)
+    configs.load_problem(ctx, init)
+    for _ in range(args.warmup + args.steps):
+        ctx.forward(0)
+    ctx.sync()
+    t = ctx.timing()
+    print(json.dumps({"forward_launch_ms": t["total_ms"] / t["step_launches"],
+                      "alg_bytes_per_launch": t["alg_bytes"] / t["step_launches"],
+                      "steps_per_launch": round(t["alg_bytes"] / t["step_launches"] / init.alg_bytes_per_step(), 3),
+                      "workload": init.name}))
+    ctx.close()
+    return 0
+
+
+def time_gradients(true, init, steps, warmup, args, quiet=False):
+    """Observed data from the true model, then `steps` timed gradient passes
+    (compute_gradient's work: forward with snapshots, misfit, adjoint with
+    kernel accumulation, k_mu smoothing) at the initial model after
+    `warmup` untimed ones; then one forward pass alone, timed per launch
+    with HIP events, for the dominant step kernel's roofline."""
+    from seispie_amd import _lib, configs
+    say = (lambda m: None) if quiet else (lambda m: print(m, file=sys.stderr, flush=True))
+    t0 = time.time()
+    data = observed_data(true, args.chunk, args.order, args.fast)
+    say("# observed data for %s in %.1fs" % (true.name, time.time() - t0))
+    ctx = _context(args)
+    ids = configs.load_problem(ctx, init)
+    for i, d in zip(ids, data):
+        ctx.set_data(i, d)
+    del data
+    sae = init.sae
+    tuned = None
+    if not args.chunk:
+        ch, od = ctx.autotune(20)
+        tuned = {"chunk": ch, "order": od, "steps_per_launch_option": ctx.get_option(_lib.OPT_FUSE)}
+        say("# autotuned %s" % tuned)
+
+    def gradient():
+        norms = ctx.gradient(sae)
+        for gi in range(len(init.grids)):
+            ctx.smooth_kernel(gi, _lib.K_MU, SMOOTH_SIGMA)
+        return norms
+
+    for w in range(warmup):
+        gradient()
+        ctx.sync()
+        say("# warmup %d done" % w)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        norms = gradient()
+        say("# step %d misfit %.6e" % (s, norms.sum()))
+    ctx.sync()
+    wall = time.perf_counter() - t0
+
+    ctx.forward(0)
+    ctx.sync()
+    ft = ctx.timing()
+    launch_ms = ft["total_ms"] / ft["step_launches"]
+    bytes_per_launch = ft["alg_bytes"] / ft["step_launches"]
+    steps_per_launch = bytes_per_launch / init.alg_bytes_per_step()
+    wide = ctx.get_option(_lib.OPT_FUSE) == 2 and init.physics != _lib.SPIN and all(g.nz % 2 == 0 for g in init.grids)
+    if steps_per_launch > 1.5:
+        kname = ("k_step2<%s, W=2> (two time steps per launch, two cells per lane)" if wide
+                 else "k_step2<%s, W=1> (two time steps per launch)")
+    else:
+        kname = "k_step3<%s>"
+    kname = kname % {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]
+    ctx.close()
+    cells = init.cell_updates(passes=2)
+    return {"wall": wall, "value": cells * steps / wall / 1e9, "cells": cells, "launch_ms": launch_ms,
+            "bytes_per_launch": bytes_per_launch, "achieved": bytes_per_launch / (launch_ms * 1e-3),
+            "steps_per_launch": steps_per_launch, "wide": wide, "kname": kname, "tuned": tuned,
+            "pass_frac": round(init.alg_bytes_per_step() * init.nt * 2 * steps / wall / HBM_PEAK, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +358,8 @@ def main():
     ap.add_argument("--forward-only", action="store_true",
                     help="time forward passes (the step kernel) instead of gradients; for counter runs")
     ap.add_argument("--no-fwi", action="store_true", help="skip the time-to-misfit measurement")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the per-configuration block (configs 1, 2, 3b, 4, 5: three gradients each)")
     ap.add_argument("--fwi-iters", type=int, default=20)
     ap.add_argument("--autograd", action="store_true",
                     help="time loss.backward() through the torch.autograd.Function (checkpointed adjoint) instead")
@@ -235,93 +400,54 @@ def main():
                     s.stf = np.ascontiguousarray(s.stf[..., :args.nt])
     if args.autograd:
         return autograd_bench(true, init, args)
-    t0 = time.time()
-    data = observed_data(true, args.chunk, args.order, args.fast)
-    print("# observed data for %s in %.1fs" % (true.name, time.time() - t0), file=sys.stderr, flush=True)
-
-    ctx = _lib.Context(0)
-    if args.no_fuse:
-        ctx.set_option(_lib.OPT_FUSE, 0)
-    if args.fuse >= 0:
-        ctx.set_option(_lib.OPT_FUSE, args.fuse)
-    if args.fast:
-        ctx.set_option(_lib.OPT_EXACT, 0)
-    if args.graph:
-        ctx.set_option(_lib.OPT_GRAPH, 1)
-    if args.chunk:
-        ctx.set_option(_lib.OPT_CHUNK, args.chunk)
-    ctx.set_option(_lib.OPT_ORDER, args.order)
-    ids = configs.load_problem(ctx, init)
-    for i, d in zip(ids, data):
-        ctx.set_data(i, d)
-    del data
-    sae = init.sae
     if args.forward_only:
-        if args.chunk:
-            ctx.set_option(_lib.OPT_CHUNK, args.chunk)
-            ctx.set_option(_lib.OPT_ORDER, args.order)
-        for _ in range(args.warmup + args.steps):
-            ctx.forward(0)
-        ctx.sync()
-        t = ctx.timing()
-        print(json.dumps({"forward_launch_ms": t["total_ms"] / t["step_launches"],
-                          "alg_bytes_per_launch": t["alg_bytes"] / t["step_launches"],
-                          "steps_per_launch": round(t["alg_bytes"] / t["step_launches"] / init.alg_bytes_per_step(), 3),
-                          "workload": init.name}))
-        return 0
-    if not args.chunk:
-        tuned = ctx.autotune(20)
-        print("# autotuned chunk=%d order=%d steps-per-launch option %d" % (tuned + (ctx.get_option(_lib.OPT_FUSE),)),
-              file=sys.stderr, flush=True)
+        return forward_only(true, init, args)
+    r = time_gradients(true, init, args.steps, args.warmup, args)
+    wall, value, cells, sae = r["wall"], r["value"], r["cells"], init.sae
 
-    for w in range(args.warmup):
-        ctx.gradient(sae)
-        ctx.sync()
-        print("# warmup %d done" % w, file=sys.stderr, flush=True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        norms = ctx.gradient(sae)
-        print("# step %d misfit %.6e" % (s, norms.sum()), file=sys.stderr, flush=True)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    tim = ctx.timing()
-
-    # dominant kernel alone: a forward pass without snapshots
-    ctx.forward(0)
-    ctx.sync()
-    ft = ctx.timing()
-    launch_ms = ft["total_ms"] / ft["step_launches"]
-    bytes_per_launch = ft["alg_bytes"] / ft["step_launches"]
-    achieved = bytes_per_launch / (launch_ms * 1e-3)
-    steps_per_launch = bytes_per_launch / init.alg_bytes_per_step()
-    wide = ctx.get_option(_lib.OPT_FUSE) == 2 and init.physics != _lib.SPIN and all(g.nz % 2 == 0 for g in init.grids)
-    if steps_per_launch > 1.5:
-        kname = ("k_step2<%s, W=2> (two time steps per launch, two cells per lane)" if wide
-                 else "k_step2<%s, W=1> (two time steps per launch)")
-    else:
-        kname = "k_step3<%s>"
-    kname = kname % {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]
-
-    cells = init.cell_updates(passes=2)
-    value = cells * args.steps / wall / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tf):
         t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0]
-                                    + (("_wide" if wide else "_fused") if steps_per_launch > 1.5 else "")
+                                    + (("_wide" if r["wide"] else "_fused") if r["steps_per_launch"] > 1.5 else "")
                                     + ("_fast" if args.fast else ""))
         if t:
             traffic = t["hbm_bytes_per_launch"]
 
+    others = None
+    if not args.no_configs and args.config == "3a" and not args.nt:
+        others = []
+        for name in CONFIG_BLOCK:
+            t0 = time.time()
+            try:
+                ct, ci = build_problem(name)
+                c = time_gradients(ct, ci, 3, 1, args, quiet=True)
+                others.append({"config": name, "workload": ci.name + " forward+adjoint gradient",
+                               "gcell_upd_s": round(c["value"], 3), "ms_per_gradient": round(c["wall"] / 3 * 1e3, 2),
+                               "gradients_timed": 3, "shots": ci.nshots, "nt": ci.nt,
+                               "grids": [[g.nx, g.nz] for g in ci.grids],
+                               "hbm_roofline_frac_of_whole_pass": c["pass_frac"],
+                               "tuned": c["tuned"],
+                               "dominant_kernel": {"kernel": c["kname"], "launch_ms": round(c["launch_ms"], 4),
+                                                   "achieved_GBps": round(c["achieved"] / 1e9, 1),
+                                                   "frac": round(c["achieved"] / HBM_PEAK, 4)}})
+            except Exception as e:  # noqa: BLE001 -- reported in the line, not hidden
+                others.append({"config": name, "error": "%s: %s" % (type(e).__name__, e)})
+            print("# config %s done in %.1fs" % (name, time.time() - t0), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
-        ncores, cb = cpu_baseline(init)
+        host, how, cb = cpu_baseline(init)
+        n = host["usable"]
         g = init.grids[0]
-        cpu = {"value": round(cb[ncores][0], 4), "unit": "Gcell-upd/s", "cores": ncores, "kind": "port",
-               "sample": "oracle (unfused reference kernel order, float32 storage / float64 math) on %dx%d, "
-                         "one shot, %d forward steps after 20 warm-up; 1 core: %.4f Gcell-upd/s"
-                         % (g.nx, g.nz, cb[ncores][1], cb[1][0])}
+        cpu = {"value": round(cb[n]["fwd_adj"], 4), "unit": "Gcell-upd/s", "cores": n, "kind": "port",
+               "sample": "oracle.c (the reference's unfused kernel sequence, float32 storage / float64 math; %s) "
+                         "on %dx%d, one shot: 200 forward steps and 200 adjoint steps (interaction every %d), "
+                         "each after 20 warm-up steps, on %d threads; forward %.4f, adjoint %.4f Gcell-upd/s; "
+                         "1 core: %.4f forward+adjoint" % (how, g.nx, g.nz, init.sae, n, cb[n]["fwd"], cb[n]["adj"],
+                                                           cb[1]["fwd_adj"]),
+               "one_core": round(cb[1]["fwd_adj"], 5), "nproc": host["nproc"], "affinity": host["affinity"],
+               "cgroup_cpus": host.get("cgroup_cpus"), "cpu_model": host["model"]}
 
     ttm = None
     if not args.no_fwi and not args.forward_only:
@@ -344,17 +470,19 @@ def main():
                    "grids": [[g.nx, g.nz] for g in init.grids], "shots": init.nshots, "nt": init.nt,
                    "adjoint_interval": sae, "cell_updates_per_step": cells,
                    "arithmetic": "fast" if args.fast else "exact (the reference's float64 operation order)",
-                   "hbm_roofline_frac_of_whole_pass": round(init.alg_bytes_per_step() * init.nt * 2 * args.steps
-                                                            / wall / HBM_PEAK, 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel": kname, "launch_ms": round(launch_ms, 4),
-                     "alg_bytes_per_launch": bytes_per_launch, "steps_per_launch": round(steps_per_launch, 3)},
+                   "step": "forward with snapshots, misfit and adjoint sources, adjoint with kernel "
+                           "accumulation, smoothing of k_mu (sigma %d), as compute_gradient" % SMOOTH_SIGMA,
+                   "tuned": r["tuned"],
+                   "hbm_roofline_frac_of_whole_pass": r["pass_frac"]},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved"] / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(r["achieved"] / HBM_PEAK, 4), "traffic": traffic,
+                     "kernel": r["kname"], "launch_ms": round(r["launch_ms"], 4),
+                     "alg_bytes_per_launch": r["bytes_per_launch"], "steps_per_launch": round(r["steps_per_launch"], 3)},
         "cpu_baseline": cpu,
         "time_to_misfit": ttm,
+        "configs": others,
     }
     print(json.dumps(line))
-    ctx.close()
     return 0
 
 

@@ -14,7 +14,9 @@
  * See oracle_body.h for the numerics contract (float32 storage, float64
  * arithmetic where Numba promotes) and the FP64 replay.
  *
- * Build: make -C oracle   (gcc -O2 -ffp-contract=off -fopenmp -shared)
+ * Build: make -C oracle   (the checker: gcc -O2 -ffp-contract=off -fopenmp -shared)
+ *        make -C oracle baseline BASELINE_OUT=<dir>   (the CPU baseline:
+ *        gcc -O3 -march=native -fopenmp, compiled on the host it is timed on)
  */
 #include <math.h>
 #include <stdlib.h>
@@ -118,6 +120,36 @@ double orc_bench_forward_f32(const orc_model *m, const orc_geom *g, const float 
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
     state_free_f32(&s);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* CPU baseline of the adjoint run (solver.py:290-321): warm + nsteps steps
+ * with the adjoint sources injected at the receivers and the interaction
+ * every sae steps against one stored snapshot (nsnap fields of the
+ * physics, values irrelevant to the timing); returns the wall time of the
+ * last nsteps. */
+double orc_bench_adjoint_f32(const orc_model *m, const orc_geom *g, const float *ad_x,
+                             const float *ad_y, const float *ad_z, int sae, const float *snap,
+                             int warm, int nsteps, int nthreads)
+{
+    state_f32 s;
+    int it;
+    long n = (long)m->nx * m->nz;
+    struct timespec t0, t1;
+    double ndt = (double)sae * m->dt;
+    float *k = (float *)calloc((size_t)(6 * n), sizeof(float));
+    omp_set_num_threads(nthreads);
+    state_init_f32(&s, m);
+    clear_f32(&s);
+    for (it = 0; it < warm + nsteps; it++) {
+        if (it == warm) clock_gettime(CLOCK_MONOTONIC, &t0);
+        step_f32(&s, m, g->nrec, g->rec_id, -1, ad_x, ad_y, ad_z, it % m->nt);
+        if (it % sae == 0)
+            interact_f32(&s, m, snap, ndt, k, k + n, k + 2 * n, k + 3 * n, k + 4 * n, k + 5 * n);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    state_free_f32(&s);
+    free(k);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
 

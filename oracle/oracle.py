@@ -20,10 +20,13 @@ with float64 storage.
 
 Pinning (tests/test_oracle_*.py): the stencil operators against polynomial
 closed forms and D- = -(D+)^T, the Ricker/taper/sponge/vps2lm closed forms,
-smoothing against a brute-force double sum, forward runs against the
-reference-held SH trace examples/inverse-checker/traces/vy_000000.npy,
-linearity/superposition/reciprocity/energy invariants, f32 against f64, and
-the adjoint kernels against finite differences of the misfit.
+smoothing against a brute-force double sum, forward runs against
+linearity/superposition/reciprocity, the discrete energy invariant, the 2-D
+SH Green's function and P and S arrival times in a homogeneous medium, f32
+against f64, and the adjoint kernels against finite differences of the
+misfit.  The reference-held SH trace examples/inverse-checker/traces/
+vy_000000.npy does NOT pin the forward run: it does not follow from the
+current example inputs (the test records the mismatch).
 """
 import ctypes
 import os
@@ -93,6 +96,25 @@ def lib():
         L.orc_diff_f64.restype = None
         _lib = L
     return _lib
+
+
+_baselines = {}
+
+
+def baseline_lib(path):
+    """A CPU-baseline build of oracle.c (make baseline / make portable):
+    the same kernel sequence compiled -O3 with contraction allowed, used
+    only for timing."""
+    if path not in _baselines:
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        L.orc_bench_forward_f32.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 3 + [ctypes.c_int] * 3
+        L.orc_bench_forward_f32.restype = ctypes.c_double
+        L.orc_bench_adjoint_f32.argtypes = ([ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 3
+                                            + [ctypes.c_int, vp] + [ctypes.c_int] * 3)
+        L.orc_bench_adjoint_f32.restype = ctypes.c_double
+        _baselines[path] = L
+    return _baselines[path]
 
 
 def _p(a):
@@ -236,12 +258,24 @@ class Sim:
             raise ValueError("nt must be a multiple of adjoint_interval")
         return kernels
 
-    def cpu_bench(self, stf, warm, nsteps, nthreads):
-        """Wall seconds of nsteps unfused forward steps on nthreads threads."""
+    def cpu_bench(self, stf, warm, nsteps, nthreads, L=None):
+        """Wall seconds of nsteps unfused forward steps on nthreads threads
+        (L: a baseline_lib build; the checker build by default)."""
         stf = [np.ascontiguousarray(s, dtype="float32") for s in stf]
         M, G = self._structs()
-        return lib().orc_bench_forward_f32(ctypes.byref(M), ctypes.byref(G), _p(stf[0]), _p(stf[1]),
-                                           _p(stf[2]), int(warm), int(nsteps), int(nthreads))
+        return (L or lib()).orc_bench_forward_f32(ctypes.byref(M), ctypes.byref(G), _p(stf[0]), _p(stf[1]),
+                                                  _p(stf[2]), int(warm), int(nsteps), int(nthreads))
+
+    def cpu_bench_adjoint(self, adstf, sae, warm, nsteps, nthreads, L=None):
+        """Wall seconds of nsteps unfused adjoint steps (adjoint sources at
+        the receivers, the interaction every sae steps) on nthreads threads.
+        adstf: (x, y, z) float arrays [nrec*nt]."""
+        ad = [np.ascontiguousarray(a, dtype="float32").reshape(-1) for a in adstf]
+        snap = np.full(NSNAP[self.physics] * self.nx * self.nz, 1e-3, np.float32)
+        M, G = self._structs()
+        return (L or lib()).orc_bench_adjoint_f32(ctypes.byref(M), ctypes.byref(G), _p(ad[0]), _p(ad[1]),
+                                                  _p(ad[2]), int(sae), _p(snap), int(warm), int(nsteps),
+                                                  int(nthreads))
 
 
 def waveform(syn, obs, nt, dt, precision="f32"):

@@ -216,6 +216,126 @@ def test_homogeneous_sh_matches_line_source_greens_function():
         assert err < 0.03, err
 
 
+def _discrete_energy(physics, snap, m, nx, nz, h, dt):
+    """Energy of the staggered leapfrog that the reference's step sequence
+    (solver.py:205-234) conserves when D- = -(D+)^T on the wavefield's
+    support and nothing is injected: with v(n+1) the velocities after step n
+    and the stresses written through the displacements, s(n) = C B u(n),
+        E(n) = 1/2 sum M v(n+1)^2 + 1/2 sum (B u(n))^T C (B u(n+1)),
+    u(n+1) = u(n) + dt v(n+1).  B: SH (D+x u, D+z u) with C = mu; P-SV
+    strains (D+x ux, D+z uz, D+z ux + D+x uz) with the isotropic C; spin
+    adds the rotation mismatch e = -phi - (D+x uz - D+z ux)/2 weighted 2 nu
+    (syy_c = 2 nu e, entering the velocities as -2 B^T) and the curvatures
+    D+x phi, D+z phi weighted mu_c + nu_c, with j as phi's inertia.
+    snap: one oracle snapshot (u before step n, v after it), float64."""
+    n = nx * nz
+    D = lambda f, w: O.diff(f, nx, nz, h, h, w).reshape(-1)
+    f = {k: np.asarray(v, np.float64) for k, v in m.items()}
+    if physics == O.SH:
+        u, v = snap[0], snap[1]
+        u1 = u + dt * v
+        pot = f["mu"] * (D(u, 1) * D(u1, 1) + D(u, 3) * D(u1, 3))
+        return 0.5 * np.sum(f["rho"] * v * v) + 0.5 * np.sum(pot)
+    ncomp = 2 if physics == O.PSV else 3
+    u, v = snap[:ncomp], snap[ncomp:]
+    u1 = u + dt * v
+
+    def strain(ux, uz):
+        return D(ux, 1), D(uz, 3), D(ux, 3) + D(uz, 1)
+
+    exx, ezz, g = strain(u[0], u[1])
+    exx1, ezz1, g1 = strain(u1[0], u1[1])
+    l2m = f["lam"] + 2 * f["mu"]
+    pot = l2m * (exx * exx1 + ezz * ezz1) + f["lam"] * (exx * ezz1 + ezz * exx1) + f["mu"] * g * g1
+    kin = f["rho"] * (v[0] * v[0] + v[1] * v[1])
+    if physics == O.SPIN:
+        e = -u[2] - 0.5 * (D(u[1], 1) - D(u[0], 3))
+        e1 = -u1[2] - 0.5 * (D(u1[1], 1) - D(u1[0], 3))
+        c = f["mu_c"] + f["nu_c"]
+        pot = pot + 4 * f["nu"] * e * e1 + c * (D(u[2], 1) * D(u1[2], 1) + D(u[2], 3) * D(u1[2], 3))
+        kin = kin + f["j"] * v[2] * v[2]
+    return 0.5 * np.sum(kin) + 0.5 * np.sum(pot)
+
+
+@pytest.mark.parametrize("physics", [O.SH, O.PSV, O.SPIN])
+def test_discrete_energy_is_conserved_without_sponge(physics):
+    """Sponge off, a heterogeneous model, a source that stops at step 150:
+    from then on the discrete energy stays constant to float64 rounding
+    while the wavefield has not reached the edge bands (where D- differs
+    from -(D+)^T).  A wrong sign, a dropped or doubled term, or a wrong
+    coefficient in any stress or velocity update breaks the constancy."""
+    from scipy.ndimage import gaussian_filter
+    nx = nz = 300
+    h, dt, nt, sae, K = 5.0, 5e-4, 360, 20, 150
+    n = nx * nz
+    rng = np.random.default_rng(5)
+    sr = lambda: (lambda a: (a / np.abs(a).max()).reshape(-1))(gaussian_filter(rng.standard_normal((nx, nz)), 6))
+    m = _homog(physics, nx, nz)
+    for k in m:
+        m[k] = (m[k] * (1 + 0.1 * sr())).astype(np.float32)
+    t = np.arange(nt) * dt
+    s = O.ricker(t, 30.0, 1.2 / 30.0, 0.0, 1e10)[1]
+    s[K:] = 0.0
+    src = [150 * nz + 150]
+    sim = O.Sim(physics, nx, nz, h, h, dt, nt, m, np.ones(n, np.float32), src, [10 * nz + 10], 0)
+    stf = [0.6 * s, s, -0.8 * s]
+    _, snaps = sim.forward(stf, sae=sae, precision="f64")
+    nsa = nt // sae
+    E = {}
+    for isa in range(nsa):
+        it = (nsa - isa) * sae - 1          # the step this snapshot brackets
+        if it >= K + sae:
+            E[it] = _discrete_energy(physics, snaps[isa], m, nx, nz, h, dt)
+    vals = np.array([E[k] for k in sorted(E)])
+    assert len(vals) >= 8 and vals[0] > 0
+    drift = np.abs(vals - vals[0]).max() / vals[0]
+    # the last snapshot's wavefield is negligible at the edges: the check is in the interior
+    last = np.abs(snaps[0]).reshape(snaps.shape[1], nx, nz)
+    edge = max(last[:, :4].max(), last[:, -4:].max(), last[:, :, :4].max(), last[:, :, -4:].max())
+    assert edge <= 1e-12 * last.max()
+    assert drift <= 1e-11, drift
+    # the same energy before the source stops is not constant (the check has teeth)
+    early = _discrete_energy(physics, snaps[nsa - 3], m, nx, nz, h, dt)   # step 59, source active
+    assert abs(early - vals[0]) > 1e-3 * vals[0]
+
+
+def _xcorr_lag(a, b, dt):
+    """Delay of b behind a (seconds) at the cross-correlation peak, with
+    parabolic refinement."""
+    c = np.correlate(b, a, mode="full")
+    k = int(np.argmax(c))
+    y0, y1, y2 = c[k - 1], c[k], c[k + 1]
+    frac = 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
+    return (k - (a.size - 1) + frac) * dt
+
+
+def test_homogeneous_psv_arrival_times_follow_vp_and_vs():
+    """A vertical point force in a homogeneous P-SV medium with vp/vs = 2
+    (lambda = 2 mu): along the force's axis uz carries the P wave, across
+    it the S wave.  Their delays between two receiver distances (30 and 80
+    cells) give vp and vs within 1%; lambda and mu in swapped roles, or a
+    wrong factor in the stress update, would move them by 10% or more."""
+    nx = nz = 300
+    h, dt, nt, f0 = 5.0, 5e-4, 720, 20.0
+    vp0, vs0, rho0 = 3000.0, 1500.0, 2600.0
+    n = nx * nz
+    lam, mu = O.vps2lm(np.full(n, vp0, np.float32), np.full(n, vs0, np.float32), np.full(n, rho0, np.float32))
+    m = {"rho": np.full(n, rho0, np.float32), "lam": lam, "mu": mu}
+    c0 = 150
+    d1, d2 = 30, 80
+    rec = [c0 * nz + c0 + d1, c0 * nz + c0 + d2,          # along z (the force's axis): P
+           (c0 + d1) * nz + c0, (c0 + d2) * nz + c0]      # along x: S
+    t = np.arange(nt) * dt
+    s = O.ricker(t, f0, 1.2 / f0, 0.0, 1e10)[1]
+    z = np.zeros(nt)
+    sim = O.Sim(O.PSV, nx, nz, h, h, dt, nt, m, np.ones(n, np.float32), [c0 * nz + c0], rec, 0)
+    uz = sim.forward([z, z, s], precision="f64")[0]["z"]
+    vp = (d2 - d1) * h / _xcorr_lag(uz[0], uz[1], dt)
+    vs = (d2 - d1) * h / _xcorr_lag(uz[2], uz[3], dt)
+    assert vp == pytest.approx(vp0, rel=0.01), vp
+    assert vs == pytest.approx(vs0, rel=0.01), vs
+
+
 @pytest.mark.parametrize("physics", [O.SH, O.PSV, O.SPIN])
 def test_float32_storage_stays_within_tolerance_of_the_fp64_replay(physics):
     nx, nz, nt, dt = 56, 52, 400, 5e-4
