@@ -231,7 +231,14 @@ def fwi_time_to_misfit(niter):
                                log=lambda s: print("# " + s, file=sys.stderr, flush=True))
     except Exception as e:  # noqa: BLE001 -- reported, not hidden
         return {"error": "%s: %s" % (type(e).__name__, e)}
+    g = init.grids[0]
+    src_z = float(g.shots[0].src_k[0] % g.nz) * g.dz
+    rec_z = float(g.shots[0].rec_k[0] % g.nz) * g.dz
     out = {"workload": init.name + " vp inversion, L-BFGS (lbfgs_mem 5, backtracking)", "iterations": niter,
+           "geometry": "sources at %g m, receivers at %g m depth, sponges on all four sides (width %d): the "
+                       "reference-style adjoint (forward operator reused, solver.py:308-321) is the misfit's "
+                       "gradient only away from the free surface's edge bands, so the acquisition sits below "
+                       "the top sponge instead of at 10 m under a free top" % (src_z, rec_z, g.sponge[0]),
            "value": None if r["time_to_misfit"] is None else round(r["time_to_misfit"], 4),
            "unit": "fraction of plain L-BFGS wall time, conditioner training included", "target": 0.6,
            "without_training": (None if r["time_to_misfit_without_training"] is None
