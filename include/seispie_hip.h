@@ -174,6 +174,20 @@ int sp_smooth_kernel(sp_ctx *ctx, int grid, int which, int sigma);
 /* the same smoothing on a caller buffer of nx*nz floats */
 int sp_smooth(sp_ctx *ctx, float *field, int nx, int nz, int sigma, int on_device);
 
+/* Allocation-free sp_smooth on a device buffer of nx*nz floats, in place.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc or hipFree; its weights and x-pass intermediate live in a
+ * fixed device array of about 17 MB per GPU, held until the process ends,
+ * which limits it to max(nx, nz) <= SP_NOALLOC_AXIS_MAX and nx*nz <=
+ * SP_NOALLOC_MAX_CELLS (larger fields are refused; use sp_smooth).  Every
+ * call on one GPU shares that array: calls that overlap on a GPU (other
+ * threads or contexts) race; a call returns when its stream has finished.
+ * Same results as sp_smooth(..., on_device = 1), bit for bit. */
+#define SP_NOALLOC_AXIS_MAX 8192
+#define SP_NOALLOC_MAX_CELLS 4194304LL
+int sp_smooth_noalloc(sp_ctx *ctx, float *field_dev, int nx, int nz, int sigma);
+
 /* Autotuner: times `steps` forward steps of the current shots for each
  * rows-per-wavefront chunk, work order and one or two steps per
  * launch, keeps the fastest for later runs and reports it; the choice is

@@ -88,6 +88,7 @@ def load():
         "sp_kernels_vps": ([vp, i], i),
         "sp_smooth_kernel": ([vp, i, i, i], i),
         "sp_smooth": ([vp, vp, i, i, i, i], i),
+        "sp_smooth_noalloc": ([vp, vp, i, i, i], i),
         "sp_last_timing": ([vp, ctypes.POINTER(ll), ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d)], i),
         "sp_autotune": ([vp, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "sp_forward_ckpt": ([vp, i], i),
@@ -113,7 +114,7 @@ def exported_symbols():
             "sp_add_shot", "sp_clear_shots", "sp_num_components", "sp_forward", "sp_get_records",
             "sp_set_data", "sp_misfit", "sp_adjoint", "sp_gradient", "sp_get_kernel", "sp_kernels_vps",
             "sp_smooth_kernel", "sp_smooth", "sp_last_timing", "sp_autotune", "sp_forward_ckpt",
-            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps", "sp_forward_steps", "sp_get_wavefield", "sp_get_option"]
+            "sp_adjoint_ckpt", "sp_set_adjoint_source", "sp_set_model_vps", "sp_forward_steps", "sp_get_wavefield", "sp_get_option", "sp_smooth_noalloc"]
 
 
 def _f32(a, n=None):
@@ -349,6 +350,14 @@ class Context:
         a = _f32(field, nx * nz).copy()
         self._chk(self.L.sp_smooth(self._p, _ptr(a), int(nx), int(nz), int(sigma), 0))
         return a
+
+    def smooth_noalloc(self, field, nx, nz, sigma):
+        """The smoothing of smooth() in place on a device tensor, through the
+        allocation-free entry point (scratch in a fixed device array; fields
+        up to 8192 cells a side and 4,194,304 cells)."""
+        self._chk(self.L.sp_smooth_noalloc(self._p, _devptr(field, int(nx) * int(nz), self.device),
+                                           int(nx), int(nz), int(sigma)))
+        return field
 
     def autotune(self, steps=20):
         """Pick the fastest rows-per-wavefront chunk and work order for the

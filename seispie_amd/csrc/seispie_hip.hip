@@ -1221,6 +1221,32 @@ void sponge_factors(int nx, int nz, int width, double alpha, int left, int right
     }
 }
 
+// Scratch of the allocation-free smoothing (sp_smooth_noalloc): the
+// weights, SP_NOALLOC_AXIS_MAX doubles for each of g, sx and sz, followed by
+// the nx*nz float intermediate of the x pass.  One copy per GPU, held until
+// the process ends.  Host code never passes `scratch` itself (on the host it
+// is a stand-in symbol, not a device address): it uses the address that
+// hipGetSymbolAddress returned for the device, kept in scratch_dev.
+constexpr long long SCRATCH_W = 3LL * SP_NOALLOC_AXIS_MAX;
+constexpr long long SCRATCH_DOUBLES = SCRATCH_W + (SP_NOALLOC_MAX_CELLS + 1) / 2;
+__device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[MAX_DEVICES];
+
+// Once per device, with that device current (called by sp_create).
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return 0;  // sp_smooth_noalloc then reports it
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_dev[device]) return 0;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return 1;
+    scratch_dev[device] = (double *)ptr;
+    return 0;
+}
+
 }  // namespace
 
 // ====================================================================== ABI
@@ -1234,6 +1260,7 @@ int sp_create(int device, sp_ctx **out)
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 4;
     if (device < 0 || device >= n) return 5;
     if (hipSetDevice(device) != hipSuccess) return 6;
+    if (scratch_setup(device)) return 8;
     sp_ctx *c = new sp_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
@@ -1766,13 +1793,13 @@ int sp_kernels_vps(sp_ctx *c, int grid)
     return 0;
 }
 
-static int smooth_dev(sp_ctx *c, float *field, int nx, int nz, int sigma)
+// init_gausian's weights (fd2d.py:71-80) in float64: g[0..nm) the Gaussian
+// by distance, sx[nx] and sz[nz] the row sums of the normaliser.
+static void smooth_weights(int nx, int nz, int sigma, double *g, double *sx, double *sz)
 {
-    if (sigma <= 0) return fail(c, "smooth sigma must be positive");
     const int nm = std::max(nx, nz);
     const double s = (double)sigma;
     const double cc = 1.0 / (std::sqrt(2.0 * M_PI) * s);
-    std::vector<double> g(nm), sx(nx), sz(nz);
     for (int d = 0; d < nm; d++) g[d] = cc * std::exp(-(double)((long long)d * d) / (2.0 * s * s));
     for (int i = 0; i < nx; i++) {
         double a = 0.0;
@@ -1784,20 +1811,36 @@ static int smooth_dev(sp_ctx *c, float *field, int nx, int nz, int sigma)
         for (int n = 0; n < nz; n++) a += g[std::abs(j - n)];
         sz[j] = a;
     }
-    DevTmp<double> dg;
-    DevTmp<float> tmp;
+}
+
+// The two passes (apply_gauxxian_x, apply_gauxxian_z: fd2d.py:82-108) on
+// the context's stream; dg holds the weights as smooth_weights lays them
+// out, tmp is an nx*nz float buffer.  Returns when the field is smoothed.
+static int smooth_launch(sp_ctx *c, float *field, float *tmp, const double *dg, int nx, int nz)
+{
+    const int nm = std::max(nx, nz);
     const long long N = (long long)nx * nz;
-    HIPCHECK(c, hipMalloc(&dg.p, (size_t)(nm + nx + nz) * sizeof(double)));
-    HIPCHECK(c, hipMalloc(&tmp.p, (size_t)N * sizeof(float)));
-    HIPCHECK(c, hipMemcpyAsync(dg.p, g.data(), nm * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(dg.p + nm, sx.data(), nx * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(dg.p + nm + nx, sz.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
     const dim3 gr((unsigned)((N + 255) / 256)), bl(256);
-    hipLaunchKernelGGL(k_smooth_x, gr, bl, 0, c->stream, field, tmp.p, dg.p, nx, nz);
-    hipLaunchKernelGGL(k_smooth_z, gr, bl, 0, c->stream, field, tmp.p, dg.p, dg.p + nm, dg.p + nm + nx, nx, nz);
+    hipLaunchKernelGGL(k_smooth_x, gr, bl, 0, c->stream, field, tmp, dg, nx, nz);
+    hipLaunchKernelGGL(k_smooth_z, gr, bl, 0, c->stream, field, tmp, dg, dg + nm, dg + nm + nx, nx, nz);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+static int smooth_dev(sp_ctx *c, float *field, int nx, int nz, int sigma)
+{
+    if (sigma <= 0) return fail(c, "smooth sigma must be positive");
+    const int nm = std::max(nx, nz);
+    std::vector<double> w((size_t)nm + nx + nz);
+    smooth_weights(nx, nz, sigma, w.data(), w.data() + nm, w.data() + nm + nx);
+    DevTmp<double> dg;
+    DevTmp<float> tmp;
+    const long long N = (long long)nx * nz;
+    HIPCHECK(c, hipMalloc(&dg.p, w.size() * sizeof(double)));
+    HIPCHECK(c, hipMalloc(&tmp.p, (size_t)N * sizeof(float)));
+    HIPCHECK(c, hipMemcpyAsync(dg.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return smooth_launch(c, field, tmp.p, dg.p, nx, nz);
 }
 
 int sp_smooth_kernel(sp_ctx *c, int grid, int which, int sigma)
@@ -1824,6 +1867,29 @@ int sp_smooth(sp_ctx *c, float *field, int nx, int nz, int sigma, int on_dev)
     }
     HIPCHECK(c, hipFree(d));
     return 0;
+}
+
+int sp_smooth_noalloc(sp_ctx *c, float *field, int nx, int nz, int sigma)
+{
+    if (!c) return 3;
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (nx < 1 || nz < 1 || !field) return fail(c, "bad field");
+    if (sigma <= 0) return fail(c, "smooth sigma must be positive");
+    if (std::max(nx, nz) > SP_NOALLOC_AXIS_MAX || (long long)nx * nz > SP_NOALLOC_MAX_CELLS)
+        return fail(c, "sp_smooth_noalloc: " + std::to_string(nx) + " x " + std::to_string(nz) +
+                           " exceeds its scratch (at most " + std::to_string(SP_NOALLOC_AXIS_MAX) +
+                           " cells a side and " + std::to_string(SP_NOALLOC_MAX_CELLS) + " cells); use sp_smooth");
+    double *base = (c->device >= 0 && c->device < MAX_DEVICES) ? scratch_dev[c->device] : nullptr;
+    if (!base) return fail(c, "sp_smooth_noalloc: no scratch address for this device");
+    // host weights in a fixed per-thread array; the copy has completed when
+    // the call returns (smooth_launch synchronises the stream)
+    thread_local double w[SCRATCH_W];
+    const int nm = std::max(nx, nz);
+    smooth_weights(nx, nz, sigma, w, w + nm, w + nm + nx);
+    HIPCHECK(c, hipMemcpyAsync(base, w, (size_t)(nm + nx + nz) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // every cell of the intermediate is written by the x pass before the z
+    // pass reads it, so nothing in the scratch needs clearing between calls
+    return smooth_launch(c, field, (float *)(base + SCRATCH_W), base, nx, nz);
 }
 
 int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
