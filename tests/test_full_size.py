@@ -56,7 +56,7 @@ def test_batched_shots_equal_single_runs_at_full_size(ctx_factory, name):
         np.testing.assert_array_equal(batch[k], alone, err_msg="shot %d" % k)
 
 
-def test_records_are_exactly_linear_in_the_source_at_full_size(ctx_factory):
+def test_records_are_linear_in_the_source_at_full_size_within_1e4(ctx_factory):
     base = configs.psv_config3()
     print("case: full-size linearity", base.name, flush=True)
     r1 = _records(ctx_factory, base)
@@ -72,11 +72,14 @@ def test_records_are_exactly_linear_in_the_source_at_full_size(ctx_factory):
 def test_one_shot_of_the_benchmark_matches_the_oracle(ctx_factory):
     """Shot 0 of the 128-shot 1000^2 workload over all 4000 steps against
     the oracle (float32 storage, float64 arithmetic; ~15 s of host time).
-    The default exact mode must equal it bit for bit, with one or two cells per lane.  The fast mode
-    (reciprocal multiplications and fused multiply-adds) is held to twice
-    the seismogram tolerance; its float32 rounding differences grow chaotically with the
-    number of steps, to ~1e-5 here, the size of the gap between the
-    oracle's own float32 and float64 modes (5.3e-6)."""
+    The default exact mode must equal it bit for bit, with one or two cells
+    per lane.  The fast mode (reciprocal multiplications and fused
+    multiply-adds) is OUTSIDE the 1e-5 seismogram tolerance at this length
+    and is held to 2e-5 only to catch a regression: its float32 rounding
+    differences grow chaotically with the number of steps, to 1.1e-5 here,
+    the size of the gap between the oracle's own float32 and float64 modes
+    (5.3e-6).  Nothing that must meet the tolerance (the bench, the plugin,
+    autograd) runs it."""
     from tests.helpers import oracle_records, rel_l2
     prob = configs.psv_config3()
     g = prob.grids[0]
