@@ -442,6 +442,139 @@ int FN(orc_adjoint)(const orc_model *m, const orc_geom *g, const T *ad_x, const 
     return 0;
 }
 
+/* Transposes of the staggered differences, row by row including their edge
+ * bands (D- rows valid for 2 <= i < n-2, D+ rows for 1 <= i < n-2), in
+ * float64: (D-^T y)_j = c1 (y_j - y_{j+1}) - c2 (y_{j-1} - y_{j+2}) and
+ * (D+^T y)_j = c1 (y_{j-1} - y_j) - c2 (y_{j-2} - y_{j+1}), each y_i taken
+ * only where row i is valid.  In the interior they equal -D+ and -D-. */
+static inline double FN(tm)(const double *y, long k, long st, int i, int n, double c1, double c2)
+{
+#define VM(q) ((q) >= 2 && (q) < n - 2)
+    double a = VM(i) ? y[k] : 0.0, b = VM(i + 1) ? y[k + st] : 0.0;
+    double cm = VM(i - 1) ? y[k - st] : 0.0, d = VM(i + 2) ? y[k + 2 * st] : 0.0;
+#undef VM
+    return c1 * (a - b) - c2 * (cm - d);
+}
+static inline double FN(tp)(const double *y, long k, long st, int i, int n, double c1, double c2)
+{
+#define VP(q) ((q) >= 1 && (q) < n - 2)
+    double a = VP(i - 1) ? y[k - st] : 0.0, b = VP(i) ? y[k] : 0.0;
+    double cm = VP(i - 2) ? y[k - 2 * st] : 0.0, d = VP(i + 1) ? y[k + st] : 0.0;
+#undef VP
+    return c1 * (a - b) - c2 * (cm - d);
+}
+
+/* Discrete adjoint of the forward run (SH and P-SV): the exact transpose of
+ * the time-stepping map of solver.py:205-234 (rounding aside), run backwards
+ * from step nt-1.  With v(n+1) = w = b (v(n) + dt (A s(n) + f) / rho),
+ * u(n+1) = u(n) + dt w, s(n+1) = s(n) + dt C B w and records R u(n+1), the
+ * adjoint state (s^, v^, u^) steps as
+ *     u^ += R^T g(n);  e = dt C s^;  q = dt u^ + B^T e;  w^ = v^ + q;
+ *     v^ = b w^;  s^ += A^T (dt v^ / rho)
+ * (B: D+ strains; A: D- divergence; transposes with their edge bands), and
+ * the gradient of L (g = dL/d records, [nrec][nt] per component, not time
+ * reversed) accumulates, before s^ takes this step's update,
+ *     dL/dmu  += dt s^ . dC/dmu B w,   dL/dlam += dt s^ . dC/dlam B w,
+ *     dL/drho -= w . q / rho.
+ * The kernels returned are dt dL/dm (the convention of the reference-style
+ * kernels).  vsnap: the forward snapshots of orc_forward with sae = 1
+ * ([nt][nsnap][N], reversed: v after step n in slot nt-1-n).  Arithmetic
+ * in float64 on T storage. */
+int FN(orc_adjoint_exact)(const orc_model *m, const orc_geom *g, const T *gx, const T *gy, const T *gz,
+                          const T *vsnap, T *klam, T *kmu, T *krho)
+{
+    int nx = m->nx, nz = m->nz, nt = m->nt, phys = m->physics, it, r, c;
+    long n = (long)nx * nz, k;
+    int NV = phys == ORC_SH ? 1 : 2, NS = phys == ORC_SH ? 2 : 3, nf = orc_nsnap(phys);
+    double dx = (double)m->dx, dz = (double)m->dz, dt = m->dt;
+    double c1x = 9.0 / (8.0 * dx), c2x = 1.0 / (24.0 * dx), c1z = 9.0 / (8.0 * dz), c2z = 1.0 / (24.0 * dz);
+    double *sh, *vh, *uh, *e, *q, *y;
+    if (phys != ORC_SH && phys != ORC_PSV) return -2;
+    sh = (double *)calloc((size_t)(NS * n), sizeof(double));
+    e = (double *)calloc((size_t)(NS * n), sizeof(double));
+    vh = (double *)calloc((size_t)(NV * n), sizeof(double));
+    uh = (double *)calloc((size_t)(NV * n), sizeof(double));
+    q = (double *)calloc((size_t)(NV * n), sizeof(double));
+    y = (double *)calloc((size_t)(NV * n), sizeof(double));
+    for (it = nt - 1; it >= 0; it--) {
+        const T *w = vsnap + (long)(nt - 1 - it) * nf * n + (long)NV * n;  /* v after step it */
+        /* records of step it (u after it) */
+        for (r = 0; r < g->nrec; r++) {
+            long kr = g->rec_id[r];
+            if (phys == ORC_SH) {
+                uh[kr] += (double)gy[(long)r * nt + it];
+            } else {
+                uh[kr] += (double)gx[(long)r * nt + it];
+                uh[n + kr] += (double)gz[(long)r * nt + it];
+            }
+        }
+        /* e = dt C s^ */
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; k++) {
+            double mu = (double)m->mu[k];
+            if (phys == ORC_SH) {
+                e[k] = dt * mu * sh[k];
+                e[n + k] = dt * mu * sh[n + k];
+            } else {
+                double lam = (double)m->lam[k], l2m = lam + 2.0 * mu;
+                e[k] = dt * (l2m * sh[k] + lam * sh[n + k]);
+                e[n + k] = dt * (lam * sh[k] + l2m * sh[n + k]);
+                e[2 * n + k] = dt * mu * sh[2 * n + k];
+            }
+        }
+        /* gradient of mu (and lam) from s^ before its update and B w */
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; k++) {
+            int j = (int)(k % nz), i = (int)(k / nz);
+            if (phys == ORC_SH) {
+                double ex = FN(dxp)(w, i, k, dx, nx, nz), ez = FN(dzp)(w, j, k, dz, nz);
+                kmu[k] = (T)((double)kmu[k] + dt * dt * (sh[k] * ex + sh[n + k] * ez));
+            } else {
+                double exx = FN(dxp)(w, i, k, dx, nx, nz), ezz = FN(dzp)(w + n, j, k, dz, nz);
+                double gxz = FN(dzp)(w, j, k, dz, nz) + FN(dxp)(w + n, i, k, dx, nx, nz);
+                klam[k] = (T)((double)klam[k] + dt * dt * (sh[k] + sh[n + k]) * (exx + ezz));
+                kmu[k] = (T)((double)kmu[k] + dt * dt * (2.0 * sh[k] * exx + 2.0 * sh[n + k] * ezz + sh[2 * n + k] * gxz));
+            }
+        }
+        /* q = dt u^ + B^T e, w^ = v^ + q, v^ = b w^; rho gradient */
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; k++) {
+            int j = (int)(k % nz), i = (int)(k / nz), cc;
+            double bt[2], acc = 0.0, rho = (double)m->rho[k];
+            if (phys == ORC_SH) {
+                bt[0] = FN(tp)(e, k, nz, i, nx, c1x, c2x) + FN(tp)(e + n, k, 1, j, nz, c1z, c2z);
+            } else {
+                bt[0] = FN(tp)(e, k, nz, i, nx, c1x, c2x) + FN(tp)(e + 2 * n, k, 1, j, nz, c1z, c2z);
+                bt[1] = FN(tp)(e + n, k, 1, j, nz, c1z, c2z) + FN(tp)(e + 2 * n, k, nz, i, nx, c1x, c2x);
+            }
+            for (cc = 0; cc < NV; cc++) {
+                double qq = dt * uh[cc * n + k] + bt[cc];
+                double wh = vh[cc * n + k] + qq;
+                acc += (double)w[cc * n + k] * qq;
+                vh[cc * n + k] = (double)m->bound[k] * wh;
+                y[cc * n + k] = dt * vh[cc * n + k] / rho;
+            }
+            krho[k] = (T)((double)krho[k] - dt * acc / rho);
+        }
+        /* s^ += A^T (dt v^ / rho) */
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; k++) {
+            int j = (int)(k % nz), i = (int)(k / nz);
+            if (phys == ORC_SH) {
+                sh[k] += FN(tm)(y, k, nz, i, nx, c1x, c2x);
+                sh[n + k] += FN(tm)(y, k, 1, j, nz, c1z, c2z);
+            } else {
+                sh[k] += FN(tm)(y, k, nz, i, nx, c1x, c2x);
+                sh[n + k] += FN(tm)(y + n, k, 1, j, nz, c1z, c2z);
+                sh[2 * n + k] += FN(tm)(y, k, 1, j, nz, c1z, c2z) + FN(tm)(y + n, k, nz, i, nx, c1x, c2x);
+            }
+        }
+    }
+    (void)c;
+    free(sh); free(e); free(vh); free(uh); free(q); free(y);
+    return 0;
+}
+
 /* diff kernel of misfit/waveform.py:5-25.  res and adstf are [nrec][nt].
  * Returns sum(res^2) in float64 (waveform.py:34 takes its square root). */
 double FN(orc_waveform)(const T *syn, const T *obs, T *adstf, T *res, int nt, double dt, int nrec)

@@ -418,7 +418,6 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             S0[2][f][c] = PF[2].s[f][c];
             S0[3][f][c] = PF[3].s[f][c];
         }
-    load2<P, W>(PF[1], RS, RM, NP4, rd, m0 + 1, nx, nz, vj, i0, i1);
 
     for (int mb = m0; mb <= mend; mb += 4) {
 #pragma unroll
@@ -426,7 +425,10 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             const int m = mb + u;
             if (m > mend) break;
             const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
-            load2<P, W>(PF[B], RS, RM, NP4, rd, m + 2, nx, nz, vj, i0, i1);
+            // prefetch one row ahead: the row of the next iteration (a
+            // second row in flight measured no faster and costs about 70
+            // VGPRs with two cells per lane)
+            load2<P, W>(PF[A], RS, RM, NP4, rd, m + 1, nx, nz, vj, i0, i1);
             const Row2<P, W> &cur = PF[D];
 #pragma unroll
             for (int f = 0; f < NS; f++)

@@ -410,6 +410,57 @@ def test_adjoint_kernels_match_finite_differences(physics, keys):
         assert adj == pytest.approx(fd, rel=2e-3), (key, adj, fd)
 
 
+@pytest.mark.parametrize("physics", [O.SH, O.PSV])
+def test_discrete_adjoint_matches_finite_differences_beside_a_free_surface(physics):
+    """The discrete adjoint (oracle.orc_adjoint_exact: the transpose of the
+    time stepping, edge bands and sponge included) is the gradient of a loss
+    linear in the records, L = sum W . records, with the source and the
+    receivers two cells below a free top and sponges on the other sides:
+    float64 replay, central differences along a smooth perturbation of the
+    whole model, measured along the perturbation the float32 model really
+    takes.  The reference-style kernels (forward operator reused) miss it
+    there, for P-SV even in sign."""
+    from scipy.ndimage import gaussian_filter
+    nx, nz, h, dt, nt = 60, 50, 5.0, 5e-4, 300
+    rng = np.random.default_rng(1)
+    sr = lambda s=3: (lambda a: (a / np.abs(a).max()).reshape(-1))(gaussian_filter(rng.standard_normal((nx, nz)), s))
+    vp = (3000 * (1 + 0.05 * sr())).astype("f4")
+    vs = (1730 * (1 + 0.05 * sr())).astype("f4")
+    rho = (2600 * (1 + 0.05 * sr())).astype("f4")
+    lam, mu = O.vps2lm(vp, vs, rho)
+    m = {"rho": rho, "lam": lam, "mu": mu}
+    bound = O.set_bound(nx, nz, 10, 0.03, 1, 1, 1, 0)
+    src = np.array([[30 * h, 2 * h, 0, 40.0, 0.03, 30.0, 1e10]])
+    rec = np.array([[i * h, 2 * h] for i in range(5, nx - 5, 3)])
+    sid = O.grid_index(src, np.float32(h), np.float32(h), nz)
+    rid = O.grid_index(rec, np.float32(h), np.float32(h), nz)
+    stf = O.make_stf(src, nt, dt)
+    comps = ["y"] if physics == O.SH else ["x", "z"]
+    W = {c: rng.standard_normal((rid.size, nt)) for c in comps}
+
+    def loss(mm):
+        obs, _ = O.Sim(physics, nx, nz, h, h, dt, nt, mm, bound, sid, rid, 0).forward(stf, precision="f64")
+        return sum(float(np.sum(W[c] * obs[c])) for c in comps)
+
+    sim = O.Sim(physics, nx, nz, h, h, dt, nt, m, bound, sid, rid, 0)
+    _, snaps = sim.forward(stf, sae=1, precision="f64")
+    K = sim.adjoint_exact(W, snaps, "f64")
+    Kr = sim.adjoint({c: -W[c][:, ::-1] for c in comps}, 1, snaps, None, "f64")
+    worst_ref = 0.0
+    for key in (["mu", "rho"] if physics == O.SH else ["lam", "mu", "rho"]):
+        base = m[key].astype(np.float64)
+        d = sr(2)
+        p32 = (base * (1 + 1e-3 * d)).astype(np.float32)
+        q32 = (base * (1 - 1e-3 * d)).astype(np.float32)
+        fd = loss(dict(m, **{key: p32})) - loss(dict(m, **{key: q32}))
+        step = p32.astype(np.float64) - q32.astype(np.float64)
+        adj = float(np.sum(K[key].astype(np.float64) / dt * step))
+        ref = float(np.sum(Kr[key].astype(np.float64) / dt * step))
+        assert adj == pytest.approx(fd, rel=2e-5), (key, adj, fd)
+        worst_ref = max(worst_ref, abs(ref - fd) / abs(fd))
+    assert worst_ref > 2e-3
+
+
 # ------------------------------------------------ reference-held vectors
 @pytest.mark.skipif(not os.path.exists(REF), reason="reference examples not mounted")
 def test_reference_trace_is_from_another_configuration():
