@@ -78,6 +78,13 @@ typedef struct sp_ctx sp_ctx;
                                rounding-level differences that grow with the run length: 1.1e-5 relative
                                L2 on seismograms after 4000 steps at 1000^2, OUTSIDE the 1e-5 seismogram
                                tolerance, so runs that must meet it keep the default) */
+#define SP_OPT_ADJOINT 9    /* sp_adjoint_ckpt runs 0 (default): the reference-style adjoint (the forward
+                               operator reused with the adjoint sources injected at the receivers,
+                               solver.py:290-321, kernels correlated every adjoint_interval steps); 1: the
+                               discrete adjoint, the exact transpose of the forward time stepping with its
+                               edge bands and sponge (SH and P-SV, adjoint_interval 1, K = dt dL/dm exactly:
+                               the gradient also beside a free surface and inside sponges, where the
+                               reference-style kernels are not).  sp_gradient and sp_adjoint refuse 1. */
 
 /* Context and device.  Replaces fd2d.setup (fd2d.py:111-120). */
 int sp_create(int device, sp_ctx **out);

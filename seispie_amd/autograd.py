@@ -26,7 +26,16 @@ Sampling the correlation every `sae` steps (the reference's
 adjoint_interval) approximates the time integral; sae = 1 is the default.
 Inside sponge layers the velocities are damped every step, so there the
 density kernel (adjoint velocity times forward velocity) is approximate;
-lam and mu are not affected.
+lam and mu are not affected.  That backward reuses the forward operator, as
+the reference's adjoint does (solver.py:308-321), so next to a free surface,
+where the difference operators' edge bands make D- differ from -(D+)^T, it
+is not the gradient of L either.
+
+adjoint="discrete" (SH and P-SV) makes the backward the library's discrete
+adjoint instead: the exact transpose of the forward time stepping, edge
+bands and sponge included, correlated every step (sae = 1), so K / dt is
+dL/dm up to float32 rounding anywhere in the grid, free surface and sponges
+included.
 
 SH uses mu and rho only (its vp gradient is zero); P-SV uses all three.
 
@@ -59,9 +68,17 @@ class Simulation:
     [nrec]) with grid indices k = i*nz + j; every shot has the same nrec."""
 
     def __init__(self, physics, nx, nz, dx, dz, nt, dt, shots, sponge=None, interval=None, sae=1, device=0,
-                 init_model=None, autotune=True):
+                 init_model=None, autotune=True, adjoint="reference"):
         if physics not in (_lib.SH, _lib.PSV, _lib.SPIN):
             raise ValueError("unknown physics %r" % (physics,))
+        if adjoint not in ("reference", "discrete"):
+            raise ValueError("adjoint must be 'reference' or 'discrete'")
+        if adjoint == "discrete":
+            if physics == _lib.SPIN:
+                raise ValueError("the discrete adjoint covers SH and P-SV")
+            if sae != 1:
+                raise ValueError("the discrete adjoint correlates every step: sae must be 1")
+        self.adjoint = adjoint
         if nt % sae:
             raise ValueError("nt must be a multiple of sae")
         nrec = {len(np.asarray(s[2]).reshape(-1)) for s in shots}
@@ -76,6 +93,7 @@ class Simulation:
         self.device = torch.device("cuda", device)
         self.ctx = _lib.Context(device)
         self.ctx.set_physics(physics, nt, dt)
+        self.ctx.set_option(_lib.OPT_ADJOINT, 1 if adjoint == "discrete" else 0)
         self.param = 1 if physics == _lib.SPIN else 0
         if init_model is None:
             ones = np.ones(nx * nz, "f4")

@@ -231,18 +231,28 @@ def fwi_config5_model(n=500, dx=5.0, true_model=True):
     return [vp, vs, rho]
 
 
-def fwi_config5(n=500, nt=3000, nshots=64, true_model=True):
+def fwi_config5(n=500, nt=3000, nshots=64, true_model=True, surface=False):
     """Learned-conditioner FWI: 500^2, dx 5 m, dt 0.5 ms, nt 3000, 64 shots of
-    a 15 Hz Ricker, 200 receivers.  Sponges on all four sides, sources at
-    125 m and receivers at 130 m depth, just below the 20-cell sponge: the
-    reference-style adjoint is the gradient only away from the stencils'
-    edge bands and the sponges (with sources and receivers at 10 m beside a
-    free surface the finite-difference slope of the misfit and the adjoint
-    kernels disagree even in sign), and an inversion needs the gradient."""
+    a 15 Hz Ricker, 200 receivers.  Default: sponges on all four sides,
+    sources at 125 m and receivers at 130 m depth, just below the 20-cell
+    sponge: the reference-style adjoint is the gradient only away from the
+    stencils' edge bands and the sponges (with sources and receivers at 10 m
+    beside a free surface the finite-difference slope of the misfit and the
+    reference-style kernels disagree even in sign).  surface=True: top free,
+    sources at 10 m, receivers at 2 dz, for the discrete adjoint."""
     dt = 5e-4
-    shots = _geometry(_lib.PSV, n, n, 5.0, 5.0, nt, dt, nshots, min(200, n), 15.0, ang=30.0, src_z=125.0, rec_z=130.0)
-    g = Grid(n, n, 5.0, 5.0, fwi_config5_model(n, 5.0, true_model), 0, sponge=(20, 0.015, 1, 1, 1, 1), shots=shots)
-    return Problem(_lib.PSV, nt, dt, [g], sae=50, name="fwi_%dx%d_%dshots_nt%d" % (n, n, nshots, nt))
+    if surface:
+        # the acquisition of the other workloads: top free, sources at 10 m,
+        # receivers two cells down; needs the discrete adjoint for a gradient
+        shots = _geometry(_lib.PSV, n, n, 5.0, 5.0, nt, dt, nshots, min(200, n), 15.0, ang=30.0)
+        sponge = (20, 0.015, 1, 1, 1, 0)
+    else:
+        shots = _geometry(_lib.PSV, n, n, 5.0, 5.0, nt, dt, nshots, min(200, n), 15.0, ang=30.0, src_z=125.0,
+                          rec_z=130.0)
+        sponge = (20, 0.015, 1, 1, 1, 1)
+    g = Grid(n, n, 5.0, 5.0, fwi_config5_model(n, 5.0, true_model), 0, sponge=sponge, shots=shots)
+    return Problem(_lib.PSV, nt, dt, [g], sae=50,
+                   name="fwi_%dx%d_%dshots_nt%d%s" % (n, n, nshots, nt, "_surface" if surface else ""))
 
 
 def load_problem(ctx, prob):

@@ -65,6 +65,7 @@ namespace {
 #include "k_step3.h"
 #include "k_step2.h"
 #include "sp_kernels.h"
+#include "k_adjx.h"
 
 // ===================================================================== host
 struct Grid {
@@ -106,6 +107,7 @@ struct sp_ctx {
     int shift = 3, wout = 58;  // strip geometry of the step kernel
     int fuse = 1;              // two steps per launch where nothing observes the state between them
     int exact = 1;             // step kernels follow the reference's float64 operations exactly
+    int adjoint_mode = 0;      // sp_adjoint_ckpt: 0 the reference-style adjoint, 1 the discrete adjoint
     int *d_counter = nullptr;  // step counter read by the kernels of a replayed graph
     const int *itp = nullptr;  // set while a graph is captured
     const int *snap_tab = nullptr;       // snapshot slots of the forward run being captured
@@ -1110,6 +1112,112 @@ int run_adjoint_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int sae, in
     return 0;
 }
 
+// The discrete adjoint of a batch (k_adjx.h): the forward state of every
+// step is recomputed segment by segment (last segment first) from the
+// checkpoints, with the velocity after each step kept as a snapshot, and
+// the adjoint runs backwards over the segment's steps.
+int run_adjoint_exact_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
+{
+    const int P = c->physics, nt = c->nt;
+    const int cap = C + 2;  // snapshots per segment, at most
+    size_t need = 0;
+    for (int s : batch) need += (size_t)cap * n_snap(P) * c->grids[c->shots[s].grid].Npad * sizeof(float);
+    if (need > c->seg_pool_bytes) {
+        if (c->seg_pool) HIPCHECK(c, hipFree(c->seg_pool));
+        c->seg_pool = nullptr;
+        c->seg_pool_bytes = 0;
+        HIPCHECK(c, hipMalloc(&c->seg_pool, need));
+        c->seg_pool_bytes = need;
+    }
+    size_t off = 0;
+    for (auto &sh : c->shots) sh.snap = nullptr;
+    for (int s : batch) {
+        Shot &S = c->shots[s];
+        S.snap = c->seg_pool + off / sizeof(float);
+        off += (size_t)cap * n_snap(P) * c->grids[S.grid].Npad * sizeof(float);
+        if (!S.state2) {
+            const long long nf = (long long)n_slots(P) * c->grids[S.grid].Npad;
+            HIPCHECK(c, hipMalloc(&S.state2, (size_t)nf * sizeof(float)));
+            if (fill(c, S.state2, nf, 0.f)) return 2;
+        }
+    }
+    int nitems = 0;
+    if (prepare_batch(c, batch, &nitems)) return 2;
+    std::vector<ShotDesc> d2;
+    for (int s : batch) {
+        ShotDesc d = make_desc(c, c->shots[s]);
+        d.state = c->shots[s].state2;
+        d2.push_back(d);
+    }
+    if (ensure(c, c->d_desc2, c->d_desc2_cap, d2.size())) return 2;
+    HIPCHECK(c, hipMemcpyAsync(c->d_desc2, d2.data(), d2.size() * sizeof(ShotDesc), hipMemcpyHostToDevice, c->stream));
+    if (clear_state(c, batch)) return 2;  // the adjoint state starts at zero
+    InteractPlan pl;
+    if (interact_lists(c, batch, pl)) return 2;
+    const int nb = (int)batch.size();
+    long long maxN = 1;
+    int maxrec = 1;
+    for (int s : batch) {
+        maxN = std::max(maxN, c->grids[c->shots[s].grid].N);
+        maxrec = std::max(maxrec, c->shots[s].nrec);
+    }
+    const dim3 sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
+    const dim3 rgrid((unsigned)((maxrec + 255) / 256), (unsigned)nb);
+    const int K = (nt + C - 1) / C;
+    for (int seg = K - 1; seg >= 0; seg--) {
+        const int lo = seg * C, hi = std::min(nt, lo + C);
+        for (int s : batch)
+            if (ckpt_copy(c, c->shots[s], seg, false, 0)) return 2;
+        // forward steps lo..hi-1; slot hi-1-it holds v after step it
+        int set2 = 0;
+        for (int it = lo; it < hi;) {
+            const int ns2 = steps_next(c, it, hi, 0, 0);
+            const int isa = hi - 1 - it;
+            const int isb = ns2 == 2 ? hi - 2 - it : -1;
+            const bool copy = ns2 == 1;
+            if (copy)
+                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa, 0, 0, c->physics);
+            if (launch_step(c, nitems, it, -1, 0, set2, ns2, c->d_desc2, ns2 == 2 ? isa : -1, isb)) return 2;
+            set2 ^= 1;
+            if (copy)
+                hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc2, nb, maxN, isa, 1, set2,
+                                   c->physics);
+            it += ns2;
+        }
+        HIPCHECK(c, hipGetLastError());
+        for (int it = hi - 1; it >= lo; it--) {
+            if (P == SP_SH) hipLaunchKernelGGL(k_xrec<SP_SH>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
+            else hipLaunchKernelGGL(k_xrec<SP_PSV>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
+            for (size_t g = 0; g < pl.per_grid.size(); g++) {
+                if (pl.per_grid[g].empty()) continue;
+                const Grid &G = c->grids[g];
+                XAdjArgs a;
+                a.shots = c->d_desc;
+                a.list = c->d_list + pl.off[g];
+                a.nlist = (int)pl.per_grid[g].size();
+                a.nx = G.nx;
+                a.nz = G.nz;
+                a.Npad = G.Npad;
+                a.kern = G.kern;
+                a.slot = hi - 1 - it;
+                a.dt = c->dt;
+                a.c1x = 9.0 / (8.0 * (double)G.dx);
+                a.c2x = 1.0 / (24.0 * (double)G.dx);
+                a.c1z = 9.0 / (8.0 * (double)G.dz);
+                a.c2z = 1.0 / (24.0 * (double)G.dz);
+                const dim3 gr((unsigned)((G.N + 255) / 256));
+                if (P == SP_SH) hipLaunchKernelGGL(k_xw<SP_SH>, gr, dim3(256), 0, c->stream, a);
+                else hipLaunchKernelGGL(k_xw<SP_PSV>, gr, dim3(256), 0, c->stream, a);
+            }
+            if (P == SP_SH) hipLaunchKernelGGL(k_xs<SP_SH>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            else hipLaunchKernelGGL(k_xs<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            HIPCHECK(c, hipGetLastError());
+        }
+    }
+    c->t_bytes += step_alg_bytes(c, batch) * nt;
+    return 0;
+}
+
 // split all shots into batches whose snapshots fit the pool
 int plan_batches(sp_ctx *c, int sae, std::vector<std::vector<int>> &out)
 {
@@ -1362,6 +1470,10 @@ int sp_set_option(sp_ctx *c, int key, long long v)
     case SP_OPT_KERNEL: c->kernel = v ? 1 : 0; return 0;
     case SP_OPT_FUSE: c->fuse = v == 2 ? 2 : (v ? 1 : 0); return 0;
     case SP_OPT_EXACT: c->exact = v ? 1 : 0; return 0;
+    case SP_OPT_ADJOINT:
+        if (v != 0 && v != 1) return fail(c, "adjoint must be 0 (reference-style) or 1 (discrete)");
+        c->adjoint_mode = (int)v;
+        return 0;
     case SP_OPT_STRIP: {
         const int sh = (int)(v >> 16), wo = (int)(v & 0xffff);
         if (sh < 3 || wo < 1 || sh + wo > 61) return fail(c, "strip geometry needs shift >= 3 and shift + width <= 61");
@@ -1669,6 +1781,7 @@ int sp_misfit(sp_ctx *c, double *norms_out)
 int sp_adjoint(sp_ctx *c, int sae)
 {
     if (int e = check_ctx(c)) return e;
+    if (c->adjoint_mode) return fail(c, "the discrete adjoint runs through sp_forward_ckpt and sp_adjoint_ckpt");
     if (sae <= 0 || c->nt % sae) return fail(c, "nt must be a positive multiple of adjoint_interval");
     if (!c->snaps_valid || c->snaps_sae != sae) return fail(c, "no snapshots for this adjoint_interval: run sp_forward(ctx, sae) first");
     for (auto &g : c->grids)
@@ -1686,6 +1799,7 @@ int sp_gradient(sp_ctx *c, int sae, double *norms_out)
     if (sae <= 0 || c->nt % sae) return fail(c, "nt must be a positive multiple of adjoint_interval");
     if (!norms_out) return fail(c, "norms_out is null");
     if (c->shots.empty()) return fail(c, "no shots");
+    if (c->adjoint_mode) return fail(c, "the discrete adjoint runs through sp_forward_ckpt and sp_adjoint_ckpt");
     std::vector<std::vector<int>> batches;
     if (plan_batches(c, sae, batches)) return 2;
     for (auto &g : c->grids)
@@ -1749,8 +1863,16 @@ int sp_adjoint_ckpt(sp_ctx *c, int sae)
         if (fill(c, g.kern, (long long)SP_NKERN * g.Npad, 0.f)) return 2;
     std::vector<int> all(c->shots.size());
     std::iota(all.begin(), all.end(), 0);
+    if (c->adjoint_mode == 1) {
+        if (c->physics == SP_SPIN) return fail(c, "the discrete adjoint covers SH and P-SV, not spin");
+        if (sae != 1) return fail(c, "the discrete adjoint takes every step: adjoint_interval must be 1");
+    }
     if (timing_begin(c)) return 2;
-    if (run_adjoint_ckpt_batch(c, all, sae, c->ckpt_C)) return 2;
+    if (c->adjoint_mode == 1) {
+        if (run_adjoint_exact_ckpt_batch(c, all, c->ckpt_C)) return 2;
+    } else if (run_adjoint_ckpt_batch(c, all, sae, c->ckpt_C)) {
+        return 2;
+    }
     c->snaps_valid = 0;
     return timing_end(c);
 }
@@ -1990,6 +2112,7 @@ int sp_get_option(const sp_ctx *c, int key, long long *value)
     case SP_OPT_STRIP: *value = ((long long)c->shift << 16) | c->wout; return 0;
     case SP_OPT_FUSE: *value = c->fuse; return 0;
     case SP_OPT_EXACT: *value = c->exact; return 0;
+    case SP_OPT_ADJOINT: *value = c->adjoint_mode; return 0;
     default: return 1;
     }
 }

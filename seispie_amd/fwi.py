@@ -55,15 +55,24 @@ class _Stream:
 class VpSolver:
     """Batched P-SV solver inverting vp (vs = vp / sqrt(3), rho fixed)."""
 
-    def __init__(self, prob, data, sigma=5, device=0, chunk=0, autotune=True):
+    def __init__(self, prob, data, sigma=5, device=0, chunk=0, autotune=True, adjoint="reference"):
         if prob.physics != _lib.PSV or len(prob.grids) != 1 or prob.grids[0].param != 0:
             raise ValueError("VpSolver needs a one-grid P-SV problem in vp/vs/rho")
+        if adjoint not in ("reference", "discrete"):
+            raise ValueError("adjoint must be 'reference' or 'discrete'")
         g = prob.grids[0]
         self.prob = prob
         self.nx, self.nz = g.nx, g.nz
         self.sae = prob.sae
         self.sigma = int(sigma)
+        self.adjoint = adjoint
         self.ctx = _lib.Context(device)
+        if adjoint == "discrete":
+            # the exact transpose of the time stepping (edge bands, sponge):
+            # the gradient also with sources and receivers beside a free surface
+            self.ctx.set_option(_lib.OPT_ADJOINT, 1)
+            c = max(2, int(round(math.sqrt(prob.nt))))
+            self.interval = c + (c & 1)
         if chunk:
             self.ctx.set_option(_lib.OPT_CHUNK, chunk)
         self.ids = configs.load_problem(self.ctx, prob)
@@ -94,7 +103,12 @@ class VpSolver:
         return float(self.ctx.misfit().sum())
 
     def compute_gradient(self):
-        norms = self.ctx.gradient(self.sae)
+        if self.adjoint == "discrete":
+            self.ctx.forward_ckpt(self.interval)
+            norms = self.ctx.misfit()
+            self.ctx.adjoint_ckpt(1)
+        else:
+            norms = self.ctx.gradient(self.sae)
         self.ctx.kernels_vps(0)
         g = self.ctx.kernel(0, _lib.K_VP).astype(np.float64) + self.ctx.kernel(0, _lib.K_VS) / SQ3
         g = g.astype(np.float32)
@@ -286,7 +300,7 @@ def observed(prob_true, device=0):
 
 
 def time_to_misfit(prob_true, prob_init, niter=20, train_steps=1500, method="lbfgs", device=0, sigma=5, log=print,
-                   data=None):
+                   data=None, adjoint="reference"):
     """Plain L-BFGS for niter iterations, then L-BFGS with the learned
     conditioner: the wall time the conditioned run needs to reach the plain
     run's final misfit, as a fraction of the plain run's wall time.  The
@@ -294,7 +308,7 @@ def time_to_misfit(prob_true, prob_init, niter=20, train_steps=1500, method="lbf
     receiver-side illumination run), so `time_to_misfit` includes training;
     `time_to_misfit_without_training` subtracts it."""
     data = observed(prob_true, device) if data is None else data
-    plain = VpSolver(prob_init, data, sigma=sigma, device=device)
+    plain = VpSolver(prob_init, data, sigma=sigma, device=device, adjoint=adjoint)
     opt = make_optimizer(plain, method, niter)
     opt.on_iteration = lambda i, t, f: log("  plain iteration %d: %.2f s, misfit %.5e" % (i, t, f))
     t0 = perf_counter()
@@ -306,7 +320,7 @@ def time_to_misfit(prob_true, prob_init, niter=20, train_steps=1500, method="lbf
         % (method, niter, hist_plain[0][1], f_target, t_plain, plain.n_gradients, plain.n_misfits))
     plain.close()
 
-    solver = VpSolver(prob_init, data, sigma=sigma, device=device)
+    solver = VpSolver(prob_init, data, sigma=sigma, device=device, adjoint=adjoint)
     pre = LearnedPrecond(solver, device=device, train_steps=train_steps)
     opt2 = make_optimizer(solver, method, niter, precond=pre)
     opt2.on_iteration = lambda i, t, f: log("  conditioned iteration %d: %.2f s, misfit %.5e" % (i, t, f))
