@@ -219,26 +219,37 @@ def _lib_spin():
     return _lib.SPIN
 
 
-def fwi_time_to_misfit(niter):
+def fwi_time_to_misfit(niter, surface=False):
     """Time-to-misfit of the learned conditioner against plain L-BFGS; a
-    failure is reported in the line instead of ending the bench."""
+    failure is reported in the line instead of ending the bench.  surface:
+    the free-surface acquisition (sources at 10 m, receivers at 2 dz, top
+    free) with the discrete adjoint and a 20-row gradient mute."""
     from seispie_amd import configs, fwi
     t0 = time.time()
     try:
-        true = configs.fwi_config5(true_model=True)
-        init = configs.fwi_config5(true_model=False)
+        true = configs.fwi_config5(true_model=True, surface=surface)
+        init = configs.fwi_config5(true_model=False, surface=surface)
+        kw = {"adjoint": "discrete", "mute_top": 20} if surface else {}
         r = fwi.time_to_misfit(true, init, niter=niter,
-                               log=lambda s: print("# " + s, file=sys.stderr, flush=True))
+                               log=lambda s: print("# " + s, file=sys.stderr, flush=True), **kw)
     except Exception as e:  # noqa: BLE001 -- reported, not hidden
         return {"error": "%s: %s" % (type(e).__name__, e)}
     g = init.grids[0]
     src_z = float(g.shots[0].src_k[0] % g.nz) * g.dz
     rec_z = float(g.shots[0].rec_k[0] % g.nz) * g.dz
+    if surface:
+        geometry = ("sources at %g m, receivers at %g m depth, top free, sponges on three sides; discrete adjoint "
+                    "(the exact transpose of the time stepping), gradient muted in the top 20 rows with a cosine "
+                    "ramp over the next 20" % (src_z, rec_z))
+    else:
+        geometry = ("sources at %g m, receivers at %g m depth, sponges on all four sides (width %d): the "
+                    "reference-style adjoint (forward operator reused, solver.py:308-321) is the misfit's "
+                    "gradient only away from the free surface's edge bands, so the acquisition sits below "
+                    "the top sponge instead of at 10 m under a free top" % (src_z, rec_z, g.sponge[0]))
     out = {"workload": init.name + " vp inversion, L-BFGS (lbfgs_mem 5, backtracking)", "iterations": niter,
-           "geometry": "sources at %g m, receivers at %g m depth, sponges on all four sides (width %d): the "
-                       "reference-style adjoint (forward operator reused, solver.py:308-321) is the misfit's "
-                       "gradient only away from the free surface's edge bands, so the acquisition sits below "
-                       "the top sponge instead of at 10 m under a free top" % (src_z, rec_z, g.sponge[0]),
+           "geometry": geometry,
+           "plain_history": [round(f, 4) for _, f in r["plain_history"]],
+           "conditioned_history": [round(f, 4) for _, f in r["learned_history"]],
            "value": None if r["time_to_misfit"] is None else round(r["time_to_misfit"], 4),
            "unit": "fraction of plain L-BFGS wall time, conditioner training included", "target": 0.6,
            "without_training": (None if r["time_to_misfit_without_training"] is None
@@ -368,6 +379,9 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the per-configuration block (configs 1, 2, 3b, 4, 5: three gradients each)")
     ap.add_argument("--fwi-iters", type=int, default=20)
+    ap.add_argument("--fwi-surface", action="store_true",
+                    help="time-to-misfit at the free-surface acquisition (discrete adjoint, 20-row gradient mute)")
+    ap.add_argument("--fwi-only", action="store_true", help="only the time-to-misfit measurement")
     ap.add_argument("--autograd", action="store_true",
                     help="time loss.backward() through the torch.autograd.Function (checkpointed adjoint) instead")
     ap.add_argument("--graph", action="store_true",
@@ -409,6 +423,9 @@ def main():
         return autograd_bench(true, init, args)
     if args.forward_only:
         return forward_only(true, init, args)
+    if args.fwi_only:
+        print(json.dumps({"time_to_misfit": fwi_time_to_misfit(args.fwi_iters, args.fwi_surface)}))
+        return 0
     r = time_gradients(true, init, args.steps, args.warmup, args)
     wall, value, cells, sae = r["wall"], r["value"], r["cells"], init.sae
 
@@ -458,7 +475,7 @@ def main():
 
     ttm = None
     if not args.no_fwi and not args.forward_only:
-        ttm = fwi_time_to_misfit(args.fwi_iters)
+        ttm = fwi_time_to_misfit(args.fwi_iters, args.fwi_surface)
 
     line = {
         "metric": "Gcell-upd/s (% HBM peak); time-to-misfit, learned vs plain precond",

@@ -212,14 +212,21 @@ def spin_config4(scale=1.0, nt=None, nshots=32):
     return Problem(_lib.SPIN, nt, dt, [g], sae=50, name="spin_%dx%d_%dshots_nt%d" % (n, n, nshots, nt))
 
 
-def fwi_config5_model(n=500, dx=5.0, true_model=True):
+def fwi_config5_model(n=500, dx=5.0, true_model=True, variant=0):
     """vp 3000 background with two +8% Gaussian blobs (sigma 60 m) over a
-    curved -8% arc; vs = vp/sqrt(3), rho 2600."""
+    curved -8% arc (variant 0, the benchmark's), or (variant 1, a second
+    model for the conditioner's ablation) a -8% and a +8% blob at other
+    places over a dipping +6% layer; vs = vp/sqrt(3), rho 2600."""
     x = np.arange(n)[:, None] * dx
     z = np.arange(n)[None, :] * dx
     L = (n - 1) * dx
     vp = np.full((n, n), 3000.0)
-    if true_model:
+    if true_model and variant == 1:
+        vp -= 240.0 * np.exp(-((x - 0.30 * L) ** 2 + (z - 0.55 * L) ** 2) / (2 * 60.0 ** 2))
+        vp += 240.0 * np.exp(-((x - 0.70 * L) ** 2 + (z - 0.30 * L) ** 2) / (2 * 50.0 ** 2))
+        dip = z - (0.65 * L + 0.15 * (x - 0.5 * L))
+        vp += 180.0 / (1.0 + np.exp(-dip / 20.0))
+    elif true_model:
         for cx, cz in ((0.35 * L, 0.40 * L), (0.65 * L, 0.40 * L)):
             vp += 240.0 * np.exp(-((x - cx) ** 2 + (z - cz) ** 2) / (2 * 60.0 ** 2))
         r = np.sqrt((x - 0.5 * L) ** 2 + (z - 0.30 * L) ** 2)
@@ -231,7 +238,7 @@ def fwi_config5_model(n=500, dx=5.0, true_model=True):
     return [vp, vs, rho]
 
 
-def fwi_config5(n=500, nt=3000, nshots=64, true_model=True, surface=False):
+def fwi_config5(n=500, nt=3000, nshots=64, true_model=True, surface=False, variant=0):
     """Learned-conditioner FWI: 500^2, dx 5 m, dt 0.5 ms, nt 3000, 64 shots of
     a 15 Hz Ricker, 200 receivers.  Default: sponges on all four sides,
     sources at 125 m and receivers at 130 m depth, just below the 20-cell
@@ -250,9 +257,10 @@ def fwi_config5(n=500, nt=3000, nshots=64, true_model=True, surface=False):
         shots = _geometry(_lib.PSV, n, n, 5.0, 5.0, nt, dt, nshots, min(200, n), 15.0, ang=30.0, src_z=125.0,
                           rec_z=130.0)
         sponge = (20, 0.015, 1, 1, 1, 1)
-    g = Grid(n, n, 5.0, 5.0, fwi_config5_model(n, 5.0, true_model), 0, sponge=sponge, shots=shots)
+    g = Grid(n, n, 5.0, 5.0, fwi_config5_model(n, 5.0, true_model, variant), 0, sponge=sponge, shots=shots)
     return Problem(_lib.PSV, nt, dt, [g], sae=50,
-                   name="fwi_%dx%d_%dshots_nt%d%s" % (n, n, nshots, nt, "_surface" if surface else ""))
+                   name="fwi_%dx%d_%dshots_nt%d%s%s" % (n, n, nshots, nt, "_surface" if surface else "",
+                                                       "_model%d" % variant if variant else ""))
 
 
 def load_problem(ctx, prob):

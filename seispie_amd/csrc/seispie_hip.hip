@@ -2042,7 +2042,7 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
             return 0;
         }
     }
-    const int chunks[] = {4, 8, 16, 32, 64, 128};
+    const int chunks[] = {4, 8, 16, 24, 32, 48, 64, 96, 128};
     // one or two steps per launch, and two steps with one or two cells per
     // lane where the batch allows it (all three give the same results)
     std::vector<int> fuses = {c->fuse};

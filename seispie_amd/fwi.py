@@ -55,7 +55,7 @@ class _Stream:
 class VpSolver:
     """Batched P-SV solver inverting vp (vs = vp / sqrt(3), rho fixed)."""
 
-    def __init__(self, prob, data, sigma=5, device=0, chunk=0, autotune=True, adjoint="reference"):
+    def __init__(self, prob, data, sigma=5, device=0, chunk=0, autotune=True, adjoint="reference", mute_top=0):
         if prob.physics != _lib.PSV or len(prob.grids) != 1 or prob.grids[0].param != 0:
             raise ValueError("VpSolver needs a one-grid P-SV problem in vp/vs/rho")
         if adjoint not in ("reference", "discrete"):
@@ -66,6 +66,14 @@ class VpSolver:
         self.sae = prob.sae
         self.sigma = int(sigma)
         self.adjoint = adjoint
+        # gradient mute below a free surface: zero in the top mute_top rows,
+        # a cosine ramp over the next mute_top rows (sources and receivers
+        # there make the misfit far from linear in the model at their cells)
+        self.mute = None
+        if mute_top > 0:
+            j = np.arange(self.nz, dtype=np.float64)
+            ramp = np.clip((j - mute_top) / mute_top, 0.0, 1.0)
+            self.mute = np.tile(0.5 - 0.5 * np.cos(np.pi * ramp), self.nx).astype(np.float32)
         self.ctx = _lib.Context(device)
         if adjoint == "discrete":
             # the exact transpose of the time stepping (edge bands, sponge):
@@ -114,6 +122,8 @@ class VpSolver:
         g = g.astype(np.float32)
         if self.sigma > 0:
             g = self.ctx.smooth(g, self.nx, self.nz, self.sigma)
+        if self.mute is not None:
+            g = g * self.mute
         self.hess = self.ctx.kernel(0, _lib.K_HESS)
         self.n_gradients += 1
         return g, float(norms.sum()), self.vp.copy()
@@ -171,16 +181,20 @@ class LearnedPrecond:
         self.fallbacks = 0
 
     # ----------------------------------------------------------- features
+    def _smooth(self, a):
+        """The solver's Gaussian smoothing (fd2d.py:71-108) on the device."""
+        out = self.solver.ctx.smooth(np.ascontiguousarray(a, np.float32).reshape(-1), self.nx, self.nz, self.smooth)
+        return out.reshape(self.nx, self.nz)
+
     def features(self, g, hess):
         g = np.asarray(g, np.float64).reshape(self.nx, self.nz)
         h = np.asarray(hess, np.float64).reshape(self.nx, self.nz)
-        from scipy.ndimage import gaussian_filter
         x = np.empty((self.net.channels[0], self.nx, self.nz), np.float32)
         x[0] = g / max(np.abs(g).max(), 1e-30)
         hn = h / max(h.max(), 1e-30)
         # smoothed like the target: the network's 9 x 9 receptive field
         # could not reproduce a Gaussian of this width itself
-        x[1] = gaussian_filter(np.log10(hn + 1e-6) / 6.0 + 1.0, self.smooth, mode="nearest")
+        x[1] = self._smooth(np.log10(hn + 1e-6) / 6.0 + 1.0)
         if x.shape[0] > 2:
             x[2] = (np.arange(self.nz) / max(self.nz - 1, 1))[None, :]
         return x
@@ -188,13 +202,7 @@ class LearnedPrecond:
     def target(self, hess, hrec):
         """log of the water-levelled inverse of the source x receiver
         illumination, smoothed, with its maximum at 0."""
-        from scipy.ndimage import gaussian_filter
-        h = np.asarray(hess, np.float64).reshape(self.nx, self.nz)
-        r = np.asarray(hrec, np.float64).reshape(self.nx, self.nz)
-        hr = (h / h.max()) * (r / r.max())
-        t = -np.log(hr + self.water)
-        t = gaussian_filter(t, self.smooth, mode="nearest")
-        return ((t - t.max()) / math.log(1.0 / self.water)).astype(np.float32)
+        return illumination_target(self, hess, hrec)
 
     # ----------------------------------------------------------- training
     def train(self, samples, steps=300, lr=2e-3, crop=128, seed=0, log_every=0):
@@ -265,6 +273,60 @@ class LearnedPrecond:
         self.net.close()
 
 
+def illumination_target(pre, hess, hrec):
+    """log of the water-levelled inverse of the source x receiver
+    illumination (hrec None: the source side alone), smoothed on the device,
+    scaled by ln(1/water) with its maximum at 0."""
+    h = np.asarray(hess, np.float64).reshape(pre.nx, pre.nz)
+    hr = h / h.max()
+    if hrec is not None:
+        r = np.asarray(hrec, np.float64).reshape(pre.nx, pre.nz)
+        hr = hr * (r / r.max())
+    t = pre._smooth(-np.log(hr + pre.water)).astype(np.float64)
+    return ((t - t.max()) / math.log(1.0 / pre.water)).astype(np.float32)
+
+
+class IlluminationPrecond:
+    """Non-learned step conditioners for the ablation of LearnedPrecond:
+    the same scale, computed directly each iteration instead of predicted.
+      mode "exact":  the network's training target, recomputed in the
+                     current model every iteration (one receiver-side
+                     illumination simulation per iteration);
+      mode "source": the source-side pseudo-Hessian alone (no extra run)."""
+
+    def __init__(self, solver, mode="exact", device=0, water=1e-4, smooth=5, smax=1e4):
+        if mode not in ("exact", "source"):
+            raise ValueError("mode must be 'exact' or 'source'")
+        self.solver, self.mode, self.device = solver, mode, device
+        self.nx, self.nz = solver.nx, solver.nz
+        self.water, self.smooth, self.smax = water, smooth, smax
+        self.scale = None
+        self.prep_seconds = 0.0
+        self.train_seconds = 0.0
+        self.train_loss = (0.0, 0.0)
+        self.fallbacks = 0
+
+    _smooth = LearnedPrecond._smooth
+
+    def refresh(self, g):
+        t0 = perf_counter()
+        hrec = None
+        if self.mode == "exact":
+            hrec = receiver_illumination(self.solver.prob, self.solver.vp, self.solver.rho, self.device)
+        y = illumination_target(self, self.solver.hess, hrec).astype(np.float64) * math.log(1.0 / self.water)
+        s = np.exp(np.clip(y, -math.log(self.smax), 0.0)).reshape(-1)
+        self.scale = s if np.all(np.isfinite(s)) else None
+        if self.scale is None:
+            self.fallbacks += 1
+        self.prep_seconds += perf_counter() - t0
+
+    def __call__(self, q):
+        return q if self.scale is None else self.scale * q
+
+    def close(self):
+        pass
+
+
 def make_optimizer(solver, method="lbfgs", niter=20, precond=None, quiet=True):
     """An optimizer of seispie_amd.optimize with the reference example's
     settings (examples/inverse-checker/config.ini:14-21)."""
@@ -300,7 +362,7 @@ def observed(prob_true, device=0):
 
 
 def time_to_misfit(prob_true, prob_init, niter=20, train_steps=1500, method="lbfgs", device=0, sigma=5, log=print,
-                   data=None, adjoint="reference"):
+                   data=None, adjoint="reference", mute_top=0):
     """Plain L-BFGS for niter iterations, then L-BFGS with the learned
     conditioner: the wall time the conditioned run needs to reach the plain
     run's final misfit, as a fraction of the plain run's wall time.  The
@@ -308,7 +370,7 @@ def time_to_misfit(prob_true, prob_init, niter=20, train_steps=1500, method="lbf
     receiver-side illumination run), so `time_to_misfit` includes training;
     `time_to_misfit_without_training` subtracts it."""
     data = observed(prob_true, device) if data is None else data
-    plain = VpSolver(prob_init, data, sigma=sigma, device=device, adjoint=adjoint)
+    plain = VpSolver(prob_init, data, sigma=sigma, device=device, adjoint=adjoint, mute_top=mute_top)
     opt = make_optimizer(plain, method, niter)
     opt.on_iteration = lambda i, t, f: log("  plain iteration %d: %.2f s, misfit %.5e" % (i, t, f))
     t0 = perf_counter()
@@ -320,7 +382,7 @@ def time_to_misfit(prob_true, prob_init, niter=20, train_steps=1500, method="lbf
         % (method, niter, hist_plain[0][1], f_target, t_plain, plain.n_gradients, plain.n_misfits))
     plain.close()
 
-    solver = VpSolver(prob_init, data, sigma=sigma, device=device, adjoint=adjoint)
+    solver = VpSolver(prob_init, data, sigma=sigma, device=device, adjoint=adjoint, mute_top=mute_top)
     pre = LearnedPrecond(solver, device=device, train_steps=train_steps)
     opt2 = make_optimizer(solver, method, niter, precond=pre)
     opt2.on_iteration = lambda i, t, f: log("  conditioned iteration %d: %.2f s, misfit %.5e" % (i, t, f))
