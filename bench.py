@@ -179,8 +179,10 @@ def autograd_bench(true, init, args):
         raise SystemExit("--autograd needs a one-grid SH or P-SV workload")
     g = init.grids[0]
     data = observed_data(true)
+    adj = "discrete" if args.discrete else "reference"
     sim = autograd.Simulation(init.physics, g.nx, g.nz, g.dx, g.dz, init.nt, init.dt,
-                              [(s.src_k, s.stf, s.rec_k) for s in g.shots], sponge=g.sponge, sae=init.sae)
+                              [(s.src_k, s.stf, s.rec_k) for s in g.shots], sponge=g.sponge,
+                              sae=1 if args.discrete else init.sae, adjoint=adj)
     dev = sim.device
     d = torch.tensor(np.stack([x[:sim.nmis] for x in data]), device=dev)
     m = [torch.tensor(f, device=dev, requires_grad=True) for f in g.fields]
@@ -207,7 +209,7 @@ def autograd_bench(true, init, args):
         "ms_per_step": round(wall / args.steps * 1e3, 2), "higher_is_better": True, "dtype": "f32 storage, f64 arithmetic",
         "data": "synthetic", "loss": float(loss.detach()),
         "config": {"workload": init.name + " waveform misfit, loss.backward()", "shots": init.nshots, "nt": init.nt,
-                   "adjoint_interval": init.sae, "checkpoint_interval": sim.interval,
+                   "adjoint": adj, "adjoint_interval": sim.sae, "checkpoint_interval": sim.interval,
                    "note": "cell updates count forward and adjoint once; the forward recomputed from "
                            "checkpoints in backward is not counted"}}))
     sim.close()
@@ -384,6 +386,8 @@ def main():
     ap.add_argument("--fwi-only", action="store_true", help="only the time-to-misfit measurement")
     ap.add_argument("--autograd", action="store_true",
                     help="time loss.backward() through the torch.autograd.Function (checkpointed adjoint) instead")
+    ap.add_argument("--discrete", action="store_true",
+                    help="with --autograd: the discrete adjoint (exact transpose of the time stepping, every step)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the forward and adjoint runs from captured hipGraphs")
     ap.add_argument("--no-fuse", action="store_true",

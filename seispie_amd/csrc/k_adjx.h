@@ -10,17 +10,17 @@
 // state (s^, v^, u^) runs backwards from step nt-1:
 //     u^ += R^T g(n)                                   k_xrec
 //     q = dt u^ + B^T e,  e = dt C s^                  k_xw (e from k_xs)
-//     v^ = b (v^ + q),  y = dt v^ / rho                k_xw
-//     s^ += A^T y,  then e = dt C s^ for step n-1      k_xs
+//     v^ = b (v^ + q)                                  k_xw
+//     s^ += A^T y, y = dt v^ / rho, then e = dt C s^   k_xs
 // and, with s^ before this step's update and w from the forward snapshot,
-//     dL/dmu, dL/dlam += dt s^ . (dC/dm) B w,   dL/drho -= w . q / rho.
+//     dL/dmu, dL/dlam += dt s^ . (dC/dm) B w,   dL/drho -= w . q / rho   k_xg
 // Kernels are kept as dt dL/dm, the convention of the reference-style ones.
 // The transposes take each neighbour only where its row of D-/D+ is valid:
 //     (D-^T y)_j = c1 (y_j - y_{j+1}) - c2 (y_{j-1} - y_{j+2}),  rows 2 <= i < n-2
 //     (D+^T y)_j = c1 (y_{j-1} - y_j) - c2 (y_{j-2} - y_{j+1}),  rows 1 <= i < n-2
 // Storage is float32 in the shot's state block, arithmetic float64:
 //     s^: stress set 0   v^: velocity set 0   u^: displacement
-//     e : stress set 1   y : velocity set 1
+//     e : stress set 1   q : velocity set 1
 
 __device__ __forceinline__ double tminus(const float *y, long long k, long long st, int i, int n, double c1, double c2)
 {
@@ -82,11 +82,47 @@ struct XAdjArgs {
     double dt, c1x, c2x, c1z, c2z;
 };
 
-// One thread per grid cell, every shot of the grid in turn: q, the new v^
-// and y, and the kernels' step-n terms (summed over the grid's shots in
-// float64, then added to the float32 kernels once).
+// q = dt u^ + B^T e and v^ = b (v^ + q); q is kept (velocity set 1) for the
+// density kernel.  One thread per cell and shot.
 template <int P>
-__global__ __launch_bounds__(256) void k_xw(XAdjArgs a)
+__global__ __launch_bounds__(256) void k_xw(const ShotDesc *shots, int nshots, long long maxN, double dt)
+{
+#pragma clang fp contract(off)
+    constexpr int NV = n_v(P);
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int s = blockIdx.y;
+    if (s >= nshots) return;
+    const ShotDesc &d = shots[s];
+    const int nx = d.nx, nz = d.nz;
+    if (k >= (long long)nx * nz) return;
+    const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
+    const long long NP = d.Npad;
+    float *st = d.state;
+    const float b1 = (float)((double)d.bx[i] * d.bzb[j]);
+    const double b = (double)(float)((double)b1 * d.bzt[j]);
+    const float *e0 = st + slot_s(P, 1, 0) * NP, *e1 = st + slot_s(P, 1, 1) * NP;
+    double bt[NV];
+    if constexpr (P == SP_SH) {
+        bt[0] = tplus(e0, k, nz, i, nx, d.c1x, d.c2x) + tplus(e1, k, 1, j, nz, d.c1z, d.c2z);
+    } else {
+        const float *e2 = st + slot_s(P, 1, 2) * NP;
+        bt[0] = tplus(e0, k, nz, i, nx, d.c1x, d.c2x) + tplus(e2, k, 1, j, nz, d.c1z, d.c2z);
+        bt[1] = tplus(e1, k, 1, j, nz, d.c1z, d.c2z) + tplus(e2, k, nz, i, nx, d.c1x, d.c2x);
+    }
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        const double q = dt * (double)st[slot_u(P, c) * NP + k] + bt[c];
+        float *vh = st + slot_v(P, 0, c) * NP + k;
+        *vh = (float)(b * ((double)*vh + q));
+        st[slot_v(P, 1, c) * NP + k] = (float)q;
+    }
+}
+
+// The kernels' step-n terms: one thread per grid cell, the grid's shots in
+// turn (summed in float64, then added to the float32 kernels once).  Only
+// loads inside the shot loop, so the loads of successive shots overlap.
+template <int P>
+__global__ __launch_bounds__(256) void k_xg(XAdjArgs a)
 {
 #pragma clang fp contract(off)
     constexpr int NV = n_v(P);
@@ -100,30 +136,14 @@ __global__ __launch_bounds__(256) void k_xw(XAdjArgs a)
     double glam = 0.0, gmu = 0.0, grho = 0.0, ghess = 0.0;
     for (int t = 0; t < a.nlist; t++) {
         const ShotDesc &d = a.shots[a.list[t]];
-        float *st = d.state;
+        const float *st = d.state;
         const float *w = d.snap + ((long long)a.slot * 2 * NV + NV) * NP;  // v after step n
         const double rho = d.mat[M_RHO * NP + k];
-        const float b1 = (float)((double)d.bx[i] * d.bzb[j]);
-        const double b = (double)(float)((double)b1 * d.bzt[j]);
-        const float *e0 = st + slot_s(P, 1, 0) * NP, *e1 = st + slot_s(P, 1, 1) * NP;
-        double bt[NV];
-        if constexpr (P == SP_SH) {
-            bt[0] = tplus(e0, k, nz, i, nx, a.c1x, a.c2x) + tplus(e1, k, 1, j, nz, a.c1z, a.c2z);
-        } else {
-            const float *e2 = st + slot_s(P, 1, 2) * NP;
-            bt[0] = tplus(e0, k, nz, i, nx, a.c1x, a.c2x) + tplus(e2, k, 1, j, nz, a.c1z, a.c2z);
-            bt[1] = tplus(e1, k, 1, j, nz, a.c1z, a.c2z) + tplus(e2, k, nz, i, nx, a.c1x, a.c2x);
-        }
         double wq = 0.0;
 #pragma unroll
         for (int c = 0; c < NV; c++) {
-            const double q = dt * (double)st[slot_u(P, c) * NP + k] + bt[c];
-            float *vh = st + slot_v(P, 0, c) * NP + k;
-            const float vn = (float)(b * ((double)*vh + q));
-            *vh = vn;
-            st[slot_v(P, 1, c) * NP + k] = (float)(dt * (double)vn / rho);
             const double wc = w[c * NP + k];
-            wq += wc * q;
+            wq += wc * (double)st[slot_v(P, 1, c) * NP + k];
             ghess += wc * wc;
         }
         grho -= wq / rho;
@@ -147,8 +167,18 @@ __global__ __launch_bounds__(256) void k_xw(XAdjArgs a)
     K[SP_K_HESS * NP + k] = (float)((double)K[SP_K_HESS * NP + k] + dt * ghess);
 }
 
-// s^ += A^T y, then e = dt C s^ for the next (earlier) step; one thread per
-// cell and shot.
+// D-^T of y = dt v^ / rho, taken on the fly from v^ and rho at the neighbours
+__device__ __forceinline__ double tminus_y(const float *vh, const float *rho, long long k, long long st, int i, int n,
+                                           double c1, double c2, double dt)
+{
+#define YV(q, kk) (((q) >= 2 && (q) < n - 2) ? dt * (double)vh[kk] / (double)rho[kk] : 0.0)
+    const double a = YV(i, k), b = YV(i + 1, k + st), cm = YV(i - 1, k - st), d = YV(i + 2, k + 2 * st);
+#undef YV
+    return c1 * (a - b) - c2 * (cm - d);
+}
+
+// s^ += A^T (dt v^ / rho), then e = dt C s^ for the next (earlier) step; one
+// thread per cell and shot.
 template <int P>
 __global__ __launch_bounds__(256) void k_xs(const ShotDesc *shots, int nshots, long long maxN, double dt)
 {
@@ -162,22 +192,23 @@ __global__ __launch_bounds__(256) void k_xs(const ShotDesc *shots, int nshots, l
     const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
     const long long NP = d.Npad;
     float *st = d.state;
-    const float *yx = st + slot_v(P, 1, 0) * NP;
+    const float *rho = d.mat + M_RHO * NP;
+    const float *vx = st + slot_v(P, 0, 0) * NP;
     const double mu = d.mat[M_MU * NP + k];
     if constexpr (P == SP_SH) {
-        const double s0 = (double)st[slot_s(P, 0, 0) * NP + k] + tminus(yx, k, nz, i, nx, d.c1x, d.c2x);
-        const double s1 = (double)st[slot_s(P, 0, 1) * NP + k] + tminus(yx, k, 1, j, nz, d.c1z, d.c2z);
+        const double s0 = (double)st[slot_s(P, 0, 0) * NP + k] + tminus_y(vx, rho, k, nz, i, nx, d.c1x, d.c2x, dt);
+        const double s1 = (double)st[slot_s(P, 0, 1) * NP + k] + tminus_y(vx, rho, k, 1, j, nz, d.c1z, d.c2z, dt);
         const float f0 = (float)s0, f1 = (float)s1;
         st[slot_s(P, 0, 0) * NP + k] = f0;
         st[slot_s(P, 0, 1) * NP + k] = f1;
         st[slot_s(P, 1, 0) * NP + k] = (float)(dt * mu * (double)f0);
         st[slot_s(P, 1, 1) * NP + k] = (float)(dt * mu * (double)f1);
     } else {
-        const float *yz = st + slot_v(P, 1, 1) * NP;
-        const double s0 = (double)st[slot_s(P, 0, 0) * NP + k] + tminus(yx, k, nz, i, nx, d.c1x, d.c2x);
-        const double s1 = (double)st[slot_s(P, 0, 1) * NP + k] + tminus(yz, k, 1, j, nz, d.c1z, d.c2z);
-        const double s2 = (double)st[slot_s(P, 0, 2) * NP + k] + tminus(yx, k, 1, j, nz, d.c1z, d.c2z) +
-                          tminus(yz, k, nz, i, nx, d.c1x, d.c2x);
+        const float *vz = st + slot_v(P, 0, 1) * NP;
+        const double s0 = (double)st[slot_s(P, 0, 0) * NP + k] + tminus_y(vx, rho, k, nz, i, nx, d.c1x, d.c2x, dt);
+        const double s1 = (double)st[slot_s(P, 0, 1) * NP + k] + tminus_y(vz, rho, k, 1, j, nz, d.c1z, d.c2z, dt);
+        const double s2 = (double)st[slot_s(P, 0, 2) * NP + k] + tminus_y(vx, rho, k, 1, j, nz, d.c1z, d.c2z, dt) +
+                          tminus_y(vz, rho, k, nz, i, nx, d.c1x, d.c2x, dt);
         const float f0 = (float)s0, f1 = (float)s1, f2 = (float)s2;
         st[slot_s(P, 0, 0) * NP + k] = f0;
         st[slot_s(P, 0, 1) * NP + k] = f1;

@@ -1188,6 +1188,8 @@ int run_adjoint_exact_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C
         for (int it = hi - 1; it >= lo; it--) {
             if (P == SP_SH) hipLaunchKernelGGL(k_xrec<SP_SH>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
             else hipLaunchKernelGGL(k_xrec<SP_PSV>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
+            if (P == SP_SH) hipLaunchKernelGGL(k_xw<SP_SH>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            else hipLaunchKernelGGL(k_xw<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
             for (size_t g = 0; g < pl.per_grid.size(); g++) {
                 if (pl.per_grid[g].empty()) continue;
                 const Grid &G = c->grids[g];
@@ -1206,8 +1208,8 @@ int run_adjoint_exact_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C
                 a.c1z = 9.0 / (8.0 * (double)G.dz);
                 a.c2z = 1.0 / (24.0 * (double)G.dz);
                 const dim3 gr((unsigned)((G.N + 255) / 256));
-                if (P == SP_SH) hipLaunchKernelGGL(k_xw<SP_SH>, gr, dim3(256), 0, c->stream, a);
-                else hipLaunchKernelGGL(k_xw<SP_PSV>, gr, dim3(256), 0, c->stream, a);
+                if (P == SP_SH) hipLaunchKernelGGL(k_xg<SP_SH>, gr, dim3(256), 0, c->stream, a);
+                else hipLaunchKernelGGL(k_xg<SP_PSV>, gr, dim3(256), 0, c->stream, a);
             }
             if (P == SP_SH) hipLaunchKernelGGL(k_xs<SP_SH>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
             else hipLaunchKernelGGL(k_xs<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
