@@ -82,7 +82,7 @@ typedef struct sp_ctx sp_ctx;
                                operator reused with the adjoint sources injected at the receivers,
                                solver.py:290-321, kernels correlated every adjoint_interval steps); 1: the
                                discrete adjoint, the exact transpose of the forward time stepping with its
-                               edge bands and sponge (SH and P-SV, adjoint_interval 1, K = dt dL/dm exactly:
+                               edge bands and sponge (SH, P-SV and spin, adjoint_interval 1, K = dt dL/dm exactly:
                                the gradient also beside a free surface and inside sponges, where the
                                reference-style kernels are not).  sp_gradient and sp_adjoint refuse 1. */
 

@@ -80,7 +80,7 @@ def lib():
             f.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 3 + [ctypes.c_int] + [vp] * 7
             f.restype = ctypes.c_int
             f = getattr(L, "orc_adjoint_exact_" + sfx)
-            f.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 7
+            f.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Geom)] + [vp] * 10
             f.restype = ctypes.c_int
             f = getattr(L, "orc_waveform_" + sfx)
             f.argtypes = [vp] * 4 + [ctypes.c_int, ctypes.c_double, ctypes.c_int]
@@ -263,22 +263,26 @@ class Sim:
 
     def adjoint_exact(self, grec, snaps, precision="f64"):
         """Discrete adjoint (the exact transpose of the time stepping, edge
-        bands and sponge included; SH and P-SV).  grec: dict comp -> dL/d
-        records [nrec, nt] (not time-reversed); snaps: forward(sae=1)
-        snapshots.  Returns dict lam, mu, rho of dt * dL/dm."""
+        bands and sponge included).  grec: dict comp -> dL/d records [nrec,
+        nt] (not time-reversed; spin may include the rotation "y");
+        snaps: forward(sae=1) snapshots.  Returns dict lam, mu, rho (and,
+        for spin, nu, j, muc: the mu_c + nu_c kernel) of dt * dL/dm."""
         dt_ = np.float32 if precision == "f32" else np.float64
         n, nt, nrec = self.nx * self.nz, self.nt, self.rec_id.size
         z = np.zeros(nrec * nt, dtype=dt_)
         gr = [np.ascontiguousarray(grec.get(c, z), dtype=dt_).reshape(-1) for c in ("x", "y", "z")]
-        kern = {k: np.zeros(n, dtype=dt_) for k in ("lam", "mu", "rho")}
+        names = ("lam", "mu", "rho", "nu", "j", "muc")
+        kern = {k: np.zeros(n, dtype=dt_) for k in names}
         snaps = np.ascontiguousarray(snaps, dtype=dt_)
         assert snaps.shape[0] == nt, "snapshots of every step (sae = 1)"
         M, G = self._structs()
         rc = getattr(lib(), "orc_adjoint_exact_" + precision)(ctypes.byref(M), ctypes.byref(G), _p(gr[0]), _p(gr[1]),
-                                                                _p(gr[2]), _p(snaps), _p(kern["lam"]),
-                                                                _p(kern["mu"]), _p(kern["rho"]))
+                                                                _p(gr[2]), _p(snaps), *[_p(kern[k]) for k in names])
         if rc != 0:
-            raise ValueError("the discrete adjoint covers SH and P-SV")
+            raise ValueError("orc_adjoint_exact failed")
+        if self.physics != SPIN:
+            for k in ("nu", "j", "muc"):
+                del kern[k]
         return kern
 
     def cpu_bench(self, stf, warm, nsteps, nthreads, L=None):

@@ -464,41 +464,45 @@ static inline double FN(tp)(const double *y, long k, long st, int i, int n, doub
     return c1 * (a - b) - c2 * (cm - d);
 }
 
-/* Discrete adjoint of the forward run (SH and P-SV): the exact transpose of
- * the time-stepping map of solver.py:205-234 (rounding aside), run backwards
- * from step nt-1.  With v(n+1) = w = b (v(n) + dt (A s(n) + f) / rho),
- * u(n+1) = u(n) + dt w, s(n+1) = s(n) + dt C B w and records R u(n+1), the
- * adjoint state (s^, v^, u^) steps as
+/* Discrete adjoint of the forward run (SH, P-SV and spin): the exact
+ * transpose of the time-stepping map of solver.py:205-234 (rounding aside),
+ * run backwards from step nt-1.  With v(n+1) = w = b (v(n) + dt (A s(n) +
+ * f) / M), u(n+1) = u(n) + dt w, s(n+1) = s(n) + dt C B w and records
+ * R u(n+1), the adjoint state (s^, v^, u^) steps as
  *     u^ += R^T g(n);  e = dt C s^;  q = dt u^ + B^T e;  w^ = v^ + q;
- *     v^ = b w^;  s^ += A^T (dt v^ / rho)
+ *     v^ = b w^;  s^ += A^T (dt v^ / M)
  * (B: D+ strains; A: D- divergence; transposes with their edge bands), and
  * the gradient of L (g = dL/d records, [nrec][nt] per component, not time
  * reversed) accumulates, before s^ takes this step's update,
- *     dL/dmu  += dt s^ . dC/dmu B w,   dL/dlam += dt s^ . dC/dlam B w,
- *     dL/drho -= w . q / rho.
+ *     dL/dm += dt s^ . dC/dm B w   (lam, mu; spin: nu, mu_c + nu_c),
+ *     dL/drho -= (w . q)_xz / rho,  spin: dL/dj -= w_y q_y / j.
+ * Spin: v = (vx, vz, vy_c), M = (rho, rho, j); s adds syx_c, syy_c, syz_c
+ * with B rows Dx+ vy_c, -vy_c - (Dx+ vz - Dz+ vx)/2, Dz+ vy_c and
+ * stiffness mu_c + nu_c, 2 nu, mu_c + nu_c; A adds Dz- syy_c to the x row,
+ * -Dx- syy_c to the z row and Dx- syx_c + Dz- syz_c + 2 syy_c to the y_c
+ * row (solver.py:69-75, 135-141).
  * The kernels returned are dt dL/dm (the convention of the reference-style
  * kernels).  vsnap: the forward snapshots of orc_forward with sae = 1
  * ([nt][nsnap][N], reversed: v after step n in slot nt-1-n).  Arithmetic
  * in float64 on T storage. */
 int FN(orc_adjoint_exact)(const orc_model *m, const orc_geom *g, const T *gx, const T *gy, const T *gz,
-                          const T *vsnap, T *klam, T *kmu, T *krho)
+                          const T *vsnap, T *klam, T *kmu, T *krho, T *knu, T *kj, T *kmuc)
 {
-    int nx = m->nx, nz = m->nz, nt = m->nt, phys = m->physics, it, r, c;
+    int nx = m->nx, nz = m->nz, nt = m->nt, phys = m->physics, it, r;
     long n = (long)nx * nz, k;
-    int NV = phys == ORC_SH ? 1 : 2, NS = phys == ORC_SH ? 2 : 3, nf = orc_nsnap(phys);
+    int NV = phys == ORC_SH ? 1 : (phys == ORC_PSV ? 2 : 3);
+    int NS = phys == ORC_SH ? 2 : (phys == ORC_PSV ? 3 : 6), nf = orc_nsnap(phys);
     double dx = (double)m->dx, dz = (double)m->dz, dt = m->dt;
     double c1x = 9.0 / (8.0 * dx), c2x = 1.0 / (24.0 * dx), c1z = 9.0 / (8.0 * dz), c2z = 1.0 / (24.0 * dz);
-    double *sh, *vh, *uh, *e, *q, *y;
-    if (phys != ORC_SH && phys != ORC_PSV) return -2;
+    double *sh, *vh, *uh, *e, *y;
     sh = (double *)calloc((size_t)(NS * n), sizeof(double));
     e = (double *)calloc((size_t)(NS * n), sizeof(double));
     vh = (double *)calloc((size_t)(NV * n), sizeof(double));
     uh = (double *)calloc((size_t)(NV * n), sizeof(double));
-    q = (double *)calloc((size_t)(NV * n), sizeof(double));
     y = (double *)calloc((size_t)(NV * n), sizeof(double));
     for (it = nt - 1; it >= 0; it--) {
         const T *w = vsnap + (long)(nt - 1 - it) * nf * n + (long)NV * n;  /* v after step it */
-        /* records of step it (u after it) */
+        /* records of step it (u after it): SH y; P-SV x, z; spin x, z, y_c */
         for (r = 0; r < g->nrec; r++) {
             long kr = g->rec_id[r];
             if (phys == ORC_SH) {
@@ -506,6 +510,7 @@ int FN(orc_adjoint_exact)(const orc_model *m, const orc_geom *g, const T *gx, co
             } else {
                 uh[kr] += (double)gx[(long)r * nt + it];
                 uh[n + kr] += (double)gz[(long)r * nt + it];
+                if (phys == ORC_SPIN && gy) uh[2 * n + kr] += (double)gy[(long)r * nt + it];
             }
         }
         /* e = dt C s^ */
@@ -520,9 +525,15 @@ int FN(orc_adjoint_exact)(const orc_model *m, const orc_geom *g, const T *gx, co
                 e[k] = dt * (l2m * sh[k] + lam * sh[n + k]);
                 e[n + k] = dt * (lam * sh[k] + l2m * sh[n + k]);
                 e[2 * n + k] = dt * mu * sh[2 * n + k];
+                if (phys == ORC_SPIN) {
+                    double gam = (double)(T)(m->mu_c[k] + m->nu_c[k]), nu = (double)m->nu[k];
+                    e[3 * n + k] = dt * gam * sh[3 * n + k];
+                    e[4 * n + k] = dt * 2.0 * nu * sh[4 * n + k];
+                    e[5 * n + k] = dt * gam * sh[5 * n + k];
+                }
             }
         }
-        /* gradient of mu (and lam) from s^ before its update and B w */
+        /* kernels from s^ before its update and B w */
 #pragma omp parallel for schedule(static)
         for (k = 0; k < n; k++) {
             int j = (int)(k % nz), i = (int)(k / nz);
@@ -531,32 +542,50 @@ int FN(orc_adjoint_exact)(const orc_model *m, const orc_geom *g, const T *gx, co
                 kmu[k] = (T)((double)kmu[k] + dt * dt * (sh[k] * ex + sh[n + k] * ez));
             } else {
                 double exx = FN(dxp)(w, i, k, dx, nx, nz), ezz = FN(dzp)(w + n, j, k, dz, nz);
-                double gxz = FN(dzp)(w, j, k, dz, nz) + FN(dxp)(w + n, i, k, dx, nx, nz);
+                double dxz = FN(dzp)(w, j, k, dz, nz), dzx = FN(dxp)(w + n, i, k, dx, nx, nz);
                 klam[k] = (T)((double)klam[k] + dt * dt * (sh[k] + sh[n + k]) * (exx + ezz));
-                kmu[k] = (T)((double)kmu[k] + dt * dt * (2.0 * sh[k] * exx + 2.0 * sh[n + k] * ezz + sh[2 * n + k] * gxz));
+                kmu[k] = (T)((double)kmu[k]
+                             + dt * dt * (2.0 * sh[k] * exx + 2.0 * sh[n + k] * ezz + sh[2 * n + k] * (dxz + dzx)));
+                if (phys == ORC_SPIN) {
+                    const T *wy = w + 2 * n;
+                    double ee = -(double)wy[k] - 0.5 * (dzx - dxz);
+                    double cx = FN(dxp)(wy, i, k, dx, nx, nz), cz = FN(dzp)(wy, j, k, dz, nz);
+                    knu[k] = (T)((double)knu[k] + dt * dt * 2.0 * sh[4 * n + k] * ee);
+                    kmuc[k] = (T)((double)kmuc[k] + dt * dt * (sh[3 * n + k] * cx + sh[5 * n + k] * cz));
+                }
             }
         }
-        /* q = dt u^ + B^T e, w^ = v^ + q, v^ = b w^; rho gradient */
+        /* q = dt u^ + B^T e, w^ = v^ + q, v^ = b w^; rho (and j) gradient */
 #pragma omp parallel for schedule(static)
         for (k = 0; k < n; k++) {
             int j = (int)(k % nz), i = (int)(k / nz), cc;
-            double bt[2], acc = 0.0, rho = (double)m->rho[k];
+            double bt[3], acc = 0.0, accj = 0.0, rho = (double)m->rho[k];
             if (phys == ORC_SH) {
                 bt[0] = FN(tp)(e, k, nz, i, nx, c1x, c2x) + FN(tp)(e + n, k, 1, j, nz, c1z, c2z);
             } else {
                 bt[0] = FN(tp)(e, k, nz, i, nx, c1x, c2x) + FN(tp)(e + 2 * n, k, 1, j, nz, c1z, c2z);
                 bt[1] = FN(tp)(e + n, k, 1, j, nz, c1z, c2z) + FN(tp)(e + 2 * n, k, nz, i, nx, c1x, c2x);
+                if (phys == ORC_SPIN) {
+                    /* E = -vy_c - (Dx+ vz - Dz+ vx)/2; curvatures Dx+ vy_c, Dz+ vy_c */
+                    bt[0] += 0.5 * FN(tp)(e + 4 * n, k, 1, j, nz, c1z, c2z);
+                    bt[1] -= 0.5 * FN(tp)(e + 4 * n, k, nz, i, nx, c1x, c2x);
+                    bt[2] = -e[4 * n + k] + FN(tp)(e + 3 * n, k, nz, i, nx, c1x, c2x)
+                          + FN(tp)(e + 5 * n, k, 1, j, nz, c1z, c2z);
+                }
             }
             for (cc = 0; cc < NV; cc++) {
+                double mass = cc == 2 ? (double)m->jj[k] : rho;
                 double qq = dt * uh[cc * n + k] + bt[cc];
                 double wh = vh[cc * n + k] + qq;
-                acc += (double)w[cc * n + k] * qq;
+                if (cc == 2) accj += (double)w[cc * n + k] * qq;
+                else acc += (double)w[cc * n + k] * qq;
                 vh[cc * n + k] = (double)m->bound[k] * wh;
-                y[cc * n + k] = dt * vh[cc * n + k] / rho;
+                y[cc * n + k] = dt * vh[cc * n + k] / mass;
             }
             krho[k] = (T)((double)krho[k] - dt * acc / rho);
+            if (phys == ORC_SPIN) kj[k] = (T)((double)kj[k] - dt * accj / (double)m->jj[k]);
         }
-        /* s^ += A^T (dt v^ / rho) */
+        /* s^ += A^T (dt v^ / M) */
 #pragma omp parallel for schedule(static)
         for (k = 0; k < n; k++) {
             int j = (int)(k % nz), i = (int)(k / nz);
@@ -567,11 +596,16 @@ int FN(orc_adjoint_exact)(const orc_model *m, const orc_geom *g, const T *gx, co
                 sh[k] += FN(tm)(y, k, nz, i, nx, c1x, c2x);
                 sh[n + k] += FN(tm)(y + n, k, 1, j, nz, c1z, c2z);
                 sh[2 * n + k] += FN(tm)(y, k, 1, j, nz, c1z, c2z) + FN(tm)(y + n, k, nz, i, nx, c1x, c2x);
+                if (phys == ORC_SPIN) {
+                    sh[3 * n + k] += FN(tm)(y + 2 * n, k, nz, i, nx, c1x, c2x);
+                    sh[4 * n + k] += FN(tm)(y, k, 1, j, nz, c1z, c2z) - FN(tm)(y + n, k, nz, i, nx, c1x, c2x)
+                                   + 2.0 * y[2 * n + k];
+                    sh[5 * n + k] += FN(tm)(y + 2 * n, k, 1, j, nz, c1z, c2z);
+                }
             }
         }
     }
-    (void)c;
-    free(sh); free(e); free(vh); free(uh); free(q); free(y);
+    free(sh); free(e); free(vh); free(uh); free(y);
     return 0;
 }
 

@@ -31,8 +31,8 @@ the reference's adjoint does (solver.py:308-321), so next to a free surface,
 where the difference operators' edge bands make D- differ from -(D+)^T, it
 is not the gradient of L either.
 
-adjoint="discrete" (SH and P-SV) makes the backward the library's discrete
-adjoint instead: the exact transpose of the forward time stepping, edge
+adjoint="discrete" (SH, P-SV and spin) makes the backward the library's
+discrete adjoint instead: the exact transpose of the forward time stepping, edge
 bands and sponge included, correlated every step (sae = 1), so K / dt is
 dL/dm up to float32 rounding anywhere in the grid, free surface and sponges
 included.
@@ -74,8 +74,6 @@ class Simulation:
         if adjoint not in ("reference", "discrete"):
             raise ValueError("adjoint must be 'reference' or 'discrete'")
         if adjoint == "discrete":
-            if physics == _lib.SPIN:
-                raise ValueError("the discrete adjoint covers SH and P-SV")
             if sae != 1:
                 raise ValueError("the discrete adjoint correlates every step: sae must be 1")
         self.adjoint = adjoint

@@ -1,19 +1,25 @@
-// seispie_hip -- the discrete adjoint (SH and P-SV): the exact transpose of
-// the forward time stepping, edge bands and sponge included.
+// seispie_hip -- the discrete adjoint (SH, P-SV and spin): the exact
+// transpose of the forward time stepping, edge bands and sponge included.
 // Part of seispie_hip.hip: included there, inside its anonymous namespace,
 // after <hip/hip_runtime.h> and the headers before this one.
 #pragma once
 
-// Forward step n (solver.py:205-234):  w = b (v + dt (A s + f) / rho),
+// Forward step n (solver.py:205-234):  w = b (v + dt (A s + f) / M),
 // u' = u + dt w, s' = s + dt C B w, records R u'  (A: D- divergence, B: D+
-// strains, C: the pointwise stiffness, b: the sponge factor).  The adjoint
+// strains, C: the pointwise stiffness, M: rho (and j for the spin velocity),
+// b: the sponge factor).  Spin adds syx_c, syy_c, syz_c with B rows
+// Dx+ vy_c, -vy_c - (Dx+ vz - Dz+ vx)/2, Dz+ vy_c and stiffness mu_c + nu_c,
+// 2 nu, mu_c + nu_c; A adds Dz- syy_c to the x row, -Dx- syy_c to the z row
+// and Dx- syx_c + Dz- syz_c + 2 syy_c to the vy_c row (solver.py:69-75,
+// 135-141).  The adjoint
 // state (s^, v^, u^) runs backwards from step nt-1:
 //     u^ += R^T g(n)                                   k_xrec
 //     q = dt u^ + B^T e,  e = dt C s^                  k_xw (e from k_xs)
 //     v^ = b (v^ + q)                                  k_xw
-//     s^ += A^T y, y = dt v^ / rho, then e = dt C s^   k_xs
+//     s^ += A^T y, y = dt v^ / M, then e = dt C s^     k_xs
 // and, with s^ before this step's update and w from the forward snapshot,
-//     dL/dmu, dL/dlam += dt s^ . (dC/dm) B w,   dL/drho -= w . q / rho   k_xg
+//     dL/dm += dt s^ . (dC/dm) B w  (lam, mu; spin nu, mu_c + nu_c),
+//     dL/drho -= (w . q)_xz / rho,  spin dL/dj -= w_y q_y / j              k_xg
 // Kernels are kept as dt dL/dm, the convention of the reference-style ones.
 // The transposes take each neighbour only where its row of D-/D+ is valid:
 //     (D-^T y)_j = c1 (y_j - y_{j+1}) - c2 (y_{j-1} - y_{j+2}),  rows 2 <= i < n-2
@@ -108,6 +114,13 @@ __global__ __launch_bounds__(256) void k_xw(const ShotDesc *shots, int nshots, l
         const float *e2 = st + slot_s(P, 1, 2) * NP;
         bt[0] = tplus(e0, k, nz, i, nx, d.c1x, d.c2x) + tplus(e2, k, 1, j, nz, d.c1z, d.c2z);
         bt[1] = tplus(e1, k, 1, j, nz, d.c1z, d.c2z) + tplus(e2, k, nz, i, nx, d.c1x, d.c2x);
+        if constexpr (P == SP_SPIN) {
+            const float *e3 = st + slot_s(P, 1, 3) * NP, *e4 = st + slot_s(P, 1, 4) * NP;
+            const float *e5 = st + slot_s(P, 1, 5) * NP;
+            bt[0] += 0.5 * tplus(e4, k, 1, j, nz, d.c1z, d.c2z);
+            bt[1] -= 0.5 * tplus(e4, k, nz, i, nx, d.c1x, d.c2x);
+            bt[2] = -(double)e4[k] + tplus(e3, k, nz, i, nx, d.c1x, d.c2x) + tplus(e5, k, 1, j, nz, d.c1z, d.c2z);
+        }
     }
 #pragma unroll
     for (int c = 0; c < NV; c++) {
@@ -133,7 +146,7 @@ __global__ __launch_bounds__(256) void k_xg(XAdjArgs a)
     const int i = (int)(k / nz), j = (int)(k - (long long)i * nz);
     const long long NP = a.Npad;
     const double dt = a.dt;
-    double glam = 0.0, gmu = 0.0, grho = 0.0, ghess = 0.0;
+    double glam = 0.0, gmu = 0.0, grho = 0.0, ghess = 0.0, gnu = 0.0, gj = 0.0, gmuc = 0.0;
     for (int t = 0; t < a.nlist; t++) {
         const ShotDesc &d = a.shots[a.list[t]];
         const float *st = d.state;
@@ -141,12 +154,14 @@ __global__ __launch_bounds__(256) void k_xg(XAdjArgs a)
         const double rho = d.mat[M_RHO * NP + k];
         double wq = 0.0;
 #pragma unroll
-        for (int c = 0; c < NV; c++) {
+        for (int c = 0; c < (P == SP_SPIN ? 2 : NV); c++) {
             const double wc = w[c * NP + k];
             wq += wc * (double)st[slot_v(P, 1, c) * NP + k];
             ghess += wc * wc;
         }
         grho -= wq / rho;
+        if constexpr (P == SP_SPIN)
+            gj -= (double)w[2 * NP + k] * (double)st[slot_v(P, 1, 2) * NP + k] / (double)d.mat[M_J * NP + k];
         const double s0 = st[slot_s(P, 0, 0) * NP + k], s1 = st[slot_s(P, 0, 1) * NP + k];
         if constexpr (P == SP_SH) {
             const double ex = fplus(w, k, nz, i, nx, a.c1x, a.c2x), ez = fplus(w, k, 1, j, nz, a.c1z, a.c2z);
@@ -155,9 +170,17 @@ __global__ __launch_bounds__(256) void k_xg(XAdjArgs a)
             const double s2 = st[slot_s(P, 0, 2) * NP + k];
             const float *wz = w + NP;
             const double exx = fplus(w, k, nz, i, nx, a.c1x, a.c2x), ezz = fplus(wz, k, 1, j, nz, a.c1z, a.c2z);
-            const double g = fplus(w, k, 1, j, nz, a.c1z, a.c2z) + fplus(wz, k, nz, i, nx, a.c1x, a.c2x);
+            const double dxz = fplus(w, k, 1, j, nz, a.c1z, a.c2z), dzx = fplus(wz, k, nz, i, nx, a.c1x, a.c2x);
             glam += dt * (s0 + s1) * (exx + ezz);
-            gmu += dt * (2.0 * s0 * exx + 2.0 * s1 * ezz + s2 * g);
+            gmu += dt * (2.0 * s0 * exx + 2.0 * s1 * ezz + s2 * (dxz + dzx));
+            if constexpr (P == SP_SPIN) {
+                const float *wy = w + 2 * NP;
+                const double s3 = st[slot_s(P, 0, 3) * NP + k], s4 = st[slot_s(P, 0, 4) * NP + k];
+                const double s5 = st[slot_s(P, 0, 5) * NP + k];
+                const double ee = -(double)wy[k] - 0.5 * (dzx - dxz);
+                gnu += dt * 2.0 * s4 * ee;
+                gmuc += dt * (s3 * fplus(wy, k, nz, i, nx, a.c1x, a.c2x) + s5 * fplus(wy, k, 1, j, nz, a.c1z, a.c2z));
+            }
         }
     }
     float *K = a.kern;
@@ -165,6 +188,11 @@ __global__ __launch_bounds__(256) void k_xg(XAdjArgs a)
     K[SP_K_MU * NP + k] = (float)((double)K[SP_K_MU * NP + k] + dt * gmu);
     K[SP_K_RHO * NP + k] = (float)((double)K[SP_K_RHO * NP + k] + dt * grho);
     K[SP_K_HESS * NP + k] = (float)((double)K[SP_K_HESS * NP + k] + dt * ghess);
+    if constexpr (P == SP_SPIN) {
+        K[SP_K_NU * NP + k] = (float)((double)K[SP_K_NU * NP + k] + dt * gnu);
+        K[SP_K_J * NP + k] = (float)((double)K[SP_K_J * NP + k] + dt * gj);
+        K[SP_K_MUC * NP + k] = (float)((double)K[SP_K_MUC * NP + k] + dt * gmuc);
+    }
 }
 
 // D-^T of y = dt v^ / rho, taken on the fly from v^ and rho at the neighbours
@@ -217,5 +245,21 @@ __global__ __launch_bounds__(256) void k_xs(const ShotDesc *shots, int nshots, l
         st[slot_s(P, 1, 0) * NP + k] = (float)(dt * (l2m * (double)f0 + lam * (double)f1));
         st[slot_s(P, 1, 1) * NP + k] = (float)(dt * (lam * (double)f0 + l2m * (double)f1));
         st[slot_s(P, 1, 2) * NP + k] = (float)(dt * mu * (double)f2);
+        if constexpr (P == SP_SPIN) {
+            const float *vy = st + slot_v(P, 0, 2) * NP, *jj = d.mat + M_J * NP;
+            const double s3 = (double)st[slot_s(P, 0, 3) * NP + k] + tminus_y(vy, jj, k, nz, i, nx, d.c1x, d.c2x, dt);
+            const double s4 = (double)st[slot_s(P, 0, 4) * NP + k] + tminus_y(vx, rho, k, 1, j, nz, d.c1z, d.c2z, dt) -
+                              tminus_y(vz, rho, k, nz, i, nx, d.c1x, d.c2x, dt) +
+                              2.0 * (dt * (double)vy[k] / (double)jj[k]);
+            const double s5 = (double)st[slot_s(P, 0, 5) * NP + k] + tminus_y(vy, jj, k, 1, j, nz, d.c1z, d.c2z, dt);
+            const float f3 = (float)s3, f4 = (float)s4, f5 = (float)s5;
+            st[slot_s(P, 0, 3) * NP + k] = f3;
+            st[slot_s(P, 0, 4) * NP + k] = f4;
+            st[slot_s(P, 0, 5) * NP + k] = f5;
+            const double gam = d.mat[M_GAM * NP + k], nu = d.mat[M_NU * NP + k];
+            st[slot_s(P, 1, 3) * NP + k] = (float)(dt * gam * (double)f3);
+            st[slot_s(P, 1, 4) * NP + k] = (float)(dt * 2.0 * nu * (double)f4);
+            st[slot_s(P, 1, 5) * NP + k] = (float)(dt * gam * (double)f5);
+        }
     }
 }

@@ -1187,9 +1187,11 @@ int run_adjoint_exact_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C
         HIPCHECK(c, hipGetLastError());
         for (int it = hi - 1; it >= lo; it--) {
             if (P == SP_SH) hipLaunchKernelGGL(k_xrec<SP_SH>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
-            else hipLaunchKernelGGL(k_xrec<SP_PSV>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
+            else if (P == SP_PSV) hipLaunchKernelGGL(k_xrec<SP_PSV>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
+            else hipLaunchKernelGGL(k_xrec<SP_SPIN>, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, it, nt);
             if (P == SP_SH) hipLaunchKernelGGL(k_xw<SP_SH>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
-            else hipLaunchKernelGGL(k_xw<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            else if (P == SP_PSV) hipLaunchKernelGGL(k_xw<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            else hipLaunchKernelGGL(k_xw<SP_SPIN>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
             for (size_t g = 0; g < pl.per_grid.size(); g++) {
                 if (pl.per_grid[g].empty()) continue;
                 const Grid &G = c->grids[g];
@@ -1209,10 +1211,12 @@ int run_adjoint_exact_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C
                 a.c2z = 1.0 / (24.0 * (double)G.dz);
                 const dim3 gr((unsigned)((G.N + 255) / 256));
                 if (P == SP_SH) hipLaunchKernelGGL(k_xg<SP_SH>, gr, dim3(256), 0, c->stream, a);
-                else hipLaunchKernelGGL(k_xg<SP_PSV>, gr, dim3(256), 0, c->stream, a);
+                else if (P == SP_PSV) hipLaunchKernelGGL(k_xg<SP_PSV>, gr, dim3(256), 0, c->stream, a);
+                else hipLaunchKernelGGL(k_xg<SP_SPIN>, gr, dim3(256), 0, c->stream, a);
             }
             if (P == SP_SH) hipLaunchKernelGGL(k_xs<SP_SH>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
-            else hipLaunchKernelGGL(k_xs<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            else if (P == SP_PSV) hipLaunchKernelGGL(k_xs<SP_PSV>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
+            else hipLaunchKernelGGL(k_xs<SP_SPIN>, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, c->dt);
             HIPCHECK(c, hipGetLastError());
         }
     }
@@ -1865,10 +1869,7 @@ int sp_adjoint_ckpt(sp_ctx *c, int sae)
         if (fill(c, g.kern, (long long)SP_NKERN * g.Npad, 0.f)) return 2;
     std::vector<int> all(c->shots.size());
     std::iota(all.begin(), all.end(), 0);
-    if (c->adjoint_mode == 1) {
-        if (c->physics == SP_SPIN) return fail(c, "the discrete adjoint covers SH and P-SV, not spin");
-        if (sae != 1) return fail(c, "the discrete adjoint takes every step: adjoint_interval must be 1");
-    }
+    if (c->adjoint_mode == 1 && sae != 1) return fail(c, "the discrete adjoint takes every step: adjoint_interval must be 1");
     if (timing_begin(c)) return 2;
     if (c->adjoint_mode == 1) {
         if (run_adjoint_exact_ckpt_batch(c, all, c->ckpt_C)) return 2;
