@@ -276,6 +276,8 @@ def _context(args):
         ctx.set_option(_lib.OPT_FUSE, 0)
     if args.fuse >= 0:
         ctx.set_option(_lib.OPT_FUSE, args.fuse)
+    if args.group:
+        ctx.set_option(_lib.OPT_GROUP, args.group)
     if args.fast:
         ctx.set_option(_lib.OPT_EXACT, 0)
     if args.graph:
@@ -323,7 +325,8 @@ def time_gradients(true, init, steps, warmup, args, quiet=False):
     tuned = None
     if not args.chunk:
         ch, od = ctx.autotune(20)
-        tuned = {"chunk": ch, "order": od, "steps_per_launch_option": ctx.get_option(_lib.OPT_FUSE)}
+        tuned = {"chunk": ch, "order": od, "steps_per_launch_option": ctx.get_option(_lib.OPT_FUSE),
+                 "group": ctx.get_option(_lib.OPT_GROUP)}
         say("# autotuned %s" % tuned)
 
     def gradient():
@@ -350,19 +353,38 @@ def time_gradients(true, init, steps, warmup, args, quiet=False):
     launch_ms = ft["total_ms"] / ft["step_launches"]
     bytes_per_launch = ft["alg_bytes"] / ft["step_launches"]
     steps_per_launch = bytes_per_launch / init.alg_bytes_per_step()
-    wide = ctx.get_option(_lib.OPT_FUSE) == 2 and init.physics != _lib.SPIN and all(g.nz % 2 == 0 for g in init.grids)
+    mode = step_mode(ctx.get_option(_lib.OPT_FUSE), init.physics, [g.nz for g in init.grids])
+    wide = mode in (2, 4)
     if steps_per_launch > 1.5:
-        kname = ("k_step2<%s, W=2> (two time steps per launch, two cells per lane)" if wide
-                 else "k_step2<%s, W=1> (two time steps per launch)")
+        kname = {1: "k_step2<%s, W=1> (two time steps per launch)",
+                 2: "k_step2<%s, W=2> (two time steps per launch, two cells per lane)",
+                 3: "k_step2g<%s, W=1> (two time steps per launch, column groups of %d wavefronts)",
+                 4: "k_step2g<%s, W=2> (two time steps per launch, column groups of %d wavefronts, "
+                    "two cells per lane)"}[mode]
     else:
         kname = "k_step3<%s>"
-    kname = kname % {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]
+    pname = {1: "SH", 2: "P-SV", 3: "spin"}[init.physics]
+    kname = kname % ((pname, ctx.get_option(_lib.OPT_GROUP) or 4) if "%d" in kname else pname)
     ctx.close()
     cells = init.cell_updates(passes=2)
     return {"wall": wall, "value": cells * steps / wall / 1e9, "cells": cells, "launch_ms": launch_ms,
             "bytes_per_launch": bytes_per_launch, "achieved": bytes_per_launch / (launch_ms * 1e-3),
-            "steps_per_launch": steps_per_launch, "wide": wide, "kname": kname, "tuned": tuned,
+            "steps_per_launch": steps_per_launch, "wide": wide, "mode": mode, "kname": kname, "tuned": tuned,
             "pass_frac": round(init.alg_bytes_per_step() * init.nt * 2 * steps / wall / HBM_PEAK, 4)}
+
+
+def step_mode(fuse, physics, nzs):
+    """The two-step launch a batch gets for an SP_OPT_FUSE value: 1, 2
+    strips of one or two cells per lane, 3, 4 column groups of one or two
+    cells per lane, falling back as the library does."""
+    from seispie_amd import _lib
+    if fuse == 3:
+        return 3
+    if fuse == 4 and physics != _lib.SPIN and all(nz % 2 == 0 for nz in nzs):
+        return 4
+    if fuse >= 2 and physics != _lib.SPIN and all(nz % 2 == 0 for nz in nzs):
+        return 2
+    return 1 if fuse else 0
 
 
 def main():
@@ -394,7 +416,10 @@ def main():
                     help="one time step per launch (the single-step kernel) instead of two where allowed")
     ap.add_argument("--fuse", type=int, default=-1,
                     help="steps per launch option: 0 one step, 1 two steps (k_step2), 2 two steps with two "
-                         "cells per lane (k_step2 with W = 2; SH and P-SV with even nz)")
+                         "cells per lane (k_step2 with W = 2; SH and P-SV with even nz), 3, 4 two steps in column "
+                         "groups with one or two cells per lane (k_step2g; nz <= 512 or 1024)")
+    ap.add_argument("--group", type=int, default=0,
+                    help="column groups (--fuse 3, 4): wavefronts per block (default 4)")
     ap.add_argument("--fast", action="store_true",
                     help="step kernels in fast mode (reciprocal multiplications and fused multiply-adds) instead "
                          "of the default exact mode, whose records equal the oracle bit for bit")
@@ -437,7 +462,8 @@ def main():
     tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tf):
         t = json.load(open(tf)).get(init.name.rsplit("_nt", 1)[0]
-                                    + (("_wide" if r["wide"] else "_fused") if r["steps_per_launch"] > 1.5 else "")
+                                    + ({1: "_fused", 2: "_wide", 3: "_group", 4: "_group_wide"}[r["mode"]]
+                                       if r["steps_per_launch"] > 1.5 else "")
                                     + ("_fast" if args.fast else ""))
         if t:
             traffic = t["hbm_bytes_per_launch"]

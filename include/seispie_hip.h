@@ -69,8 +69,12 @@ typedef struct sp_ctx sp_ctx;
 #define SP_OPT_FUSE 7       /* 1 (default): shots advance two steps per launch wherever no snapshot,
                                kernel interaction or checkpoint needs the state between them; 2: the same
                                with two z cells per lane (SH and P-SV batches whose grids all have even nz;
-                               others use 1); 0: one step.  All give the same results bit for bit; the
-                               autotuner picks among them */
+                               others use 1); 3: two steps per launch in column groups, blocks whose
+                               wavefronts march one span of a shot's z column together and pass the cells
+                               at their edges through LDS, so halo cells are kept only at a span's ends
+                               inside the column (SP_OPT_GROUP wavefronts per block); 4: column groups with
+                               two z cells per lane (SH and P-SV with even nz, else as 2); 0: one step.  All
+                               give the same results bit for bit; the autotuner picks among them */
 #define SP_OPT_EXACT 8      /* 1 (default): the step kernels perform the reference's float64 operations one
                                by one in its order (correctly rounded quotients, no fused multiply-add), so
                                records equal the reference arithmetic bit for bit at any length; 0:
@@ -85,6 +89,8 @@ typedef struct sp_ctx sp_ctx;
                                edge bands and sponge (SH, P-SV and spin, adjoint_interval 1, K = dt dL/dm exactly:
                                the gradient also beside a free surface and inside sponges, where the
                                reference-style kernels are not).  sp_gradient and sp_adjoint refuse 1. */
+#define SP_OPT_GROUP 10     /* column groups (SP_OPT_FUSE 3, 4): wavefronts per block, 1..8 (0, the default:
+                               4); a span covers 64 x that many x (1 or 2) cells */
 
 /* Context and device.  Replaces fd2d.setup (fd2d.py:111-120). */
 int sp_create(int device, sp_ctx **out);
@@ -198,8 +204,9 @@ int sp_smooth(sp_ctx *ctx, float *field, int nx, int nz, int sigma, int on_devic
 int sp_smooth_noalloc(sp_ctx *ctx, float *field_dev, int nx, int nz, int sigma);
 
 /* Autotuner: times `steps` forward steps of the current shots for each
- * rows-per-wavefront chunk, work order and one or two steps per
- * launch, keeps the fastest for later runs and reports it; the choice is
+ * rows-per-wavefront chunk, work order and one or two steps per launch, in
+ * strips or in column groups of 2, 4 or 8 wavefronts (SP_OPT_FUSE,
+ * SP_OPT_GROUP), keeps the fastest for later runs and reports it; the choice is
  * remembered for the process and reused for the same physics and grids.  Overwrites wavefields and records (every run clears
  * them first).  No reference counterpart (the reference launches one
  * thread per cell with threads_per_block from the config, fd2d.py:185-187). */

@@ -11,7 +11,8 @@ import numpy as np
 SH, PSV, SPIN = 1, 2, 3
 F_RHO, F_LAM, F_MU, F_NU, F_J, F_LAMC, F_MUC, F_NUC = range(8)
 K_LAM, K_MU, K_RHO, K_NU, K_J, K_MUC, K_VP, K_VS, K_RHOP, K_HESS = range(10)
-OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP, OPT_FUSE, OPT_EXACT, OPT_ADJOINT = range(1, 10)
+OPT_CHUNK, OPT_ORDER, OPT_GRAPH, OPT_SNAP_MB, OPT_KERNEL, OPT_STRIP, OPT_FUSE, OPT_EXACT, OPT_ADJOINT, OPT_GROUP = \
+    range(1, 11)
 WF_V, WF_U, WF_S = 0, 1, 2
 
 N_REC = {SH: 1, PSV: 2, SPIN: 3}   # record components: SH y; P-SV x, z; spin x, z, y

@@ -139,17 +139,104 @@ __device__ __forceinline__ void zplus(float (&m1)[W], float (&p1)[W], float (&p2
     }
 }
 
+// ------------------------------------------------- column groups
+// In a column group the wavefronts of one block march the same rows of one
+// shot and together cover a span of its z column, wavefront g holding cells
+// j0 + 64 W g .. j0 + 64 W (g + 1) - 1.  A z neighbour across a wavefront
+// boundary is the neighbouring wavefront's edge cell, passed through LDS
+// instead of recomputed in halo lanes, so only the span's ends inside the
+// column keep halo cells.
+//
+// Cells next to a wavefront's edges: l0, l1 are the two cells before its
+// first cell (the previous wavefront's last two), r0, r1 the two after its
+// last cell (the next wavefront's first two).
+struct Nbr {
+    float l0, l1, r0, r1;
+};
+
+// the fields whose z neighbours the stencils take: stresses in div_sy /
+// div_sxz / div_sxyz_c (szy; szz, sxz; and syy_c, syz_c for spin) and
+// velocities in div_vy / div_vxz (vy; vx, vz; and vy_c) (solver.py:54-75,
+// 112-126)
+constexpr int n_zs(int p) { return p == SP_SH ? 1 : (p == SP_PSV ? 2 : 4); }
+constexpr int n_zv(int p) { return p == SP_SH ? 1 : (p == SP_PSV ? 2 : 3); }
+// exchanged rows per iteration: s(n), v(n+1), s(n+1), v(n+2)
+constexpr int n_xch(int p) { return 2 * n_zs(p) + 2 * n_zv(p); }
+// wavefronts per block: at most 8, so a kernel may use 256 VGPRs (with 16,
+// 128 VGPRs make the one-cell P-SV march spill)
+constexpr int grp_max(int, int) { return 8; }
+
+// D- neighbours (z-2, z-1, z+1) as zminus, the lanes at the wavefront's
+// edges (lf: first lane, ll: last lane) taking them from nb.  Every
+// rotation runs first, with all 64 lanes active, and the edge values are
+// selected afterwards: a rotation inside a conditional would run with the
+// edge lane switched off, and its neighbour would read a disabled lane.
+template <int W>
+__device__ __forceinline__ void zminus_g(float (&m2)[W], float (&m1)[W], float (&p1)[W], const float (&x)[W],
+                                         const Nbr &nb, bool lf, bool ll)
+{
+    if constexpr (W == 1) {
+        const float rm = lane_m1(x[0]), rp = lane_p1(x[0]);
+        m1[0] = lf ? nb.l1 : rm;
+        const float rmm = lane_m1(m1[0]);
+        m2[0] = lf ? nb.l0 : rmm;
+        p1[0] = ll ? nb.r0 : rp;
+    } else {
+        const float ra = lane_m1(x[0]), rb = lane_m1(x[1]), rp = lane_p1(x[0]);
+        const float a_m = lf ? nb.l0 : ra, b_m = lf ? nb.l1 : rb;
+        const float a_p = ll ? nb.r0 : rp;
+        m2[0] = a_m;  m1[0] = b_m;  p1[0] = x[1];
+        m2[1] = b_m;  m1[1] = x[0]; p1[1] = a_p;
+    }
+}
+// D+ neighbours (z-1, z+1, z+2) as zplus
+template <int W>
+__device__ __forceinline__ void zplus_g(float (&m1)[W], float (&p1)[W], float (&p2)[W], const float (&x)[W],
+                                        const Nbr &nb, bool lf, bool ll)
+{
+    if constexpr (W == 1) {
+        const float rm = lane_m1(x[0]), rp = lane_p1(x[0]);
+        m1[0] = lf ? nb.l1 : rm;
+        p1[0] = ll ? nb.r0 : rp;
+        const float rpp = lane_p1(p1[0]);
+        p2[0] = ll ? nb.r1 : rpp;
+    } else {
+        const float rb = lane_m1(x[1]), ra = lane_p1(x[0]), rc = lane_p1(x[1]);
+        const float b_m = lf ? nb.l1 : rb;
+        const float a_p = ll ? nb.r0 : ra, b_p = ll ? nb.r1 : rc;
+        m1[0] = b_m;  p1[0] = x[1]; p2[0] = a_p;
+        m1[1] = x[0]; p1[1] = a_p;  p2[1] = b_p;
+    }
+}
+// strips (G false) or column groups (G true; nb[k] for the k-th exchanged field)
+template <int W, bool G>
+__device__ __forceinline__ void zm(float (&m2)[W], float (&m1)[W], float (&p1)[W], const float (&x)[W],
+                                   const Nbr *nb, int k, bool lf, bool ll)
+{
+    if constexpr (G) zminus_g<W>(m2, m1, p1, x, nb[k], lf, ll);
+    else zminus<W>(m2, m1, p1, x);
+}
+template <int W, bool G>
+__device__ __forceinline__ void zp(float (&m1)[W], float (&p1)[W], float (&p2)[W], const float (&x)[W],
+                                   const Nbr *nb, int k, bool lf, bool ll)
+{
+    if constexpr (G) zplus_g<W>(m1, p1, p2, x, nb[k], lf, ll);
+    else zplus<W>(m1, p1, p2, x);
+}
+
 // stress divergence of one velocity row (div_sy / div_sxz, solver.py:54-66)
-// from the stress window rows r-2..r+1 (sa..sd)
-template <int P, bool EX, int W>
+// from the stress window rows r-2..r+1 (sa..sd); column groups take the
+// z neighbours of sc's fields 1, 2, 4, 5 across wavefront edges from nb
+template <int P, bool EX, int W, bool G = false>
 __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&sa)[n_s(P)][W],
                                         const float (&sb)[n_s(P)][W], const float (&sc)[n_s(P)][W],
                                         const float (&sd)[n_s(P)][W], bool xm_ok, const bool (&zm_ok)[W],
-                                        const DCo &KX, const DCo &KZ)
+                                        const DCo &KX, const DCo &KZ, const Nbr *nb = nullptr, bool lf = false,
+                                        bool ll = false)
 {
     if constexpr (P == SP_SH) {
         float zm2[W], zm1[W], zp1[W];
-        zminus<W>(zm2, zm1, zp1, sc[1]);
+        zm<W, G>(zm2, zm1, zp1, sc[1], nb, 0, lf, ll);
 #pragma unroll
         for (int c = 0; c < W; c++) {
             const double ddx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
@@ -158,8 +245,8 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&s
         }
     } else {
         float xzm2[W], xzm1[W], xzp1[W], zzm2[W], zzm1[W], zzp1[W];
-        zminus<W>(xzm2, xzm1, xzp1, sc[2]);
-        zminus<W>(zzm2, zzm1, zzp1, sc[1]);
+        zm<W, G>(xzm2, xzm1, xzp1, sc[2], nb, 1, lf, ll);
+        zm<W, G>(zzm2, zzm1, zzp1, sc[1], nb, 0, lf, ll);
 #pragma unroll
         for (int c = 0; c < W; c++) {
             const double dxx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
@@ -172,8 +259,8 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&s
         if constexpr (P == SP_SPIN) {
             // div_sy on the couple stresses, then div_sxyz_c (solver.py:69-75, 217-218)
             float yzm2[W], yzm1[W], yzp1[W], yym2[W], yym1[W], yyp1[W];
-            zminus<W>(yzm2, yzm1, yzp1, sc[5]);
-            zminus<W>(yym2, yym1, yyp1, sc[4]);
+            zm<W, G>(yzm2, yzm1, yzp1, sc[5], nb, 3, lf, ll);
+            zm<W, G>(yym2, yym1, yyp1, sc[4], nb, 2, lf, ll);
 #pragma unroll
             for (int c = 0; c < W; c++) {
                 const double dxyx = xm_ok ? dmn<EX>(sa[3][c], sb[3][c], sc[3][c], sd[3][c], KX) : 0.0;
@@ -241,18 +328,20 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
 }
 
 // stress update of one row (add_sy / add_sxz, solver.py:120-148) from the
-// velocity window rows q-1..q+2 (wa..wd) and the old stresses
-template <int P, bool EX, int W>
+// velocity window rows q-1..q+2 (wa..wd) and the old stresses; column
+// groups take the z neighbours of wb's fields 0, 1, 2 from nb
+template <int P, bool EX, int W, bool G = false>
 __device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa)[n_v(P)][W],
                                         const float (&wb)[n_v(P)][W], const float (&wc)[n_v(P)][W],
                                         const float (&wd)[n_v(P)][W], const float (&so)[n_s(P)][W],
                                         const float (&lam)[W], const float (&mu)[W], const float (&nu)[W],
                                         const float (&gam)[W], bool xp_ok, const bool (&zp_ok)[W], const DCo &KX,
-                                        const DCo &KZ, double dt)
+                                        const DCo &KZ, double dt, const Nbr *nb = nullptr, bool lf = false,
+                                        bool ll = false)
 {
     if constexpr (P == SP_SH) {
         float zm1[W], zp1[W], zp2[W];
-        zplus<W>(zm1, zp1, zp2, wb[0]);
+        zp<W, G>(zm1, zp1, zp2, wb[0], nb, 0, lf, ll);
 #pragma unroll
         for (int c = 0; c < W; c++) {
             const float dvdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
@@ -263,10 +352,10 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa
         }
     } else {
         float xm1[W], xp1[W], xp2[W], zm1[W], zp1[W], zp2[W];
-        zplus<W>(xm1, xp1, xp2, wb[0]);
-        zplus<W>(zm1, zp1, zp2, wb[1]);
+        zp<W, G>(xm1, xp1, xp2, wb[0], nb, 0, lf, ll);
+        zp<W, G>(zm1, zp1, zp2, wb[1], nb, 1, lf, ll);
         float cm1[W], cp1[W], cp2[W];
-        if constexpr (P == SP_SPIN) zplus<W>(cm1, cp1, cp2, wb[2]);
+        if constexpr (P == SP_SPIN) zp<W, G>(cm1, cp1, cp2, wb[2], nb, 2, lf, ll);
 #pragma unroll
         for (int c = 0; c < W; c++) {
             const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
@@ -317,15 +406,27 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, in
     src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
 }
 
-template <int P, bool EX, int W>
-__global__ __launch_bounds__(256) void k_step2(StepArgs a)
+// The march of one work item.  Strips (G false): four independent
+// wavefronts per block, each a strip with its own halo lanes.  Column
+// groups (G true): a block is one work item, its wavefronts cover 64 W
+// cells each of the shot's z column from cell item.y on and exchange edge
+// cells through xl, the LDS ring [4 iterations][grp_max][n_xch][4 cells]:
+// the rows a wavefront computes in iteration m go to slot m & 3, and the
+// stencils of iteration m read the neighbours' rows of iterations m-1 and
+// m-2, after one barrier per iteration.  Groups that do not reach the
+// column's ends keep 6 halo cells there, as strips do; wavefronts past the
+// column's last cell take part in the barriers only.
+template <int P, bool EX, int W, bool G>
+__device__ __forceinline__ void step2_body(StepArgs a, float *xl)
 {
     static_assert(W == 1 || (W == 2 && P != SP_SPIN), "two cells per lane: SH and P-SV");
     constexpr int NS = n_s(P), NV = n_v(P);
+    constexpr int GM = grp_max(P, W), NX = n_xch(P), NZS = n_zs(P), NZV = n_zv(P);
     const int nb = gridDim.x, bb = blockIdx.x;
     const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
     const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
-    const int wave = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
+    const int gw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wavefront of the block
+    const int wave = __builtin_amdgcn_readfirstlane(G ? lb : lb * 4 + gw);
     if (wave >= a.nitems) return;
     const int L = threadIdx.x & 63;
     const int4 item = a.items[2 * wave];
@@ -341,10 +442,38 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
         snap_u = ((cint_p)a.snap_tab)[itx + 1];
     }
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
-    const int j = item.y - a.shift + W * L;  // the lane's first cell
+    // the march starts at m0 = i0 - 3 and ends at mend (one barrier per
+    // iteration in column groups)
+    const int m0 = i0 - 3, mend = i1 + 5;
+    if constexpr (G) {
+        if (item.y + 64 * W * gw >= nz) {
+            for (int m = m0; m <= mend; m++) __syncthreads();
+            return;
+        }
+    }
+    // the lane's first cell (column groups: the group starts at cell item.y)
+    const int j = G ? item.y + W * (64 * gw + L) : item.y - a.shift + W * L;
     const int jc = min(max(j, 0), nz - W);   // the first cell it loads
     const bool jin = j >= 0 && j < nz;       // every cell of the lane (nz is a multiple of W)
-    const bool outl = jin && W * L >= a.shift && W * L < a.shift + a.wout;
+    // column groups: a group's output cells are [glo, ghi), 6 halo cells
+    // kept at each of its edges inside the column; gact wavefronts hold
+    // cells of the column
+    int glo = 0, ghi = nz, gact = 1;
+    if constexpr (G) {
+        const int span = 64 * W * (int)(blockDim.x >> 6);
+        glo = item.y == 0 ? 0 : item.y + 6;
+        ghi = item.y + span >= nz ? nz : item.y + span - 6;
+        gact = min((int)(blockDim.x >> 6), (nz - item.y + 64 * W - 1) / (64 * W));
+    }
+    const bool outl = G ? jin && j >= glo && j < ghi : jin && W * L >= a.shift && W * L < a.shift + a.wout;
+    // column groups: edge lanes, their cells' positions in a ring entry, and
+    // the neighbouring wavefronts; the group's first and last wavefronts
+    // read their own entries there, values that reach only halo cells or
+    // masked stencils
+    const bool lf = L == 0, ll = L == 63;
+    const bool xw = W == 1 ? (L < 2 || L >= 62) : (lf || ll);
+    const int xpos = W == 1 ? (L < 2 ? L : L - 60) : (lf ? 0 : 2);
+    const int gp = max(gw - 1, 0), gn = min(gw + 1, gact - 1);
     bool zm_ok[W], zp_ok[W];
     double bzb[W], bzt[W];
     bool bz_one = true;
@@ -404,8 +533,37 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
     }
     Row2<P, W> PF[4];
 
-    // the march starts at m0 = i0 - 3: s(n) rows m0-3..m0-1 fill window slots 1..3
-    const int m0 = i0 - 3, mend = i1 + 5;
+    // column groups: the edge cells of field f of a row computed in
+    // iteration m go to ring entry x (x0 + k for the k-th exchanged field)
+    // of slot m & 3; nbr reads what the neighbouring wavefronts put there
+    auto put = [&](int sl, int x, const float (&v)[W]) {
+        if (xw) {
+            float *e = xl + ((sl * GM + gw) * NX + x) * 4 + xpos;
+            e[0] = v[0];
+            if constexpr (W == 2) e[1] = v[1];
+        }
+    };
+    auto nbr = [&](int sl, int x) {
+        const float *pv = xl + ((sl * GM + gp) * NX + x) * 4 + 2;
+        const float *nv = xl + ((sl * GM + gn) * NX + x) * 4;
+        return Nbr{pv[0], pv[1], nv[0], nv[1]};
+    };
+    // exchanged fields (z neighbours taken by the stencils): stresses 1, 2
+    // (SH: 1; spin: 1, 2, 4, 5) and velocities 0, 1 (SH: 0; spin: 0, 1, 2)
+    auto zsf = [](int k) { return P == SP_SH ? 1 : (k < 2 ? k + 1 : k + 2); };
+    auto put_s = [&](int sl, int x0, const float (&s)[NS][W]) {
+#pragma unroll
+        for (int k = 0; k < NZS; k++) put(sl, x0 + k, s[zsf(k)]);
+    };
+    auto put_v = [&](int sl, int x0, const float (&v)[NV][W]) {
+#pragma unroll
+        for (int k = 0; k < NZV; k++) put(sl, x0 + k, v[k]);
+    };
+    // ring entries: s(n) [0, NZS), v(n+1) [NZS, NZS+NZV), s(n+1), v(n+2)
+    constexpr int X0 = 0, X1 = NZS, X2 = NZS + NZV, X3 = 2 * NZS + NZV;
+    const int sl0 = m0 & 3;  // slot of iteration m0 (m0 may be negative)
+
+    // s(n) rows m0-3..m0-1 fill window slots 1..3
     load2<P, W>(PF[1], RS, RM, NP4, rd, m0 - 3, nx, nz, vj, i0, i1);
     load2<P, W>(PF[2], RS, RM, NP4, rd, m0 - 2, nx, nz, vj, i0, i1);
     load2<P, W>(PF[3], RS, RM, NP4, rd, m0 - 1, nx, nz, vj, i0, i1);
@@ -418,6 +576,8 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             S0[2][f][c] = PF[2].s[f][c];
             S0[3][f][c] = PF[3].s[f][c];
         }
+    // the s(n) row the first iteration's phase A centres on
+    if constexpr (G) put_s((sl0 + 3) & 3, X0, S0[3]);
 
     for (int mb = m0; mb <= mend; mb += 4) {
 #pragma unroll
@@ -425,6 +585,10 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             const int m = mb + u;
             if (m > mend) break;
             const int A = (u + 1) & 3, B = (u + 2) & 3, C = (u + 3) & 3, D = u;
+            // ring slots of this iteration and the two before it
+            const int slm = (sl0 + u) & 3, sl1 = (sl0 + u + 3) & 3, sl2 = (sl0 + u + 2) & 3;
+            Nbr nb[NZS > NZV ? NZS : NZV];
+            if constexpr (G) __syncthreads();
             // prefetch one row ahead: the row of the next iteration (a
             // second row in flight measured no faster and costs about 70
             // VGPRs with two cells per lane)
@@ -434,13 +598,18 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             for (int f = 0; f < NS; f++)
 #pragma unroll
                 for (int c = 0; c < W; c++) S0[D][f][c] = cur.s[f][c];
+            if constexpr (G) put_s(slm, X0, S0[D]);
 
             // ---- A: v(n+1) at row m-1; s(n) rows m-3..m = slots A, B, C, D
             {
                 const int r = m - 1;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV][W];
-                vel_row<P, EX, W>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ);
+                if constexpr (G) {
+#pragma unroll
+                    for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X0 + k);
+                }
+                vel_row<P, EX, W, G>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
                 inject<P, W>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
@@ -457,18 +626,24 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 #pragma unroll
                     for (int f = 0; f < NV; f++) W1[C][f][c] = vn[f];
                 }
+                if constexpr (G) put_v(slm, X1, W1[C]);
             }
             // ---- B: s(n+1) at row m-3; v(n+1) rows m-4..m-1 = slots D, A, B, C
             {
                 const int q = m - 3;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS][W];
-                str_row<P, EX, W>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB, xp_ok,
-                                  zp_ok, KX, KZ, dt);
+                if constexpr (G) {
+#pragma unroll
+                    for (int k = 0; k < NZV; k++) nb[k] = nbr(sl2, X1 + k);
+                }
+                str_row<P, EX, W, G>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB,
+                                     xp_ok, zp_ok, KX, KZ, dt, nb, lf, ll);
 #pragma unroll
                 for (int f = 0; f < NS; f++)
 #pragma unroll
                     for (int c = 0; c < W; c++) S1[A][f][c] = sn[f][c];
+                if constexpr (G) put_s(slm, X2, S1[A]);
 #pragma unroll
                 for (int c = 0; c < W; c++) {
                     LW[D][c] = cur.lamB[c];
@@ -482,7 +657,11 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int r = m - 4;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV][W], u1[NV][W], u2[NV][W];
-                vel_row<P, EX, W>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ);
+                if constexpr (G) {
+#pragma unroll
+                    for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X2 + k);
+                }
+                vel_row<P, EX, W, G>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
                 inject<P, W>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
@@ -501,6 +680,7 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                         W2[D][f][c] = vn[f];
                     }
                 }
+                if constexpr (G) put_v(slm, X3, W2[D]);
                 // rows outside the chunk: every lane's offset is out of range
                 const bool rok = r >= i0 && r < i1;
                 const unsigned ov = rok ? vo : OOB;
@@ -552,8 +732,12 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
                 const int q = m - 6;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS][W];
-                str_row<P, EX, W>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok, KX,
-                                  KZ, dt);
+                if constexpr (G) {
+#pragma unroll
+                    for (int k = 0; k < NZV; k++) nb[k] = nbr(sl2, X3 + k);
+                }
+                str_row<P, EX, W, G>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok,
+                                     KX, KZ, dt, nb, lf, ll);
                 const bool qok = q >= i0 && q < i1;
                 const unsigned oq = qok ? vo : OOB;
                 const int orow = qok ? q * nz * 4 : 0;
@@ -562,4 +746,18 @@ __global__ __launch_bounds__(256) void k_step2(StepArgs a)
             }
         }
     }
+}
+
+template <int P, bool EX, int W>
+__global__ __launch_bounds__(256) void k_step2(StepArgs a)
+{
+    step2_body<P, EX, W, false>(a, nullptr);
+}
+
+// column groups: blocks of up to grp_max wavefronts
+template <int P, bool EX, int W>
+__global__ __launch_bounds__(64 * grp_max(P, W)) void k_step2g(StepArgs a)
+{
+    __shared__ float xl[4 * grp_max(P, W) * n_xch(P) * 4];
+    step2_body<P, EX, W, true>(a, xl);
 }

@@ -115,6 +115,8 @@ struct sp_ctx {
     size_t d_snaptab_cap = 0;
     int inter_sae = 0, inter_nsa = 0;    // interactions of the adjoint run being captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
+    int group = 0;             // column groups: wavefronts per block (0: 4)
+    int grp_waves = 0;         // column groups of the prepared batch (cur_fuse 3, 4): wavefronts per block
     int graph = 0;
     long long snap_mb = 0;
     // snapshot pool
@@ -330,11 +332,19 @@ int strip_width(const sp_ctx *c) { return c->cur_fuse ? 52 : (c->cur_kernel == 1
 
 // 0: one step per launch; 1: two steps per launch (k_step2, one cell per
 // lane); 2: two steps per launch with two cells per lane (SH and P-SV, every
-// grid of the batch with even nz; single steps keep the one-cell strips)
+// grid of the batch with even nz); 3: two steps per launch in column groups
+// (k_step2g), one cell per lane; 4: the same with two cells per lane (SH
+// and P-SV with even nz, else as 2).  Single steps keep the one-cell strips.
 int fuse_ok(const sp_ctx *c, const std::vector<int> &batch)
 {
     if (!c->fuse || batch_kernel(c, batch) != 1) return 0;
-    if (c->fuse == 2 && c->physics != SP_SPIN) {
+    if (c->fuse == 3) return 3;
+    if (c->fuse == 4 && c->physics != SP_SPIN) {
+        bool even = true;
+        for (int s : batch) even = even && c->grids[c->shots[s].grid].nz % 2 == 0;
+        if (even) return 4;
+    }
+    if (c->fuse >= 2 && c->physics != SP_SPIN) {
         bool even = true;
         for (int s : batch) even = even && c->grids[c->shots[s].grid].nz % 2 == 0;
         if (even) return 2;
@@ -458,6 +468,49 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
             }
         }
     };
+    if (c->cur_fuse >= 3) {
+        // column groups: one item per block (shot, chunk, span), the block's
+        // wavefronts covering a span of 64 W gb cells; spans overlap by 12
+        // cells (6 halo cells on each side of an inner boundary) and the
+        // last reaches the column's end.  Order 0: shot-major, a chunk's
+        // spans consecutive; otherwise chunk-major per grid, each span's
+        // shots consecutive.
+        const int W = c->cur_fuse == 4 ? 2 : 1;
+        const int gb = c->group > 0 ? c->group : 4;
+        const int span = 64 * W * gb, stride = span - 12;
+        c->grp_waves = gb;
+        items.clear();
+        auto spans = [&](int nz) {
+            std::vector<int> j0s;
+            for (int j0 = 0;; j0 += stride) {
+                j0s.push_back(j0);
+                if (j0 + span >= nz) break;
+            }
+            return j0s;
+        };
+        if (c->order == 0) {
+            for (int b = 0; b < (int)batch.size(); b++) {
+                const Grid &g = c->grids[c->shots[batch[b]].grid];
+                const std::vector<int> j0s = spans(g.nz);
+                for (int i0 = 0; i0 < g.nx; i0 += chunk)
+                    for (int j0 : j0s) push(b, j0, i0);
+            }
+        } else {
+            std::vector<std::vector<int>> per_grid(c->grids.size());
+            for (int b = 0; b < (int)batch.size(); b++) per_grid[c->shots[batch[b]].grid].push_back(b);
+            for (size_t g = 0; g < per_grid.size(); g++) {
+                const std::vector<int> j0s = spans(c->grids[g].nz);
+                for (int i0 = 0; i0 < c->grids[g].nx; i0 += chunk)
+                    for (int j0 : j0s)
+                        for (int b : per_grid[g]) push(b, j0, i0);
+            }
+        }
+        if (ensure(c, c->d_items_w, c->d_items_w_cap, items.size())) return 2;
+        HIPCHECK(c, hipMemcpyAsync(c->d_items_w, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->n_items_w = (int)items.size() / 2;
+    }
     if (c->cur_fuse == 2) {
         build(116);
         if (ensure(c, c->d_items_w, c->d_items_w_cap, items.size())) return 2;
@@ -499,7 +552,36 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     a.wout = strip_width(c);
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
     const bool ex = c->exact;
-    if (nsteps == 2 && c->cur_fuse == 2) {
+    if (nsteps == 2 && c->cur_fuse >= 3) {
+        a.items = c->d_items_w;
+        a.nitems = c->n_items_w;
+        grid = dim3((unsigned)c->n_items_w);
+        block = dim3((unsigned)(64 * c->grp_waves));
+        if (c->cur_fuse == 4) {
+            if (c->physics == SP_SH) {
+                if (ex) hipLaunchKernelGGL((k_step2g<SP_SH, true, 2>), grid, block, 0, c->stream, a);
+                else hipLaunchKernelGGL((k_step2g<SP_SH, false, 2>), grid, block, 0, c->stream, a);
+            } else {
+                if (ex) hipLaunchKernelGGL((k_step2g<SP_PSV, true, 2>), grid, block, 0, c->stream, a);
+                else hipLaunchKernelGGL((k_step2g<SP_PSV, false, 2>), grid, block, 0, c->stream, a);
+            }
+        } else {
+            switch (c->physics) {
+            case SP_SH:
+                if (ex) hipLaunchKernelGGL((k_step2g<SP_SH, true, 1>), grid, block, 0, c->stream, a);
+                else hipLaunchKernelGGL((k_step2g<SP_SH, false, 1>), grid, block, 0, c->stream, a);
+                break;
+            case SP_PSV:
+                if (ex) hipLaunchKernelGGL((k_step2g<SP_PSV, true, 1>), grid, block, 0, c->stream, a);
+                else hipLaunchKernelGGL((k_step2g<SP_PSV, false, 1>), grid, block, 0, c->stream, a);
+                break;
+            default:
+                if (ex) hipLaunchKernelGGL((k_step2g<SP_SPIN, true, 1>), grid, block, 0, c->stream, a);
+                else hipLaunchKernelGGL((k_step2g<SP_SPIN, false, 1>), grid, block, 0, c->stream, a);
+                break;
+            }
+        }
+    } else if (nsteps == 2 && c->cur_fuse == 2) {
         a.items = c->d_items_w;
         a.nitems = c->n_items_w;
         a.wout = 116;
@@ -1474,11 +1556,18 @@ int sp_set_option(sp_ctx *c, int key, long long v)
         return 0;
     case SP_OPT_GRAPH: c->graph = v ? 1 : 0; return 0;  // forward runs without snapshots, own or set stream
     case SP_OPT_KERNEL: c->kernel = v ? 1 : 0; return 0;
-    case SP_OPT_FUSE: c->fuse = v == 2 ? 2 : (v ? 1 : 0); return 0;
+    case SP_OPT_FUSE:
+        if (v < 0 || v > 4) return fail(c, "fuse must be 0..4");
+        c->fuse = (int)v;
+        return 0;
     case SP_OPT_EXACT: c->exact = v ? 1 : 0; return 0;
     case SP_OPT_ADJOINT:
         if (v != 0 && v != 1) return fail(c, "adjoint must be 0 (reference-style) or 1 (discrete)");
         c->adjoint_mode = (int)v;
+        return 0;
+    case SP_OPT_GROUP:
+        if (v < 0 || v > 8) return fail(c, "group must be 0 (4) or 1..8 wavefronts per block");
+        c->group = (int)v;
         return 0;
     case SP_OPT_STRIP: {
         const int sh = (int)(v >> 16), wo = (int)(v & 0xffff);
@@ -2032,7 +2121,7 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
     for (const Shot &sh : c->shots)
         key += ":" + std::to_string(c->grids[sh.grid].nx) + "x" + std::to_string(c->grids[sh.grid].nz);
     static std::mutex mu;
-    static std::map<std::string, std::array<int, 3>> cache;
+    static std::map<std::string, std::array<int, 4>> cache;
     {
         std::lock_guard<std::mutex> lock(mu);
         auto f = cache.find(key);
@@ -2040,32 +2129,41 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
             c->chunk = f->second[0];
             c->order = f->second[1];
             c->fuse = f->second[2];
+            c->group = f->second[3];
             if (chunk_out) *chunk_out = c->chunk;
             if (order_out) *order_out = c->order;
             return 0;
         }
     }
     const int chunks[] = {4, 8, 16, 24, 32, 48, 64, 96, 128};
-    // one or two steps per launch, and two steps with one or two cells per
-    // lane where the batch allows it (all three give the same results)
+    // one or two steps per launch, two steps in strips of one or two cells
+    // per lane and in column groups, each where the batch allows it (all
+    // give the same results)
     std::vector<int> fuses = {c->fuse};
     if (c->fuse && fuse_ok(c, all)) {
-        fuses.push_back(0);
         const int user = c->fuse;
-        c->fuse = 2;
-        const bool wide = fuse_ok(c, all) == 2;
+        for (int f = 0; f <= 4; f++) {
+            if (f == user) continue;
+            c->fuse = f;
+            if (f == 0 || fuse_ok(c, all) == f) fuses.push_back(f);
+        }
         c->fuse = user;
-        if (wide) fuses.push_back(user == 2 ? 1 : 2);
     }
     float best = 1e30f;
-    int bc = 16, bo = 0, bf = c->fuse;
+    int bc = 16, bo = 0, bf = c->fuse, bg = c->group;
     if (timing_begin(c)) return 2;
     for (int fu : fuses) {
         c->fuse = fu;
-        for (int order = 0; order < 4; order++) {
+        // column groups: two, four or eight wavefronts per block, and two
+        // orders (shot-major and chunk-major)
+        std::vector<int> groups = {c->group};
+        if (fu >= 3) groups = {2, 4, 8};
+        for (int gr : groups)
+        for (int order = 0; order < (fu >= 3 ? 2 : 4); order++) {
             for (int ch : chunks) {
                 c->chunk = ch;
                 c->order = order;
+                c->group = gr;
                 int nitems = 0;
                 if (prepare_batch(c, all, &nitems) || clear_state(c, all)) return 2;
                 if (launch_step(c, nitems, 0, -1, 0, 0)) return 2;
@@ -2086,6 +2184,7 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
                     bc = ch;
                     bo = order;
                     bf = fu;
+                    bg = gr;
                 }
             }
         }
@@ -2093,10 +2192,11 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
     c->chunk = bc;
     c->order = bo;
     c->fuse = bf;
+    c->group = bg;
     c->snaps_valid = 0;
     {
         std::lock_guard<std::mutex> lock(mu);
-        cache[key] = {bc, bo, bf};
+        cache[key] = {bc, bo, bf, bg};
     }
     if (chunk_out) *chunk_out = bc;
     if (order_out) *order_out = bo;
@@ -2116,6 +2216,7 @@ int sp_get_option(const sp_ctx *c, int key, long long *value)
     case SP_OPT_FUSE: *value = c->fuse; return 0;
     case SP_OPT_EXACT: *value = c->exact; return 0;
     case SP_OPT_ADJOINT: *value = c->adjoint_mode; return 0;
+    case SP_OPT_GROUP: *value = c->group; return 0;
     default: return 1;
     }
 }

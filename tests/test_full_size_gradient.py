@@ -3,8 +3,9 @@ accumulation: solver.py:290-404) at the benchmark workloads' full sizes.
 
 * P-SV 1000^2, nt 4000, sae 50: shot 0 alone gives the oracle's kernels bit
   for bit (one shot per grid: the sums follow the oracle's order), with one
-  and with two cells per lane; its misfit norms agree to 1e-12 (float64
-  partial sums of the same float32 residuals, added in another order).
+  and with two cells per lane, in strips and in column groups; its misfit
+  norms agree to 1e-12 (float64 partial sums of the same float32 residuals,
+  added in another order).
 * The 128-shot batch (the timed workload: time-major kernel sums over
   128 shots, snapshot pool filled in sub-batches) equals, within the
   kernel tolerance, the 124-shot batch without shots {0, 41, 86, 127} plus
@@ -78,7 +79,7 @@ def test_benchmark_shot0_gradient_equals_the_oracle(ctx_factory):
     print("case: 1000^2 x 4000 shot 0 gradient vs oracle", flush=True)
     data = _records(ctx_factory, one_t)
     kref, nref = oracle_gradient(one_i, data)
-    for opts in ({}, {"fuse": 2, "chunk": 128}):
+    for opts in ({}, {"fuse": 2, "chunk": 128}, {"fuse": 4, "chunk": 64}):
         K, norms = _gradient(ctx_factory, one_i, data, PSV_K, **opts)
         np.testing.assert_allclose(norms, nref, rtol=1e-12)  # float64 partial sums in another order
         for nm, _ in PSV_K:
