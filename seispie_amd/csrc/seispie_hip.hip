@@ -2135,7 +2135,7 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
             return 0;
         }
     }
-    const int chunks[] = {4, 8, 16, 24, 32, 48, 64, 96, 128};
+    const int chunks[] = {4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 200, 256};
     // one or two steps per launch, two steps in strips of one or two cells
     // per lane and in column groups, each where the batch allows it (all
     // give the same results)
