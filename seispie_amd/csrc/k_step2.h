@@ -508,11 +508,15 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     while (p2 < ni && ik[p2] < (i0 - 1) * nz) ++p2;
     int row1 = p1 < ni ? ik[p1] / nz : 0x7fffffff;
     int row2 = p2 < ni ? ik[p2] / nz : 0x7fffffff;
-    // receivers recorded by this chunk (forward only): rows i0..i1-1
+    // receivers recorded by this chunk (forward only): rows i0..i1-1; the
+    // next two receivers' cells and record rows wait in scalar registers
+    // (a row's test uses values loaded rows earlier)
     const cint_p rk = (cint_p)d.rec_k;
     const cint_p rrow = (cint_p)d.rec_row;
+    const int nrec = a.adjoint ? 0 : d.nrec;
     int pr = __builtin_amdgcn_readfirstlane(item2.z);
-    int rec_row = (!a.adjoint && pr < d.nrec) ? rk[pr] / nz : 0x7fffffff;
+    int nk1 = pr < nrec ? rk[pr] : 0x7fffffff, nr1 = pr < nrec ? rrow[pr] : 0;
+    int nk2 = pr + 1 < nrec ? rk[pr + 1] : 0x7fffffff, nr2 = pr + 1 < nrec ? rrow[pr + 1] : 0;
     const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
 
     float S0[4][NS][W], W1[4][NV][W], S1[4][NS][W], W2[4][NV][W];
@@ -710,21 +714,22 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                 }
                 // save_obs of both steps (solver.py:151-156) for receivers in
                 // this row; rows past the chunk belong to the next chunk
-                if (r == rec_row && r < i1) {
-                    const int kend = (r + 1) * nz, kr = r * nz + j;
-                    while (pr < d.nrec && rk[pr] < kend) {
-                        const int row = rrow[pr];
-                        const int c = rk[pr] - kr;  // 0 .. W-1 in the lane holding the receiver
-                        const unsigned orc = (outl && c >= 0 && c < W) ? (unsigned)(row * a.nt + itx) * 4u : OOB;
-                        const int cs = d.nrec * a.nt * 4;
+                if (r < i1 && nk1 < (r + 1) * nz) {
+                    const int kr = r * nz + j, cs = d.nrec * a.nt * 4;
+                    do {
+                        const int c = nk1 - kr;  // 0 .. W-1 in the lane holding the receiver
+                        const unsigned orc = (outl && c >= 0 && c < W) ? (unsigned)(nr1 * a.nt + itx) * 4u : OOB;
 #pragma unroll
                         for (int f = 0; f < NV; f++) {
                             bst(RR, orc, f * cs, (W == 2 && c == 1) ? u1[f][W - 1] : u1[f][0]);
                             bst(RR, orc, f * cs + 4, (W == 2 && c == 1) ? u2[f][W - 1] : u2[f][0]);
                         }
                         ++pr;
-                    }
-                    rec_row = pr < d.nrec ? rk[pr] / nz : 0x7fffffff;
+                        nk1 = nk2;
+                        nr1 = nr2;
+                        nk2 = pr + 1 < nrec ? rk[pr + 1] : 0x7fffffff;
+                        nr2 = pr + 1 < nrec ? rrow[pr + 1] : 0;
+                    } while (nk1 < (r + 1) * nz);
                 }
             }
             // ---- D: s(n+2) at row m-6; v(n+2) rows m-7..m-4 = slots A, B, C, D

@@ -15,9 +15,10 @@
 //   * loads always issue (rows and lanes clamped into the grid); stores
 //     always issue, with an offset past the end of the buffer for lanes or
 //     rows that must not be written -- the buffer range check drops them;
-//   * snapshots and receiver records are separate kernels (k_snap,
-//     k_record); source injection reads the time function with scalar
-//     loads only;
+//   * snapshots are a separate kernel (k_snap); receiver records of a
+//     forward step are stored in a wave-uniform branch of the rows that hold
+//     receivers, as the two-step kernel does; source injection reads the
+//     time function with scalar loads only;
 //   * the row loop is unrolled by four with windows indexed by row & 3, so
 //     the register windows never move.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
@@ -117,6 +118,17 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     int pi = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
     while (pi < ni && ik[pi] < (i0 - 1) * nz) ++pi;  // the list pointer starts at row i0-4
     int src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+    // receivers recorded by this chunk (forward only): rows i0..i1-1.  The
+    // next two receivers' cells and record rows wait in scalar registers, so
+    // a row's test uses values loaded rows earlier and the loads of later
+    // receivers overlap the march
+    const cint_p rk = (cint_p)d.rec_k;
+    const cint_p rrow = (cint_p)d.rec_row;
+    const int nrec = a.adjoint ? 0 : d.nrec;
+    int pr = __builtin_amdgcn_readfirstlane(item2.z);
+    int nk1 = pr < nrec ? rk[pr] : 0x7fffffff, nr1 = pr < nrec ? rrow[pr] : 0;
+    int nk2 = pr + 1 < nrec ? rk[pr + 1] : 0x7fffffff, nr2 = pr + 1 < nrec ? rrow[pr + 1] : 0;
+    const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
 
     float S[4][NS], W[4][NV];
     Row3<P> PF[4];
@@ -248,6 +260,21 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
                     bst(RS, ov, slot_u(P, f) * NP4, un[f]);
                     bst(RS, ov, slot_v(P, wr, f) * NP4, vn[f]);
                 }
+            }
+            // save_obs (solver.py:151-156): displacement after the step, for
+            // receivers in this row; rows past the chunk belong to the next
+            if (r < i1 && nk1 < (r + 1) * nz) {
+                const int kr = r * nz + j, cs = d.nrec * a.nt * 4;
+                do {
+                    const unsigned orc = (outl && nk1 == kr) ? (unsigned)(nr1 * a.nt + itx) * 4u : OOB;
+#pragma unroll
+                    for (int f = 0; f < NV; f++) bst(RR, orc, f * cs, un[f]);
+                    ++pr;
+                    nk1 = nk2;
+                    nr1 = nr2;
+                    nk2 = pr + 1 < nrec ? rk[pr + 1] : 0x7fffffff;
+                    nr2 = pr + 1 < nrec ? rrow[pr + 1] : 0;
+                } while (nk1 < (r + 1) * nz);
             }
 #pragma unroll
             for (int f = 0; f < NV; f++) W[C][f] = vn[f];

@@ -680,7 +680,7 @@ int clear_state(sp_ctx *c, const std::vector<int> &batch)
 // every replay starts from the same set.  Needs a stream other than the null
 // stream.  Returns 1 (nothing run) where a graph does not apply; *it_done is
 // the first step left to the caller.
-int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int sae, int nsa, int &set, int &it_done)
+int run_forward_graph(sp_ctx *c, int nitems, int sae, int nsa, int &set, int &it_done)
 {
     it_done = 0;
     if (sae > 0 && !c->cur_fuse) return 1;
@@ -711,7 +711,6 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int sae, int ns
                                    c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
     }
-    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -723,9 +722,6 @@ int run_forward_graph(sp_ctx *c, int nitems, int nb, int maxrec, int sae, int ns
         rc = launch_step(c, nitems, it, -1, 0, s, ns);
         s ^= 1;
         nl++;
-        if (!rc && ns == 1)
-            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
-                               c->itp);
         it += ns;
     }
     if (!rc) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_counter, block);
@@ -762,17 +758,13 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
     if (clear_state(c, batch)) return 2;
     const int nsa = sae > 0 ? c->nt / sae : 0;
     const int nb = (int)batch.size();
-    int maxrec = 1;
     long long maxN = 1;
-    for (int s : batch) {
-        maxrec = std::max(maxrec, c->shots[s].nrec);
-        maxN = std::max(maxN, c->grids[c->shots[s].grid].N);
-    }
-    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256)), sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
-    const bool split = c->cur_kernel == 1;  // snapshots and records as kernels of their own
+    for (int s : batch) maxN = std::max(maxN, c->grids[c->shots[s].grid].N);
+    const dim3 sgrid((unsigned)((maxN + 255) / 256), (unsigned)nb);
+    const bool split = c->cur_kernel == 1;  // snapshots of single steps as a kernel of their own
     int set = 0, it0 = 0;
     if (c->graph && split && c->stream) {
-        const int rc = run_forward_graph(c, nitems, nb, maxrec, sae, nsa, set, it0);
+        const int rc = run_forward_graph(c, nitems, sae, nsa, set, it0);
         if (rc != 0 && rc != 1) return rc;
     }
     // the steps no graph replay covered, eagerly
@@ -792,10 +784,6 @@ int run_forward_batch(sp_ctx *c, const std::vector<int> &batch, int sae)
         if (split) {
             if (copy)
                 hipLaunchKernelGGL(k_snap, sgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxN, isa, 1, set, c->physics);
-            // the two-step kernel records both of its steps itself
-            if (ns == 1)
-                hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt,
-                                   c->physics, c->itp);
             HIPCHECK(c, hipGetLastError());
         }
         it += ns;
@@ -1085,10 +1073,6 @@ int run_forward_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
     int nitems = 0;
     if (prepare_batch(c, batch, &nitems)) return 2;
     if (clear_state(c, batch)) return 2;
-    const int nb = (int)batch.size();
-    int maxrec = 1;
-    for (int s : batch) maxrec = std::max(maxrec, c->shots[s].nrec);
-    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
     int set = 0;
     for (int it = 0; it < c->nt;) {
         if (it % C == 0)
@@ -1097,11 +1081,6 @@ int run_forward_ckpt_batch(sp_ctx *c, const std::vector<int> &batch, int C)
         const int ns = steps_next(c, it, c->nt, 0, C);
         if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
         set ^= 1;
-        if (c->cur_kernel == 1 && ns == 1) {
-            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
-                               nullptr);
-            HIPCHECK(c, hipGetLastError());
-        }
         it += ns;
     }
     c->t_bytes += step_alg_bytes(c, batch) * c->nt;
@@ -1794,16 +1773,12 @@ int sp_forward_steps(sp_ctx *c, int it0, int it1)
     if (it0 > 0 && it0 != c->fwd_next) return fail(c, "sp_forward_steps must continue from step " + std::to_string(c->fwd_next));
     std::vector<int> all(c->shots.size());
     std::iota(all.begin(), all.end(), 0);
-    const int nb = (int)all.size();
-    int maxrec = 1;
-    for (int s : all) maxrec = std::max(maxrec, c->shots[s].nrec);
-    const dim3 rgrid((unsigned)(((long long)nb * maxrec + 255) / 256));
     int nitems = 0;
     if (prepare_batch(c, all, &nitems)) return 2;
     if (it0 == 0) {
         c->snaps_valid = 0;
         c->ckpt_valid = 0;
-    c->fwd_next = -1;
+        c->fwd_next = -1;
         if (clear_state(c, all)) return 2;
         c->cur_set = 0;
     }
@@ -1813,11 +1788,6 @@ int sp_forward_steps(sp_ctx *c, int it0, int it1)
         const int ns = steps_next(c, it, it1, 0, 0);
         if (launch_step(c, nitems, it, -1, 0, set, ns)) return 2;
         set ^= 1;
-        if (c->cur_kernel == 1 && ns == 1) {
-            hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, c->stream, c->d_desc, nb, maxrec, it, c->nt, c->physics,
-                               nullptr);
-            HIPCHECK(c, hipGetLastError());
-        }
         it += ns;
     }
     c->cur_set = set;

@@ -4,23 +4,6 @@
 // after <hip/hip_runtime.h> and the headers before this one.
 #pragma once
 
-// save_obs (solver.py:151-156) for every receiver of a batch, after a step:
-// one thread per (shot, receiver) of the batch.
-__global__ __launch_bounds__(256) void k_record(const ShotDesc *shots, int nshots, int maxrec, int it, int nt, int P,
-                                                const int *itp)
-{
-    if (itp) it += *itp;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int s = t / maxrec, q = t - s * maxrec;
-    if (s >= nshots) return;
-    const ShotDesc &d = shots[s];
-    if (q >= d.nrec) return;
-    const int k = d.rec_k[q], row = d.rec_row[q];
-    const int nv = n_v(P);
-    for (int f = 0; f < nv; f++)
-        d.rec[((long long)f * d.nrec + row) * nt + it] = d.state[(long long)(2 * n_s(P) + 2 * nv + f) * d.Npad + k];
-}
-
 // advances the step counter of a replayed graph (one thread, vector store)
 __global__ void k_advance(int *counter, int steps)
 {
