@@ -184,3 +184,46 @@ def test_column_groups_in_replayed_graphs(ctx_factory, physics, fuse):
             ctx.set_data(i, d)
         np.testing.assert_array_equal(ctx.gradient(prob.sae), n_e)
         np.testing.assert_array_equal(ctx.kernel(0, _lib.K_MU), k_e)
+
+
+@pytest.mark.parametrize("physics,fuse", [(_lib.SH, 4), (_lib.PSV, 3), (_lib.PSV, 4), (_lib.SPIN, 3)])
+@pytest.mark.parametrize("mode,sae", [(0, 5), (1, 1)])
+def test_checkpointed_adjoints_with_column_groups(ctx_factory, physics, fuse, mode, sae):
+    """sp_forward_ckpt / sp_adjoint_ckpt, reference-style (mode 0) and
+    discrete (mode 1): segments recomputed from checkpoints by the two-step
+    kernel, which writes the snapshots of both of its steps, in column
+    groups of 2 wavefronts (three spans of a 250-cell column) against the
+    strips, beside a free surface; kernels and records bit for bit."""
+    from tests.test_discrete_adjoint import _spin_surface_problem, _surface_problem
+    nt = 200
+    prob = (_spin_surface_problem(70, 250, nt) if physics == _lib.SPIN else _surface_problem(physics, 70, 250, nt))
+    g = prob.grids[0]
+    ncomp = 1 if physics == _lib.SH else 2
+    rng = np.random.default_rng(3)
+    W = [rng.standard_normal((ncomp, s.rec_k.size, nt)).astype(np.float32) for s in g.shots]
+    kids = [_lib.K_LAM, _lib.K_MU, _lib.K_RHO, _lib.K_HESS]
+    if physics == _lib.SPIN:
+        kids += [_lib.K_NU, _lib.K_J, _lib.K_MUC]
+    print("case: checkpointed adjoint", physics, "fuse", fuse, "mode", mode, "sae", sae, flush=True)
+    out = []
+    for f in (1, fuse):
+        ctx = ctx_factory()
+        ctx.set_option(_lib.OPT_FUSE, f)
+        ctx.set_option(_lib.OPT_GROUP, 2)
+        ctx.set_option(_lib.OPT_CHUNK, 24)
+        ctx.set_option(_lib.OPT_ADJOINT, mode)
+        ids = configs.load_problem(ctx, prob)
+        ctx.forward_ckpt(20)
+        recs = [ctx.records(i) for i in ids]
+        for i, w in zip(ids, W):
+            ctx.set_adjoint_source(i, w)
+        ctx.adjoint_ckpt(sae)
+        out.append((recs, [ctx.kernel(0, k) for k in kids]))
+        ctx.close()
+    (r1, k1), (rg, kg) = out
+    for a, b in zip(r1, rg):
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(k1, kg):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(k1[1]).max() > 0
