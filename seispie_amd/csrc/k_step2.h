@@ -281,10 +281,12 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&s
 // `zint`: every lane of the strip lies outside the top/bottom sponges; with
 // bx = 1 the sponge factor is exactly 1 and its arithmetic is skipped
 // (the same float64 result, bit for bit).
+// irr, irj: RN(1/rho), RN(1/j) for the exact arithmetic (the caller
+// computes them, or reuses them from the pair's first step)
 template <int P, bool EX>
 __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&dsv)[n_v(P)], const float (&vo)[n_v(P)],
-                                           float rho, float jj, float bxr, double bzb, double bzt, bool zint, bool rin,
-                                           bool jin, double dt)
+                                           float rho, float jj, double irr, double irj, float bxr, double bzb,
+                                           double bzt, bool zint, bool rin, bool jin, double dt)
 {
     if constexpr (EX) {
         double b = 1.0;
@@ -292,14 +294,14 @@ __device__ __forceinline__ void vel_finish(float (&vn)[n_v(P)], const float (&ds
             const float b1 = (float)((double)bxr * bzb);
             b = (double)(float)((double)b1 * bzt);
         }
-        const double rr = (double)rho, irr = rcp_rn(rr);
+        const double rr = (double)rho;
 #pragma unroll
         for (int f = 0; f < n_v(P); f++) {
             double rf = rr, ir = irr;
             if constexpr (P == SP_SPIN) {
                 if (f == 2) {  // vy_c divides by the micro-inertia j (solver.py:226)
                     rf = (double)jj;
-                    ir = rcp_rn(rf);
+                    ir = irj;
                 }
             }
             const float v = vel_x(vo[f], dsv[f], dt, rf, ir, b);
@@ -523,11 +525,14 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     // materials of rows m-1 (rho, j) and m-3 (lam, mu, nu, gam) by
     // iteration: three iterations later they are rows m-4 and m-6
     float RW[4][W], JW[4][W], LW[4][W], MW[4][W], NW[4][W], GW[4][W];
+    // spin, exact arithmetic: RN(1/rho), RN(1/j) of the same rows
+    double IRW[4][W], IJW[4][W];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
 #pragma unroll
         for (int c = 0; c < W; c++) {
             RW[u][c] = JW[u][c] = 1.f;
+            IRW[u][c] = IJW[u][c] = 1.0;
             LW[u][c] = MW[u][c] = NW[u][c] = GW[u][c] = 0.f;
 #pragma unroll
             for (int f = 0; f < NS; f++) S0[u][f][c] = S1[u][f][c] = 0.f;
@@ -624,7 +629,19 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                         dc[f] = dsv[f][c];
                         vc[f] = cur.v[f][c];
                     }
-                    vel_finish<P, EX>(vn, dc, vc, cur.rhoA[c], cur.jA[c], bxr, bzb[c], bzt[c], zint, rin, jin, dt);
+                    // exact reciprocals of rho and j; spin keeps them for the
+                    // pair's second step at this row (three iterations on)
+                    double irr = 0.0, irj = 0.0;
+                    if constexpr (EX) {
+                        irr = rcp_rn((double)cur.rhoA[c]);
+                        if constexpr (P == SP_SPIN) {
+                            irj = rcp_rn((double)cur.jA[c]);
+                            IRW[D][c] = irr;
+                            IJW[D][c] = irj;
+                        }
+                    }
+                    vel_finish<P, EX>(vn, dc, vc, cur.rhoA[c], cur.jA[c], irr, irj, bxr, bzb[c], bzt[c], zint, rin,
+                                      jin, dt);
                     RW[D][c] = cur.rhoA[c];
                     JW[D][c] = cur.jA[c];
 #pragma unroll
@@ -676,7 +693,17 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                         dc[f] = dsv[f][c];
                         vc[f] = W1[D][f][c];
                     }
-                    vel_finish<P, EX>(vn, dc, vc, RW[A][c], JW[A][c], bxr, bzb[c], bzt[c], zint, rin, jin, dt);
+                    double irr = 0.0, irj = 0.0;
+                    if constexpr (EX) {
+                        if constexpr (P == SP_SPIN) {
+                            irr = IRW[A][c];
+                            irj = IJW[A][c];
+                        } else {
+                            irr = rcp_rn((double)RW[A][c]);
+                        }
+                    }
+                    vel_finish<P, EX>(vn, dc, vc, RW[A][c], JW[A][c], irr, irj, bxr, bzb[c], bzt[c], zint, rin, jin,
+                                      dt);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
                         u1[f][c] = upd_disp<EX>(cur.u[f][c], W1[D][f][c], dt);
