@@ -240,7 +240,7 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&s
 #pragma unroll
         for (int c = 0; c < W; c++) {
             const double ddx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
-            const double ddz = zm_ok[c] ? dmn<EX>(zm2[c], zm1[c], sc[1][c], zp1[c], KZ) : 0.0;
+            const double ddz = lsel(zm_ok[c], dmn<EX>(zm2[c], zm1[c], sc[1][c], zp1[c], KZ));
             dsv[0][c] = (float)(ddx + ddz);
         }
     } else {
@@ -251,8 +251,8 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&s
         for (int c = 0; c < W; c++) {
             const double dxx = xm_ok ? dmn<EX>(sa[0][c], sb[0][c], sc[0][c], sd[0][c], KX) : 0.0;
             const double dxxz = xm_ok ? dmn<EX>(sa[2][c], sb[2][c], sc[2][c], sd[2][c], KX) : 0.0;
-            const double dzxz = zm_ok[c] ? dmn<EX>(xzm2[c], xzm1[c], sc[2][c], xzp1[c], KZ) : 0.0;
-            const double dzzz = zm_ok[c] ? dmn<EX>(zzm2[c], zzm1[c], sc[1][c], zzp1[c], KZ) : 0.0;
+            const double dzxz = lsel(zm_ok[c], dmn<EX>(xzm2[c], xzm1[c], sc[2][c], xzp1[c], KZ));
+            const double dzzz = lsel(zm_ok[c], dmn<EX>(zzm2[c], zzm1[c], sc[1][c], zzp1[c], KZ));
             dsv[0][c] = (float)(dxx + dzxz);
             dsv[1][c] = (float)(dxxz + dzzz);
         }
@@ -264,9 +264,9 @@ __device__ __forceinline__ void vel_row(float (&dsv)[n_v(P)][W], const float (&s
 #pragma unroll
             for (int c = 0; c < W; c++) {
                 const double dxyx = xm_ok ? dmn<EX>(sa[3][c], sb[3][c], sc[3][c], sd[3][c], KX) : 0.0;
-                const double dzyz = zm_ok[c] ? dmn<EX>(yzm2[c], yzm1[c], sc[5][c], yzp1[c], KZ) : 0.0;
+                const double dzyz = lsel(zm_ok[c], dmn<EX>(yzm2[c], yzm1[c], sc[5][c], yzp1[c], KZ));
                 const float dsy = (float)(dxyx + dzyz);
-                const double dzyy = zm_ok[c] ? dmn<EX>(yym2[c], yym1[c], sc[4][c], yyp1[c], KZ) : 0.0;
+                const double dzyy = lsel(zm_ok[c], dmn<EX>(yym2[c], yym1[c], sc[4][c], yyp1[c], KZ));
                 const double dxyy = xm_ok ? dmn<EX>(sa[4][c], sb[4][c], sc[4][c], sd[4][c], KX) : 0.0;
                 dsv[0][c] = (float)((double)dsv[0][c] + dzyy);
                 dsv[1][c] = (float)((double)dsv[1][c] - dxyy);
@@ -347,7 +347,7 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa
 #pragma unroll
         for (int c = 0; c < W; c++) {
             const float dvdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
-            const float dvdz = zp_ok[c] ? (float)dpl<EX>(zm1[c], wb[0][c], zp1[c], zp2[c], KZ) : 0.f;
+            const float dvdz = lsel(zp_ok[c], (float)dpl<EX>(zm1[c], wb[0][c], zp1[c], zp2[c], KZ));
             const double dtmu = dt * (double)mu[c];
             sn[0][c] = upd_sadd<EX>(so[0][c], dtmu, (double)dvdx);
             sn[1][c] = upd_sadd<EX>(so[1][c], dtmu, (double)dvdz);
@@ -362,8 +362,8 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa
         for (int c = 0; c < W; c++) {
             const float dvxdx = xp_ok ? (float)dpl<EX>(wa[0][c], wb[0][c], wc[0][c], wd[0][c], KX) : 0.f;
             const float dvzdx = xp_ok ? (float)dpl<EX>(wa[1][c], wb[1][c], wc[1][c], wd[1][c], KX) : 0.f;
-            const float dvxdz = zp_ok[c] ? (float)dpl<EX>(xm1[c], wb[0][c], xp1[c], xp2[c], KZ) : 0.f;
-            const float dvzdz = zp_ok[c] ? (float)dpl<EX>(zm1[c], wb[1][c], zp1[c], zp2[c], KZ) : 0.f;
+            const float dvxdz = lsel(zp_ok[c], (float)dpl<EX>(xm1[c], wb[0][c], xp1[c], xp2[c], KZ));
+            const float dvzdz = lsel(zp_ok[c], (float)dpl<EX>(zm1[c], wb[1][c], zp1[c], zp2[c], KZ));
             const double l2m = (double)lam[c] + 2.0 * (double)mu[c];
             sn[0][c] = upd_snorm<EX>(so[0][c], dt, l2m, dvxdx, mul_rn(lam[c], dvzdz));
             sn[1][c] = upd_snorm<EX>(so[1][c], dt, l2m, dvzdz, mul_rn(lam[c], dvxdx));
@@ -371,7 +371,7 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa
             if constexpr (P == SP_SPIN) {
                 // div_vy on vy_c + add_sy_c (solver.py:135-141, 227-228)
                 const float dcdx = xp_ok ? (float)dpl<EX>(wa[2][c], wb[2][c], wc[2][c], wd[2][c], KX) : 0.f;
-                const float dcdz = zp_ok[c] ? (float)dpl<EX>(cm1[c], wb[2][c], cp1[c], cp2[c], KZ) : 0.f;
+                const float dcdz = lsel(zp_ok[c], (float)dpl<EX>(cm1[c], wb[2][c], cp1[c], cp2[c], KZ));
                 const double dtg = dt * (double)gam[c];
                 sn[4][c] = upd_srot<EX>(so[4][c], 2.0 * dt, nu[c], wb[2][c], sub_rn(dvzdx, dvxdz));
                 sn[3][c] = upd_sadd<EX>(so[3][c], dtg, (double)dcdx);
@@ -424,6 +424,10 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     static_assert(W == 1 || (W == 2 && P != SP_SPIN), "two cells per lane: SH and P-SV");
     constexpr int NS = n_s(P), NV = n_v(P);
     constexpr int GM = grp_max(P, W), NX = n_xch(P), NZS = n_zs(P), NZV = n_zv(P);
+    // strips skip the arithmetic of prologue rows that feed no output row of
+    // the chunk (3 of the first 9 iterations' s(n+1) rows, 6 v(n+2) rows, 9
+    // s(n+2) rows); in column groups the branches cost more than they save
+    constexpr bool SKIP = !G;
     const int nb = gridDim.x, bb = blockIdx.x;
     const int q8 = nb >> 3, r8 = nb & 7, bx8 = bb & 7, by8 = bb >> 3;
     const int lb = bx8 < r8 ? bx8 * (q8 + 1) + by8 : r8 * (q8 + 1) + (bx8 - r8) * q8 + by8;
@@ -598,6 +602,15 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
             const int slm = (sl0 + u) & 3, sl1 = (sl0 + u + 3) & 3, sl2 = (sl0 + u + 2) & 3;
             Nbr nb[NZS > NZV ? NZS : NZV];
             if constexpr (G) __syncthreads();
+            // The field offsets (slot x field bytes) are loop-invariant, but
+            // kept as such they exhaust the SGPRs and are spilled to VGPR
+            // lanes, one v_readlane per use.  Hiding the field size and the
+            // stress/velocity set from the optimiser for each row makes the
+            // compiler recompute them on the scalar unit instead (1-3%
+            // faster).
+            int np4 = NP4, rdv = rd;
+            asm volatile("" : "+s"(np4), "+s"(rdv));
+            const int NP4 = np4, rd = rdv, wr = rdv ^ 1;
             // prefetch one row ahead: the row of the next iteration (a
             // second row in flight measured no faster and costs about 70
             // VGPRs with two cells per lane)
@@ -658,13 +671,15 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
 #pragma unroll
                     for (int k = 0; k < NZV; k++) nb[k] = nbr(sl2, X1 + k);
                 }
-                str_row<P, EX, W, G>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB, cur.gamB,
-                                     xp_ok, zp_ok, KX, KZ, dt, nb, lf, ll);
+                if (!SKIP || m >= i0) {  // rows before i0-3 feed no output row of the chunk
+                    str_row<P, EX, W, G>(sn, W1[D], W1[A], W1[B], W1[C], S0[A], cur.lamB, cur.muB, cur.nuB,
+                                         cur.gamB, xp_ok, zp_ok, KX, KZ, dt, nb, lf, ll);
 #pragma unroll
-                for (int f = 0; f < NS; f++)
+                    for (int f = 0; f < NS; f++)
 #pragma unroll
-                    for (int c = 0; c < W; c++) S1[A][f][c] = sn[f][c];
-                if constexpr (G) put_s(slm, X2, S1[A]);
+                        for (int c = 0; c < W; c++) S1[A][f][c] = sn[f][c];
+                    if constexpr (G) put_s(slm, X2, S1[A]);
+                }
 #pragma unroll
                 for (int c = 0; c < W; c++) {
                     LW[D][c] = cur.lamB[c];
@@ -678,40 +693,49 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                 const int r = m - 4;
                 const bool rin = r >= 0 && r < nx, xm_ok = r >= 2 && r < nx - 2;
                 float dsv[NV][W], u1[NV][W], u2[NV][W];
-                if constexpr (G) {
+                // rows before i0-1 feed no output row of the chunk: their
+                // arithmetic is skipped (stores below are masked there)
+                if (!SKIP || m >= i0 + 3) {
+                    if constexpr (G) {
 #pragma unroll
-                    for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X2 + k);
-                }
-                vel_row<P, EX, W, G>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                inject<P, W>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
-                const float bxr = bxp[min(max(r, 0), nx - 1)];
-#pragma unroll
-                for (int c = 0; c < W; c++) {
-                    float dc[NV], vc[NV], vn[NV];
-#pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        dc[f] = dsv[f][c];
-                        vc[f] = W1[D][f][c];
+                        for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X2 + k);
                     }
-                    double irr = 0.0, irj = 0.0;
-                    if constexpr (EX) {
-                        if constexpr (P == SP_SPIN) {
-                            irr = IRW[A][c];
-                            irj = IJW[A][c];
-                        } else {
-                            irr = rcp_rn((double)RW[A][c]);
+                    vel_row<P, EX, W, G>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
+                    inject<P, W>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
+                    const float bxr = bxp[min(max(r, 0), nx - 1)];
+#pragma unroll
+                    for (int c = 0; c < W; c++) {
+                        float dc[NV], vc[NV], vn[NV];
+#pragma unroll
+                        for (int f = 0; f < NV; f++) {
+                            dc[f] = dsv[f][c];
+                            vc[f] = W1[D][f][c];
+                        }
+                        double irr = 0.0, irj = 0.0;
+                        if constexpr (EX) {
+                            if constexpr (P == SP_SPIN) {
+                                irr = IRW[A][c];
+                                irj = IJW[A][c];
+                            } else {
+                                irr = rcp_rn((double)RW[A][c]);
+                            }
+                        }
+                        vel_finish<P, EX>(vn, dc, vc, RW[A][c], JW[A][c], irr, irj, bxr, bzb[c], bzt[c], zint, rin,
+                                          jin, dt);
+#pragma unroll
+                        for (int f = 0; f < NV; f++) {
+                            u1[f][c] = upd_disp<EX>(cur.u[f][c], W1[D][f][c], dt);
+                            u2[f][c] = upd_disp<EX>(u1[f][c], vn[f], dt);
+                            W2[D][f][c] = vn[f];
                         }
                     }
-                    vel_finish<P, EX>(vn, dc, vc, RW[A][c], JW[A][c], irr, irj, bxr, bzb[c], bzt[c], zint, rin, jin,
-                                      dt);
+                    if constexpr (G) put_v(slm, X3, W2[D]);
+                } else {
 #pragma unroll
-                    for (int f = 0; f < NV; f++) {
-                        u1[f][c] = upd_disp<EX>(cur.u[f][c], W1[D][f][c], dt);
-                        u2[f][c] = upd_disp<EX>(u1[f][c], vn[f], dt);
-                        W2[D][f][c] = vn[f];
-                    }
+                    for (int f = 0; f < NV; f++)
+#pragma unroll
+                        for (int c = 0; c < W; c++) u1[f][c] = u2[f][c] = 0.f;
                 }
-                if constexpr (G) put_v(slm, X3, W2[D]);
                 // rows outside the chunk: every lane's offset is out of range
                 const bool rok = r >= i0 && r < i1;
                 const unsigned ov = rok ? vo : OOB;
@@ -764,12 +788,19 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                 const int q = m - 6;
                 const bool xp_ok = q >= 1 && q < nx - 2;
                 float sn[NS][W];
-                if constexpr (G) {
+                if (!SKIP || m >= i0 + 6) {  // rows before i0: no output (stores masked)
+                    if constexpr (G) {
 #pragma unroll
-                    for (int k = 0; k < NZV; k++) nb[k] = nbr(sl2, X3 + k);
+                        for (int k = 0; k < NZV; k++) nb[k] = nbr(sl2, X3 + k);
+                    }
+                    str_row<P, EX, W, G>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok,
+                                         zp_ok, KX, KZ, dt, nb, lf, ll);
+                } else {
+#pragma unroll
+                    for (int f = 0; f < NS; f++)
+#pragma unroll
+                        for (int c = 0; c < W; c++) sn[f][c] = 0.f;
                 }
-                str_row<P, EX, W, G>(sn, W2[A], W2[B], W2[C], W2[D], S1[B], LW[A], MW[A], NW[A], GW[A], xp_ok, zp_ok,
-                                     KX, KZ, dt, nb, lf, ll);
                 const bool qok = q >= i0 && q < i1;
                 const unsigned oq = qok ? vo : OOB;
                 const int orow = qok ? q * nz * 4 : 0;

@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
         for (int f = 0; f < NV; f++) W[u][f] = 0.f;
     }
 
-    // prologue (i0 is a multiple of 4): rows i0-3, i0-2, i0-1 fill stress
-    // slots 1, 2, 3; rows up to i0+1 are prefetched.
+    // prologue: rows i0-3, i0-2, i0-1 fill stress slots 1, 2, 3 (slots are
+    // counted from row i0, so any i0 works); rows up to i0+1 are prefetched.
     load3<P>(PF[1], RS, RM, bxp, NP4, rd, i0 - 3, nx, nz, jc, i0, i1);
     load3<P>(PF[2], RS, RM, bxp, NP4, rd, i0 - 2, nx, nz, jc, i0, i1);
     load3<P>(PF[3], RS, RM, bxp, NP4, rd, i0 - 1, nx, nz, jc, i0, i1);
@@ -180,24 +180,24 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             if constexpr (P == SP_SH) {
                 float zm1 = lane_m1(Sc[1]), zm2 = lane_m1(zm1), zp1 = lane_p1(Sc[1]);
                 double ddx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
-                double ddz = zm_ok ? dmn<EX>(zm2, zm1, Sc[1], zp1, KZ) : 0.0;
+                double ddz = lsel(zm_ok, dmn<EX>(zm2, zm1, Sc[1], zp1, KZ));
                 dsv[0] = (float)(ddx + ddz);
             } else {
                 float xzm1 = lane_m1(Sc[2]), xzm2 = lane_m1(xzm1), xzp1 = lane_p1(Sc[2]);
                 float zzm1 = lane_m1(Sc[1]), zzm2 = lane_m1(zzm1), zzp1 = lane_p1(Sc[1]);
                 double dxx = xm_ok ? dmn<EX>(Sa[0], Sb[0], Sc[0], Sd[0], KX) : 0.0;
                 double dxxz = xm_ok ? dmn<EX>(Sa[2], Sb[2], Sc[2], Sd[2], KX) : 0.0;
-                double dzxz = zm_ok ? dmn<EX>(xzm2, xzm1, Sc[2], xzp1, KZ) : 0.0;
-                double dzzz = zm_ok ? dmn<EX>(zzm2, zzm1, Sc[1], zzp1, KZ) : 0.0;
+                double dzxz = lsel(zm_ok, dmn<EX>(xzm2, xzm1, Sc[2], xzp1, KZ));
+                double dzzz = lsel(zm_ok, dmn<EX>(zzm2, zzm1, Sc[1], zzp1, KZ));
                 dsv[0] = (float)(dxx + dzxz);
                 dsv[1] = (float)(dxxz + dzzz);
                 if constexpr (P == SP_SPIN) {
                     float yzm1 = lane_m1(Sc[5]), yzm2 = lane_m1(yzm1), yzp1 = lane_p1(Sc[5]);
                     float yym1 = lane_m1(Sc[4]), yym2 = lane_m1(yym1), yyp1 = lane_p1(Sc[4]);
                     double dxyx = xm_ok ? dmn<EX>(Sa[3], Sb[3], Sc[3], Sd[3], KX) : 0.0;
-                    double dzyz = zm_ok ? dmn<EX>(yzm2, yzm1, Sc[5], yzp1, KZ) : 0.0;
+                    double dzyz = lsel(zm_ok, dmn<EX>(yzm2, yzm1, Sc[5], yzp1, KZ));
                     float dsy = (float)(dxyx + dzyz);
-                    double dzyy = zm_ok ? dmn<EX>(yym2, yym1, Sc[4], yyp1, KZ) : 0.0;
+                    double dzyy = lsel(zm_ok, dmn<EX>(yym2, yym1, Sc[4], yyp1, KZ));
                     double dxyy = xm_ok ? dmn<EX>(Sa[4], Sb[4], Sc[4], Sd[4], KX) : 0.0;
                     dsv[0] = (float)((double)dsv[0] + dzyy);
                     dsv[1] = (float)((double)dsv[1] - dxyy);
@@ -287,35 +287,45 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             float(&Wd)[NV] = W[C];
             const bool xp_ok = qr >= 1 && qr < nx - 2;
             const unsigned oq = (outl && qr >= i0 && qr < i1) ? (unsigned)(qr * nz + j) * 4u : OOB;
-            if constexpr (P == SP_SH) {
-                float zm1 = lane_m1(Wb[0]), zp1 = lane_p1(Wb[0]), zp2 = lane_p1(zp1);
-                const float dvdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
-                const float dvdz = zp_ok ? (float)dpl<EX>(zm1, Wb[0], zp1, zp2, KZ) : 0.f;
-                const double dtmu = dt * (double)cur.mu;
-                bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_sadd<EX>(Sa[0], dtmu, (double)dvdx));
-                bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_sadd<EX>(Sa[1], dtmu, (double)dvdz));
+            // rows before i0 are no output of the chunk: their arithmetic is
+            // skipped (the stores, masked there, always issue)
+            float sn[NS];
+            if (qr < i0) {
+#pragma unroll
+                for (int f = 0; f < NS; f++) sn[f] = 0.f;
             } else {
-                float xm1 = lane_m1(Wb[0]), xp1 = lane_p1(Wb[0]), xp2 = lane_p1(xp1);
-                float zm1 = lane_m1(Wb[1]), zp1 = lane_p1(Wb[1]), zp2 = lane_p1(zp1);
-                const float dvxdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
-                const float dvzdx = xp_ok ? (float)dpl<EX>(Wa[1], Wb[1], Wc[1], Wd[1], KX) : 0.f;
-                const float dvxdz = zp_ok ? (float)dpl<EX>(xm1, Wb[0], xp1, xp2, KZ) : 0.f;
-                const float dvzdz = zp_ok ? (float)dpl<EX>(zm1, Wb[1], zp1, zp2, KZ) : 0.f;
-                const float lam = cur.lam, mu = cur.mu;
-                const double l2m = (double)lam + 2.0 * (double)mu;
-                bst(RS, oq, slot_s(P, wr, 0) * NP4, upd_snorm<EX>(Sa[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz)));
-                bst(RS, oq, slot_s(P, wr, 1) * NP4, upd_snorm<EX>(Sa[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx)));
-                bst(RS, oq, slot_s(P, wr, 2) * NP4, upd_sadd<EX>(Sa[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx))));
-                if constexpr (P == SP_SPIN) {
-                    float cm1 = lane_m1(Wb[2]), cp1 = lane_p1(Wb[2]), cp2 = lane_p1(cp1);
-                    const float dcdx = xp_ok ? (float)dpl<EX>(Wa[2], Wb[2], Wc[2], Wd[2], KX) : 0.f;
-                    const float dcdz = zp_ok ? (float)dpl<EX>(cm1, Wb[2], cp1, cp2, KZ) : 0.f;
-                    const double dtg = dt * (double)cur.gam;
-                    bst(RS, oq, slot_s(P, wr, 4) * NP4, upd_srot<EX>(Sa[4], dt2, cur.nu, Wb[2], sub_rn(dvzdx, dvxdz)));
-                    bst(RS, oq, slot_s(P, wr, 3) * NP4, upd_sadd<EX>(Sa[3], dtg, (double)dcdx));
-                    bst(RS, oq, slot_s(P, wr, 5) * NP4, upd_sadd<EX>(Sa[5], dtg, (double)dcdz));
+                if constexpr (P == SP_SH) {
+                    float zm1 = lane_m1(Wb[0]), zp1 = lane_p1(Wb[0]), zp2 = lane_p1(zp1);
+                    const float dvdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
+                    const float dvdz = lsel(zp_ok, (float)dpl<EX>(zm1, Wb[0], zp1, zp2, KZ));
+                    const double dtmu = dt * (double)cur.mu;
+                    sn[0] = upd_sadd<EX>(Sa[0], dtmu, (double)dvdx);
+                    sn[1] = upd_sadd<EX>(Sa[1], dtmu, (double)dvdz);
+                } else {
+                    float xm1 = lane_m1(Wb[0]), xp1 = lane_p1(Wb[0]), xp2 = lane_p1(xp1);
+                    float zm1 = lane_m1(Wb[1]), zp1 = lane_p1(Wb[1]), zp2 = lane_p1(zp1);
+                    const float dvxdx = xp_ok ? (float)dpl<EX>(Wa[0], Wb[0], Wc[0], Wd[0], KX) : 0.f;
+                    const float dvzdx = xp_ok ? (float)dpl<EX>(Wa[1], Wb[1], Wc[1], Wd[1], KX) : 0.f;
+                    const float dvxdz = lsel(zp_ok, (float)dpl<EX>(xm1, Wb[0], xp1, xp2, KZ));
+                    const float dvzdz = lsel(zp_ok, (float)dpl<EX>(zm1, Wb[1], zp1, zp2, KZ));
+                    const float lam = cur.lam, mu = cur.mu;
+                    const double l2m = (double)lam + 2.0 * (double)mu;
+                    sn[0] = upd_snorm<EX>(Sa[0], dt, l2m, dvxdx, mul_rn(lam, dvzdz));
+                    sn[1] = upd_snorm<EX>(Sa[1], dt, l2m, dvzdz, mul_rn(lam, dvxdx));
+                    sn[2] = upd_sadd<EX>(Sa[2], dt, (double)mul_rn(mu, add_rn(dvxdz, dvzdx)));
+                    if constexpr (P == SP_SPIN) {
+                        float cm1 = lane_m1(Wb[2]), cp1 = lane_p1(Wb[2]), cp2 = lane_p1(cp1);
+                        const float dcdx = xp_ok ? (float)dpl<EX>(Wa[2], Wb[2], Wc[2], Wd[2], KX) : 0.f;
+                        const float dcdz = lsel(zp_ok, (float)dpl<EX>(cm1, Wb[2], cp1, cp2, KZ));
+                        const double dtg = dt * (double)cur.gam;
+                        sn[4] = upd_srot<EX>(Sa[4], dt2, cur.nu, Wb[2], sub_rn(dvzdx, dvxdz));
+                        sn[3] = upd_sadd<EX>(Sa[3], dtg, (double)dcdx);
+                        sn[5] = upd_sadd<EX>(Sa[5], dtg, (double)dcdz);
+                    }
                 }
             }
+#pragma unroll
+            for (int f = 0; f < NS; f++) bst(RS, oq, slot_s(P, wr, f) * NP4, sn[f]);
         }
     }
 }

@@ -390,10 +390,19 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     c->cur_fuse = fuse_ok(c, batch);
     std::vector<int4> items;
     int W = 0;
-    auto push = [&](int b, int j0, int i0) {
+    // a grid's rows in ceil(nx / chunk) chunks of equal length (within one
+    // row), so no wavefront marches a longer chunk than the others
+    auto chunks_of = [&](int nx) {
+        const int nch = (nx + chunk - 1) / chunk;
+        std::vector<int2> v;
+        for (int k = 0; k < nch; k++)
+            v.push_back(make_int2((int)((long long)k * nx / nch), (int)((long long)(k + 1) * nx / nch)));
+        return v;
+    };
+    auto push = [&](int b, int j0, int2 rows) {
         const Shot &sh = c->shots[batch[b]];
         const Grid &g = c->grids[sh.grid];
-        const int i1 = std::min(i0 + chunk, g.nx);
+        const int i0 = rows.x, i1 = rows.y;
         auto first = [](const std::vector<int> &v, long long key) {
             return (int)(std::lower_bound(v.begin(), v.end(), (int)std::max(key, -1LL)) - v.begin());
         };
@@ -417,11 +426,11 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
             for (size_t g = 0; g < per_grid.size(); g++) {
                 const Grid &G = c->grids[g];
                 const int nstrip = (G.nz + W - 1) / W;
-                for (int i0 = 0; i0 < G.nx; i0 += chunk)
+                for (int2 rows : chunks_of(G.nx))
                     for (int sg = 0; sg < nstrip; sg += 4)
                         for (int b : per_grid[g]) {
                             for (int sp = sg; sp < sg + 4; sp++) {
-                                if (sp < nstrip) push(b, sp * W, i0);
+                                if (sp < nstrip) push(b, sp * W, rows);
                                 else items.push_back(make_int4(-1, 0, 0, 0)), items.push_back(make_int4(0, 0, 0, 0));
                             }
                         }
@@ -438,9 +447,9 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
                 const Grid &G = c->grids[g];
                 for (size_t s0 = 0; s0 < list.size(); s0 += 4) {
                     const size_t s1 = std::min(list.size(), s0 + 4);
-                    for (int i0 = 0; i0 < G.nx; i0 += chunk)
+                    for (int2 rows : chunks_of(G.nx))
                         for (int j0 = 0; j0 < G.nz; j0 += W)
-                            for (size_t q = s0; q < s1; q++) push(list[q], j0, i0);
+                            for (size_t q = s0; q < s1; q++) push(list[q], j0, rows);
                 }
             }
         } else if (c->order == 1) {
@@ -456,15 +465,16 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
                 for (int sp = 0; sp < maxs; sp++)
                     for (int b = 0; b < (int)batch.size(); b++) {
                         const Grid &g = c->grids[c->shots[batch[b]].grid];
-                        const int i0 = ch * chunk, j0 = sp * W;
-                        if (i0 >= g.nx || j0 >= g.nz) continue;
-                        push(b, j0, i0);
+                        const std::vector<int2> rows = chunks_of(g.nx);
+                        const int j0 = sp * W;
+                        if (ch >= (int)rows.size() || j0 >= g.nz) continue;
+                        push(b, j0, rows[ch]);
                     }
         } else {
             for (int b = 0; b < (int)batch.size(); b++) {
                 const Grid &g = c->grids[c->shots[batch[b]].grid];
-                for (int i0 = 0; i0 < g.nx; i0 += chunk)
-                    for (int j0 = 0; j0 < g.nz; j0 += W) push(b, j0, i0);
+                for (int2 rows : chunks_of(g.nx))
+                    for (int j0 = 0; j0 < g.nz; j0 += W) push(b, j0, rows);
             }
         }
     };
@@ -492,17 +502,17 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
             for (int b = 0; b < (int)batch.size(); b++) {
                 const Grid &g = c->grids[c->shots[batch[b]].grid];
                 const std::vector<int> j0s = spans(g.nz);
-                for (int i0 = 0; i0 < g.nx; i0 += chunk)
-                    for (int j0 : j0s) push(b, j0, i0);
+                for (int2 rows : chunks_of(g.nx))
+                    for (int j0 : j0s) push(b, j0, rows);
             }
         } else {
             std::vector<std::vector<int>> per_grid(c->grids.size());
             for (int b = 0; b < (int)batch.size(); b++) per_grid[c->shots[batch[b]].grid].push_back(b);
             for (size_t g = 0; g < per_grid.size(); g++) {
                 const std::vector<int> j0s = spans(c->grids[g].nz);
-                for (int i0 = 0; i0 < c->grids[g].nx; i0 += chunk)
+                for (int2 rows : chunks_of(c->grids[g].nx))
                     for (int j0 : j0s)
-                        for (int b : per_grid[g]) push(b, j0, i0);
+                        for (int b : per_grid[g]) push(b, j0, rows);
             }
         }
         if (ensure(c, c->d_items_w, c->d_items_w_cap, items.size())) return 2;

@@ -140,6 +140,13 @@ __device__ __forceinline__ float upd_disp(float u, float v, double dt)
     if constexpr (EX) return disp_x(u, v, dt);
     else return (float)((double)u + (double)v * dt);
 }
+// v where the lane's condition holds, else 0.  The argument is evaluated
+// unconditionally, so a per-lane condition becomes a select instead of an
+// exec-masked branch around the stencil (scalar instructions and a break in
+// the scheduling region per use).
+__device__ __forceinline__ double lsel(bool ok, double v) { return ok ? v : 0.0; }
+__device__ __forceinline__ float lsel(bool ok, float v) { return ok ? v : 0.f; }
+
 // derivative coefficients of a shot in both forms
 struct DCo {
     double c1, c2, k9h, k9l, k1h, k1l;
