@@ -438,7 +438,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     const int4 item = a.items[2 * wave];
     if (item.x < 0) return;
     const int4 item2 = a.items[2 * wave + 1];
-    const ShotDesc &d = a.shots[item.x];
+    cdesc_t &d = ((cdesc_t *)a.shots)[item.x];
     const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     // snapshot slots of the pair's two steps (solver.py:193-203): given, or
     // in a replayed graph looked up for the step

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const int4 item = a.items[2 * wave];
     if (item.x < 0) return;
     const int4 item2 = a.items[2 * wave + 1];
-    const ShotDesc &d = a.shots[item.x];
+    cdesc_t &d = ((cdesc_t *)a.shots)[item.x];
     const int itx = a.itp ? a.it + *(cint_p)a.itp : a.it;
     const int nx = d.nx, nz = d.nz, i0 = item.z, i1 = item.w;
     const int j = item.y - a.shift + L;

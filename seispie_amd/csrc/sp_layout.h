@@ -50,6 +50,9 @@ struct ShotDesc {
 // loads, counted by lgkmcnt and never by the vector memory counter
 typedef const int __attribute__((address_space(4))) *cint_p;
 typedef const float __attribute__((address_space(4))) *cflt_p;
+// a shot descriptor read the same way: its fields stay scalar loads even
+// where the kernel has stored before (in a branch of a row loop)
+typedef const __attribute__((address_space(4))) struct ShotDesc cdesc_t;
 
 struct StepArgs {
     const ShotDesc *shots;
