@@ -79,11 +79,13 @@ __device__ __forceinline__ void load2(Row2<P, W> &p, rsrc_t RS, rsrc_t RM, int N
                                       unsigned vj, int i0, int i1)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
-    // rows clamped to what the chunk needs (and to the grid)
-    const int ms = min(max(mp, max(i0 - 6, 0)), min(i1 + 5, nx - 1));
-    const int ma = min(max(mp - 1, max(i0 - 4, 0)), min(i1 + 4, nx - 1));
-    const int mb = min(max(mp - 3, max(i0 - 3, 0)), min(i1 + 2, nx - 1));
-    const int mu_ = min(max(mp - 4, i0), i1 - 1);
+    // rows clamped to the rows the chunk reads, i0-6 .. i1+5 (and to the
+    // grid): one pair of bounds for every field, so the march keeps two
+    // scalars live for them instead of eight; the few prologue and epilogue
+    // rows this lets in are loaded but feed no output
+    const int lo = max(i0 - 6, 0), hi = min(i1 + 5, nx - 1);
+    const int ms = min(max(mp, lo), hi), ma = min(max(mp - 1, lo), hi);
+    const int mb = min(max(mp - 3, lo), hi), mu_ = min(max(mp - 4, lo), hi);
     const int rb = nz * 4;
     const int os = ms * rb, oa = ma * rb, ob = mb * rb, ou = mu_ * rb;
 #pragma unroll
@@ -383,19 +385,21 @@ __device__ __forceinline__ void str_row(float (&sn)[n_s(P)][W], const float (&wa
 
 // sources (forward) or adjoint sources (adjoint) of velocity row r at step
 // `it`: float32 adds in list order (stf_dsy / stf_dsxz, solver.py:78-93);
-// j is the lane's first cell
+// j is the lane's first cell; pi and src_row: the next list entry and its
+// x row
 template <int P, int W>
-__device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, int nz, int &pi, int &src_row, int ni,
-                                       cint_p ik, cint_p irow, cflt_p ival, long long nrows, int nt, int it)
+__device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, int nz, int &pi, int &src_row,
+                                       cdesc_t *dp, int adjoint, int nt, int it)
 {
     if (r != src_row) return;
+    const InjLists L = inj_lists(fresh(dp), adjoint);
     const int kend = (r + 1) * nz, kr = r * nz + j;
-    while (pi < ni && ik[pi] < kend) {
-        const int kk = ik[pi];
-        const int row = irow[pi];
-        const float s0 = ival[(long long)row * nt + it];
+    while (pi < L.n && L.k[pi] < kend) {
+        const int kk = L.k[pi];
+        const int row = L.row[pi];
+        const float s0 = L.val[(long long)row * nt + it];
         float s1 = 0.f;
-        if constexpr (P != SP_SH) s1 = ival[(nrows + row) * nt + it];
+        if constexpr (P != SP_SH) s1 = L.val[(L.nrows + row) * nt + it];
 #pragma unroll
         for (int c = 0; c < W; c++) {
             if (kk == kr + c) {
@@ -405,7 +409,7 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, in
         }
         ++pi;
     }
-    src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+    src_row = pi < L.n ? L.xi[pi] : 0x7fffffff;
 }
 
 // The march of one work item.  Strips (G false): four independent
@@ -503,27 +507,28 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     const unsigned vj = (unsigned)jc * 4u;            // per-lane load offset
     const unsigned vo = outl ? (unsigned)j * 4u : OOB;  // per-lane store offset (OOB: not written)
 
-    const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
-    const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
-    const int ni = a.adjoint ? d.nrec : d.nsrc;
-    const cflt_p ival = (cflt_p)(a.adjoint ? (const float *)d.adstf : d.stf);
-    const long long nrows = a.adjoint ? d.nrec : d.nsrc;
     // injection pointers: step it from row i0-4, step it+1 from row i0-1
-    int p1 = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
-    int p2 = p1;
-    while (p2 < ni && ik[p2] < (i0 - 1) * nz) ++p2;
-    int row1 = p1 < ni ? ik[p1] / nz : 0x7fffffff;
-    int row2 = p2 < ni ? ik[p2] / nz : 0x7fffffff;
+    int p1, p2, row1, row2;
+    {
+        const InjLists L = inj_lists(&d, a.adjoint);
+        p1 = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
+        p2 = p1;
+        while (p2 < L.n && L.k[p2] < (i0 - 1) * nz) ++p2;
+        row1 = p1 < L.n ? L.xi[p1] : 0x7fffffff;
+        row2 = p2 < L.n ? L.xi[p2] : 0x7fffffff;
+    }
     // receivers recorded by this chunk (forward only): rows i0..i1-1; the
     // next two receivers' cells and record rows wait in scalar registers
     // (a row's test uses values loaded rows earlier)
-    const cint_p rk = (cint_p)d.rec_k;
-    const cint_p rrow = (cint_p)d.rec_row;
-    const int nrec = a.adjoint ? 0 : d.nrec;
-    int pr = __builtin_amdgcn_readfirstlane(item2.z);
-    int nk1 = pr < nrec ? rk[pr] : 0x7fffffff, nr1 = pr < nrec ? rrow[pr] : 0;
-    int nk2 = pr + 1 < nrec ? rk[pr + 1] : 0x7fffffff, nr2 = pr + 1 < nrec ? rrow[pr + 1] : 0;
-    const rsrc_t RR = make_rsrc(d.rec, (long long)n_rec(P) * d.nrec * a.nt * 4);
+    int pr = __builtin_amdgcn_readfirstlane(item2.z), nk1, nr1, nk2, nr2;
+    {
+        const cint_p rk = (cint_p)d.rec_k, rrow = (cint_p)d.rec_row;
+        const int nrec = a.adjoint ? 0 : d.nrec;
+        nk1 = pr < nrec ? rk[pr] : 0x7fffffff;
+        nr1 = pr < nrec ? rrow[pr] : 0;
+        nk2 = pr + 1 < nrec ? rk[pr + 1] : 0x7fffffff;
+        nr2 = pr + 1 < nrec ? rrow[pr + 1] : 0;
+    }
 
     float S0[4][NS][W], W1[4][NV][W], S1[4][NS][W], W2[4][NV][W];
     // materials of rows m-1 (rho, j) and m-3 (lam, mu, nu, gam) by
@@ -632,7 +637,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                     for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X0 + k);
                 }
                 vel_row<P, EX, W, G>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                inject<P, W>(dsv, r, j, nz, p1, row1, ni, ik, irow, ival, nrows, a.nt, itx);
+                inject<P, W>(dsv, r, j, nz, p1, row1, &d, a.adjoint, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
                 for (int c = 0; c < W; c++) {
@@ -701,7 +706,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                         for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X2 + k);
                     }
                     vel_row<P, EX, W, G>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                    inject<P, W>(dsv, r, j, nz, p2, row2, ni, ik, irow, ival, nrows, a.nt, itx + 1);
+                    inject<P, W>(dsv, r, j, nz, p2, row2, &d, a.adjoint, a.nt, itx + 1);
                     const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
                     for (int c = 0; c < W; c++) {
@@ -766,7 +771,11 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                 // save_obs of both steps (solver.py:151-156) for receivers in
                 // this row; rows past the chunk belong to the next chunk
                 if (r < i1 && nk1 < (r + 1) * nz) {
-                    const int kr = r * nz + j, cs = d.nrec * a.nt * 4;
+                    cdesc_t *dq = fresh(&d);
+                    const cint_p rk = (cint_p)dq->rec_k, rrow = (cint_p)dq->rec_row;
+                    const int nrec = dq->nrec;
+                    const rsrc_t RR = make_rsrc(dq->rec, (long long)n_rec(P) * nrec * a.nt * 4);
+                    const int kr = r * nz + j, cs = nrec * a.nt * 4;
                     do {
                         const int c = nk1 - kr;  // 0 .. W-1 in the lane holding the receiver
                         const unsigned orc = (outl && c >= 0 && c < W) ? (unsigned)(nr1 * a.nt + itx) * 4u : OOB;

@@ -81,7 +81,7 @@ struct Grid {
 struct Shot {
     int grid = 0, nsrc = 0, nrec = 0;
     std::vector<int> src_sorted, rec_sorted;  // grid indices in list order
-    int *d_idx = nullptr;  // src_k, src_row, rec_k, rec_row
+    int *d_idx = nullptr;  // src_k, src_row, rec_k, rec_row, src_i, rec_i
     float *stf = nullptr, *state = nullptr, *rec = nullptr, *data = nullptr, *adstf = nullptr;
     float *snap = nullptr;  // into the context's snapshot pool
     float *state2 = nullptr;  // second state block: forward recomputation from checkpoints
@@ -295,6 +295,8 @@ ShotDesc make_desc(const sp_ctx *c, const Shot &s)
     d.src_row = s.d_idx + s.nsrc;
     d.rec_k = s.d_idx + 2 * s.nsrc;
     d.rec_row = s.d_idx + 2 * s.nsrc + s.nrec;
+    d.src_i = s.d_idx + 2 * s.nsrc + 2 * s.nrec;
+    d.rec_i = s.d_idx + 3 * s.nsrc + 2 * s.nrec;
     d.nsrc = s.nsrc;
     d.nrec = s.nrec;
     d.nx = g.nx;
@@ -1709,14 +1711,18 @@ int sp_add_shot(sp_ctx *c, int grid, int nsrc, const int *src_k, const float *st
     std::iota(ro.begin(), ro.end(), 0);
     std::stable_sort(so.begin(), so.end(), [&](int a, int b) { return src_k[a] < src_k[b]; });
     std::stable_sort(ro.begin(), ro.end(), [&](int a, int b) { return rec_k[a] < rec_k[b]; });
-    std::vector<int> idx(2 * (size_t)nsrc + 2 * (size_t)nrec + 1, 0);
+    // [src_k | src_row | rec_k | rec_row | src_i | rec_i]: grid index, row
+    // of the time function / trace, and x row of the grid
+    std::vector<int> idx(3 * (size_t)nsrc + 3 * (size_t)nrec + 1, 0);
     for (int q = 0; q < nsrc; q++) {
         idx[q] = src_k[so[q]];
         idx[nsrc + q] = so[q];
+        idx[2 * nsrc + 2 * nrec + q] = src_k[so[q]] / G.nz;
     }
     for (int q = 0; q < nrec; q++) {
         idx[2 * nsrc + q] = rec_k[ro[q]];
         idx[2 * nsrc + nrec + q] = ro[q];
+        idx[3 * nsrc + 2 * nrec + q] = rec_k[ro[q]] / G.nz;
     }
     s.src_sorted.assign(idx.begin(), idx.begin() + nsrc);
     s.rec_sorted.assign(idx.begin() + 2 * nsrc, idx.begin() + 2 * nsrc + nrec);

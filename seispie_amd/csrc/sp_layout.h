@@ -44,6 +44,7 @@ struct ShotDesc {
     double c1x, c2x, c1z, c2z;  // 9/(8h), 1/(24h)
     // exact mode: 9/(8h) and 1/(24h) as unevaluated sums hi + lo of two doubles
     double k9hx, k9lx, k1hx, k1lx, k9hz, k9lz, k1hz, k1lz;
+    const int *src_i, *rec_i;  // x row (k / nz) of each sorted source / receiver
 };
 
 // read-only, wave-uniform data through the constant address space: scalar
@@ -53,6 +54,35 @@ typedef const float __attribute__((address_space(4))) *cflt_p;
 // a shot descriptor read the same way: its fields stay scalar loads even
 // where the kernel has stored before (in a branch of a row loop)
 typedef const __attribute__((address_space(4))) struct ShotDesc cdesc_t;
+
+// The injection lists of a launch: sources (forward) or the receivers with
+// their adjoint sources (adjoint), sorted by grid index
+struct InjLists {
+    cint_p k, row, xi;  // grid index, row of the time function, x row
+    cflt_p val;         // time functions [n_inj][nrows][nt]
+    int n;
+    long long nrows;
+};
+__device__ __forceinline__ InjLists inj_lists(cdesc_t *dp, int adjoint)
+{
+    InjLists L;
+    L.k = (cint_p)(adjoint ? dp->rec_k : dp->src_k);
+    L.row = (cint_p)(adjoint ? dp->rec_row : dp->src_row);
+    L.xi = (cint_p)(adjoint ? dp->rec_i : dp->src_i);
+    L.val = (cflt_p)(adjoint ? (const float *)dp->adstf : dp->stf);
+    L.n = adjoint ? dp->nrec : dp->nsrc;
+    L.nrows = L.n;
+    return L;
+}
+// A descriptor pointer the optimiser cannot follow from one row to the
+// next: what a row loop reads through it in a rarely taken branch (list
+// bases, record buffer) is loaded there, from the scalar cache, instead of
+// being held in (and spilled from) SGPRs for the whole march.
+__device__ __forceinline__ cdesc_t *fresh(cdesc_t *dp)
+{
+    asm volatile("" : "+s"(dp));
+    return dp;
+}
 
 struct StepArgs {
     const ShotDesc *shots;
