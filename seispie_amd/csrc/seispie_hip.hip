@@ -369,7 +369,7 @@ int steps_next(const sp_ctx *c, int it, int end, int inter_sae, int ckpt_C)
 
 int auto_chunk(const sp_ctx *c, const std::vector<int> &batch)
 {
-    if (c->chunk > 0) return batch_kernel(c, batch) == 1 ? (c->chunk + 3) / 4 * 4 : c->chunk;
+    if (c->chunk > 0) return c->chunk;
     // enough wavefronts to fill 256 CUs several times over, long chunks otherwise
     long long waves64 = 0;
     for (int s : batch) {
