@@ -8,7 +8,7 @@ L-BFGS and against the two non-learned conditioners it is built from:
 The gain over plain L-BFGS must hold on both models: after one iteration
 the conditioned run is below plain's misfit after four.  The non-learned
 rows are measured and printed, not ranked: over 20 iterations (bench.py's
-workload, DESIGN records the numbers) the source-side pseudo-Hessian alone
+FWI workload) the source-side pseudo-Hessian alone
 reaches plain's final misfit as fast as the network (0.40 / 0.43 of plain's
 time against 0.34 / 0.51), so the gain comes from illumination
 compensation that the network reproduces from the solver's own fields."""
