@@ -422,7 +422,7 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, in
 // m-2, after one barrier per iteration.  Groups that do not reach the
 // column's ends keep 6 halo cells there, as strips do; wavefronts past the
 // column's last cell take part in the barriers only.
-template <int P, bool EX, int W, bool G>
+template <int P, bool EX, int W, bool G, bool SQ>
 __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
 {
     static_assert(W == 1 || (W == 2 && P != SP_SPIN), "two cells per lane: SH and P-SV");
@@ -498,7 +498,11 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     }
     const int NP4 = (int)(d.Npad * 4);
     const double dt = a.dt;
-    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
+    // square cells (SQ: dx = dz in every grid of the batch): one set of
+    // difference coefficients serves both axes, eight scalar registers fewer
+    // for the march (1-2% faster, the same values)
+    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx};
+    const DCo KZ = SQ ? KX : DCo{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
     const int rd = a.rd, wr = a.rd ^ 1;
     const bool zint = __all(bz_one);
     const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);
@@ -820,16 +824,16 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     }
 }
 
-template <int P, bool EX, int W>
+template <int P, bool EX, int W, bool SQ>
 __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 {
-    step2_body<P, EX, W, false>(a, nullptr);
+    step2_body<P, EX, W, false, SQ>(a, nullptr);
 }
 
 // column groups: blocks of up to grp_max wavefronts
-template <int P, bool EX, int W>
+template <int P, bool EX, int W, bool SQ>
 __global__ __launch_bounds__(64 * grp_max(P, W)) void k_step2g(StepArgs a)
 {
     __shared__ float xl[4 * grp_max(P, W) * n_xch(P) * 4];
-    step2_body<P, EX, W, true>(a, xl);
+    step2_body<P, EX, W, true, SQ>(a, xl);
 }

@@ -115,6 +115,7 @@ struct sp_ctx {
     size_t d_snaptab_cap = 0;
     int inter_sae = 0, inter_nsa = 0;    // interactions of the adjoint run being captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
+    int cur_sq = 0;            // every grid of the prepared batch has square cells (dx = dz)
     int group = 0;             // column groups: wavefronts per block (0: 4)
     int grp_waves = 0;         // column groups of the prepared batch (cur_fuse 3, 4): wavefronts per block
     int graph = 0;
@@ -390,6 +391,11 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     const int chunk = auto_chunk(c, batch);
     c->cur_kernel = batch_kernel(c, batch);
     c->cur_fuse = fuse_ok(c, batch);
+    c->cur_sq = 1;
+    for (int sb : batch) {
+        const Grid &g = c->grids[c->shots[sb].grid];
+        c->cur_sq = c->cur_sq && g.dx == g.dz;
+    }
     std::vector<int4> items;
     int W = 0;
     // a grid's rows in ceil(nx / chunk) chunks of equal length (within one
@@ -541,6 +547,24 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     return 0;
 }
 
+// a two-step kernel: strips (k_step2) or column groups (k_step2g) of W cells
+// per lane, exact or fast arithmetic, square cells or not
+template <int P, int W>
+void launch_two(bool group, bool ex, bool sq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+{
+    if (group) {
+        if (ex && sq) hipLaunchKernelGGL((k_step2g<P, true, W, true>), grid, block, 0, st, a);
+        else if (ex) hipLaunchKernelGGL((k_step2g<P, true, W, false>), grid, block, 0, st, a);
+        else if (sq) hipLaunchKernelGGL((k_step2g<P, false, W, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_step2g<P, false, W, false>), grid, block, 0, st, a);
+    } else {
+        if (ex && sq) hipLaunchKernelGGL((k_step2<P, true, W, true>), grid, block, 0, st, a);
+        else if (ex) hipLaunchKernelGGL((k_step2<P, true, W, false>), grid, block, 0, st, a);
+        else if (sq) hipLaunchKernelGGL((k_step2<P, false, W, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_step2<P, false, W, false>), grid, block, 0, st, a);
+    }
+}
+
 // One launch from step `it`: `nsteps` (1 or 2) steps of every shot of the
 // prepared batch, reading stress/velocity set `rd` and writing set rd ^ 1.
 int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd, int nsteps = 1,
@@ -564,33 +588,20 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     a.wout = strip_width(c);
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
     const bool ex = c->exact;
+    const bool sq = c->cur_sq;
     if (nsteps == 2 && c->cur_fuse >= 3) {
         a.items = c->d_items_w;
         a.nitems = c->n_items_w;
         grid = dim3((unsigned)c->n_items_w);
         block = dim3((unsigned)(64 * c->grp_waves));
         if (c->cur_fuse == 4) {
-            if (c->physics == SP_SH) {
-                if (ex) hipLaunchKernelGGL((k_step2g<SP_SH, true, 2>), grid, block, 0, c->stream, a);
-                else hipLaunchKernelGGL((k_step2g<SP_SH, false, 2>), grid, block, 0, c->stream, a);
-            } else {
-                if (ex) hipLaunchKernelGGL((k_step2g<SP_PSV, true, 2>), grid, block, 0, c->stream, a);
-                else hipLaunchKernelGGL((k_step2g<SP_PSV, false, 2>), grid, block, 0, c->stream, a);
-            }
+            if (c->physics == SP_SH) launch_two<SP_SH, 2>(true, ex, sq, grid, block, c->stream, a);
+            else launch_two<SP_PSV, 2>(true, ex, sq, grid, block, c->stream, a);
         } else {
             switch (c->physics) {
-            case SP_SH:
-                if (ex) hipLaunchKernelGGL((k_step2g<SP_SH, true, 1>), grid, block, 0, c->stream, a);
-                else hipLaunchKernelGGL((k_step2g<SP_SH, false, 1>), grid, block, 0, c->stream, a);
-                break;
-            case SP_PSV:
-                if (ex) hipLaunchKernelGGL((k_step2g<SP_PSV, true, 1>), grid, block, 0, c->stream, a);
-                else hipLaunchKernelGGL((k_step2g<SP_PSV, false, 1>), grid, block, 0, c->stream, a);
-                break;
-            default:
-                if (ex) hipLaunchKernelGGL((k_step2g<SP_SPIN, true, 1>), grid, block, 0, c->stream, a);
-                else hipLaunchKernelGGL((k_step2g<SP_SPIN, false, 1>), grid, block, 0, c->stream, a);
-                break;
+            case SP_SH: launch_two<SP_SH, 1>(true, ex, sq, grid, block, c->stream, a); break;
+            case SP_PSV: launch_two<SP_PSV, 1>(true, ex, sq, grid, block, c->stream, a); break;
+            default: launch_two<SP_SPIN, 1>(true, ex, sq, grid, block, c->stream, a); break;
             }
         }
     } else if (nsteps == 2 && c->cur_fuse == 2) {
@@ -598,28 +609,14 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
         a.nitems = c->n_items_w;
         a.wout = 116;
         grid = dim3((unsigned)((c->n_items_w + 3) / 4));
-        if (c->physics == SP_SH) {
-            if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true, 2>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_SH, false, 2>), grid, block, 0, c->stream, a);
-        } else {
-            if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true, 2>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_PSV, false, 2>), grid, block, 0, c->stream, a);
-        }
+        if (c->physics == SP_SH) launch_two<SP_SH, 2>(false, ex, sq, grid, block, c->stream, a);
+        else launch_two<SP_PSV, 2>(false, ex, sq, grid, block, c->stream, a);
     } else if (nsteps == 2) {
         if (!c->cur_fuse) return fail(c, "internal: two-step launch without the two-step geometry");
         switch (c->physics) {
-        case SP_SH:
-            if (ex) hipLaunchKernelGGL((k_step2<SP_SH, true, 1>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_SH, false, 1>), grid, block, 0, c->stream, a);
-            break;
-        case SP_PSV:
-            if (ex) hipLaunchKernelGGL((k_step2<SP_PSV, true, 1>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_PSV, false, 1>), grid, block, 0, c->stream, a);
-            break;
-        default:
-            if (ex) hipLaunchKernelGGL((k_step2<SP_SPIN, true, 1>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step2<SP_SPIN, false, 1>), grid, block, 0, c->stream, a);
-            break;
+        case SP_SH: launch_two<SP_SH, 1>(false, ex, sq, grid, block, c->stream, a); break;
+        case SP_PSV: launch_two<SP_PSV, 1>(false, ex, sq, grid, block, c->stream, a); break;
+        default: launch_two<SP_SPIN, 1>(false, ex, sq, grid, block, c->stream, a); break;
         }
     } else if (c->cur_kernel == 1) {
         switch (c->physics) {
