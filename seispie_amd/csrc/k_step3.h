@@ -79,7 +79,7 @@ __device__ __forceinline__ void load3(Row3<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
     }
 }
 
-template <int P, bool EX>
+template <int P, bool EX, bool SQ>
 __global__ __launch_bounds__(256) void k_step3(StepArgs a)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
@@ -102,7 +102,9 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
     const int NP4 = (int)(d.Npad * 4);  // bytes per field
     const double dt = a.dt, dt2 = 2.0 * a.dt;
-    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx}, KZ{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
+    // square cells (SQ): one set of difference coefficients for both axes
+    const DCo KX{d.c1x, d.c2x, d.k9hx, d.k9lx, d.k1hx, d.k1lx};
+    const DCo KZ = SQ ? KX : DCo{d.c1z, d.c2z, d.k9hz, d.k9lz, d.k1hz, d.k1lz};
     const int rd = a.rd, wr = a.rd ^ 1;
     const double bzb = d.bzb[jc], bzt = d.bzt[jc];
     const rsrc_t RS = make_rsrc(d.state, (long long)n_slots(P) * NP4);

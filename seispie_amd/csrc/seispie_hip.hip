@@ -565,6 +565,16 @@ void launch_two(bool group, bool ex, bool sq, dim3 grid, dim3 block, hipStream_t
     }
 }
 
+// the one-step kernel k_step3, exact or fast arithmetic, square cells or not
+template <int P>
+void launch_one(bool ex, bool sq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+{
+    if (ex && sq) hipLaunchKernelGGL((k_step3<P, true, true>), grid, block, 0, st, a);
+    else if (ex) hipLaunchKernelGGL((k_step3<P, true, false>), grid, block, 0, st, a);
+    else if (sq) hipLaunchKernelGGL((k_step3<P, false, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_step3<P, false, false>), grid, block, 0, st, a);
+}
+
 // One launch from step `it`: `nsteps` (1 or 2) steps of every shot of the
 // prepared batch, reading stress/velocity set `rd` and writing set rd ^ 1.
 int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd, int nsteps = 1,
@@ -620,18 +630,9 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
         }
     } else if (c->cur_kernel == 1) {
         switch (c->physics) {
-        case SP_SH:
-            if (ex) hipLaunchKernelGGL((k_step3<SP_SH, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step3<SP_SH, false>), grid, block, 0, c->stream, a);
-            break;
-        case SP_PSV:
-            if (ex) hipLaunchKernelGGL((k_step3<SP_PSV, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step3<SP_PSV, false>), grid, block, 0, c->stream, a);
-            break;
-        default:
-            if (ex) hipLaunchKernelGGL((k_step3<SP_SPIN, true>), grid, block, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_step3<SP_SPIN, false>), grid, block, 0, c->stream, a);
-            break;
+        case SP_SH: launch_one<SP_SH>(ex, sq, grid, block, c->stream, a); break;
+        case SP_PSV: launch_one<SP_PSV>(ex, sq, grid, block, c->stream, a); break;
+        default: launch_one<SP_SPIN>(ex, sq, grid, block, c->stream, a); break;
         }
     } else {
         switch (c->physics) {
