@@ -206,10 +206,14 @@ int sp_smooth_noalloc(sp_ctx *ctx, float *field_dev, int nx, int nz, int sigma);
 /* Autotuner: times `steps` forward steps of the current shots for each
  * rows-per-wavefront chunk, work order and one or two steps per launch, in
  * strips or in column groups of 2, 4 or 8 wavefronts (SP_OPT_FUSE,
- * SP_OPT_GROUP), keeps the fastest for later runs and reports it; the choice is
- * remembered for the process and reused for the same physics and grids.  Overwrites wavefields and records (every run clears
- * them first).  No reference counterpart (the reference launches one
- * thread per cell with threads_per_block from the config, fd2d.py:185-187). */
+ * SP_OPT_GROUP); the chunks are a fixed list plus, for 2 to 6 wavefronts per
+ * SIMD, the shortest chunk whose launch fits in that many.  The six fastest
+ * are timed again over more steps (at least 3 x steps, about 25 ms) and the
+ * fastest of those is kept for later runs and reported; the choice is
+ * remembered for the process and reused for the same physics and grids.
+ * Overwrites wavefields and records (every run clears them first).  No
+ * reference counterpart (the reference launches one thread per cell with
+ * threads_per_block from the config, fd2d.py:185-187). */
 int sp_autotune(sp_ctx *ctx, int steps, int *chunk_out, int *order_out);
 /* current value of an option (SP_OPT_*), e.g. the autotuner's SP_OPT_FUSE choice */
 int sp_get_option(const sp_ctx *ctx, int key, long long *value);

@@ -2090,6 +2090,31 @@ int sp_smooth_noalloc(sp_ctx *c, float *field, int nx, int nz, int sigma)
     return smooth_launch(c, field, (float *)(base + SCRATCH_W), base, nx, nz);
 }
 
+// Wavefronts one launch of the batch takes at chunk length `chunk` under the
+// current fuse and group options: the work lists prepare_batch builds
+// (strips of strip_width cells, or column-group spans of 64 W gb cells
+// overlapping by 12), times ceil(nx / chunk) chunks per shot.
+long long launch_waves(const sp_ctx *c, const std::vector<int> &batch, int chunk)
+{
+    const int fu = fuse_ok(c, batch), kern = batch_kernel(c, batch);
+    long long n = 0;
+    for (int s : batch) {
+        const Grid &g = c->grids[c->shots[s].grid];
+        long long per;
+        if (fu >= 3) {
+            const int gb = c->group > 0 ? c->group : 4, span = 64 * (fu == 4 ? 2 : 1) * gb;
+            long long spans = 1;
+            for (int j0 = 0; j0 + span < g.nz; j0 += span - 12) spans++;
+            per = spans * gb;
+        } else {
+            const int w = fu == 2 ? 116 : (fu ? 52 : (kern == 1 ? c->wout : 58));
+            per = (g.nz + w - 1) / w;
+        }
+        n += per * ((g.nx + chunk - 1) / chunk);
+    }
+    return n;
+}
+
 int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
 {
     if (int e = check_ctx(c)) return e;
@@ -2119,7 +2144,27 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
             return 0;
         }
     }
-    const int chunks[] = {4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 200, 256};
+    const int fixed_chunks[] = {4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 200, 256};
+    int nsimd = 0, max_nx = 0;
+    HIPCHECK(c, hipDeviceGetAttribute(&nsimd, hipDeviceAttributeMultiprocessorCount, c->device));
+    nsimd *= 4;
+    for (int s : all) max_nx = std::max(max_nx, c->grids[c->shots[s].grid].nx);
+    // the fixed chunk lengths, and for k = 2..6 wavefronts per SIMD the
+    // shortest chunk whose launch fits in k full rounds of the SIMDs (a
+    // launch a little over a round boundary leaves most SIMDs idle while a
+    // few run one wavefront more)
+    auto chunk_list = [&]() {
+        std::vector<int> v(std::begin(fixed_chunks), std::end(fixed_chunks));
+        for (int k = 2; k <= 6; k++)
+            for (int ch = 4; ch <= std::max(4, max_nx); ch++)
+                if (launch_waves(c, all, ch) <= (long long)k * nsimd) {
+                    v.push_back(ch);
+                    break;
+                }
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        return v;
+    };
     // one or two steps per launch, two steps in strips of one or two cells
     // per lane and in column groups, each where the batch allows it (all
     // give the same results)
@@ -2156,16 +2201,20 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
         HIPCHECK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
         return 0;
     };
-    // every setting for `steps` steps, then the four fastest again for
-    // three times as many (a 20-step time moves by a few percent from run
-    // to run, as much as neighbouring settings differ)
+    // every setting for `steps` steps, then the six fastest again for at
+    // least three times as many and about 25 ms each (a 20-step time moves
+    // by a few percent from run to run, as much as neighbouring settings
+    // differ, and by more where a step takes tens of microseconds)
     std::vector<std::pair<float, std::array<int, 4>>> timed;
     for (int fu : fuses) {
         // column groups: two, four or eight wavefronts per block, and two
         // orders (shot-major and chunk-major)
         std::vector<int> groups = {c->group};
         if (fu >= 3) groups = {2, 4, 8};
-        for (int gr : groups)
+        for (int gr : groups) {
+            c->fuse = fu;
+            c->group = gr;
+            const std::vector<int> chunks = chunk_list();
             for (int order = 0; order < (fu >= 3 ? 2 : 4); order++)
                 for (int ch : chunks) {
                     const std::array<int, 4> st = {fu, gr, order, ch};
@@ -2173,13 +2222,15 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
                     if (time_setting(st, steps, &ms)) return 2;
                     timed.push_back({ms, st});
                 }
+        }
     }
     std::sort(timed.begin(), timed.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    const int steps2 = std::min(c->nt, std::max(3 * steps, (int)std::ceil(25.0 * steps / std::max(timed[0].first, 1e-3f))));
     float best = 1e30f;
     std::array<int, 4> win = timed[0].second;
-    for (size_t q = 0; q < std::min<size_t>(4, timed.size()); q++) {
+    for (size_t q = 0; q < std::min<size_t>(6, timed.size()); q++) {
         float ms = 0;
-        if (time_setting(timed[q].second, std::min(3 * steps, c->nt), &ms)) return 2;
+        if (time_setting(timed[q].second, steps2, &ms)) return 2;
         if (ms < best) {
             best = ms;
             win = timed[q].second;

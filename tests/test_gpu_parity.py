@@ -186,6 +186,30 @@ def test_tiny_grids(ctx_factory, fuse):
         _check_forward(prob, recs, nonzero=False)
 
 
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+def test_autotuned_forward_equals_oracle_bit_for_bit(ctx_factory, physics):
+    """sp_autotune times the launch settings (one or two steps per launch,
+    strips or column groups, work orders, fixed and occupancy-derived chunk
+    lengths) on the loaded shots and leaves the fastest one set: the forward
+    run after it equals the oracle bit for bit, and a second context with
+    the same batch shape takes the cached choice."""
+    prob = _small(physics, nt=120)
+    print("case: autotune", physics, flush=True)
+    ctx = ctx_factory()
+    ctx.set_option(_lib.OPT_FUSE, 4)
+    ids = configs.load_problem(ctx, prob)
+    ch, od = ctx.autotune(10)
+    assert ch >= 4 and 0 <= od <= 3
+    choice = (ch, od, ctx.get_option(_lib.OPT_FUSE), ctx.get_option(_lib.OPT_GROUP))
+    ctx.forward(0)
+    _check_forward(prob, [ctx.records(i) for i in ids])
+    ctx2 = ctx_factory()
+    ctx2.set_option(_lib.OPT_FUSE, 4)
+    configs.load_problem(ctx2, prob)
+    ch2, od2 = ctx2.autotune(10)
+    assert (ch2, od2, ctx2.get_option(_lib.OPT_FUSE), ctx2.get_option(_lib.OPT_GROUP)) == choice
+
+
 def _data_for(prob, scale=1.02):
     """Observed data: oracle records of a perturbed model (mu and rho)."""
     out = []
