@@ -411,6 +411,23 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, in
     }
     src_row = pi < L.n ? L.xi[pi] : 0x7fffffff;
 }
+// the same adds from an injection queue in LDS (adjoint launches, with an
+// adjoint source at every receiver)
+template <int P, int W>
+__device__ __forceinline__ void inject_q(float (&dsv)[n_v(P)][W], int r, int j, int nz, InjQ &q, cdesc_t *dp,
+                                         int adjoint, int nt, int it)
+{
+    const int kr = r * nz + j;
+    injq_row<P>(q, r, dp, adjoint, nt, it, [&](int kk, float s0, float s1) {
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            if (kk == kr + c) {
+                dsv[0][c] = add_rn(dsv[0][c], s0);
+                if constexpr (P != SP_SH) dsv[1][c] = add_rn(dsv[1][c], s1);
+            }
+        }
+    });
+}
 
 // The march of one work item.  Strips (G false): four independent
 // wavefronts per block, each a strip with its own halo lanes.  Column
@@ -422,8 +439,8 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, in
 // m-2, after one barrier per iteration.  Groups that do not reach the
 // column's ends keep 6 halo cells there, as strips do; wavefronts past the
 // column's last cell take part in the barriers only.
-template <int P, bool EX, int W, bool G, bool SQ>
-__device__ __forceinline__ void step2_body(StepArgs a, float *xl)
+template <int P, bool EX, int W, bool G, bool SQ, bool IQ>
+__device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
 {
     static_assert(W == 1 || (W == 2 && P != SP_SPIN), "two cells per lane: SH and P-SV");
     constexpr int NS = n_s(P), NV = n_v(P);
@@ -511,15 +528,24 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     const unsigned vj = (unsigned)jc * 4u;            // per-lane load offset
     const unsigned vo = outl ? (unsigned)j * 4u : OOB;  // per-lane store offset (OOB: not written)
 
-    // injection pointers: step it from row i0-4, step it+1 from row i0-1
+    // injection pointers: step it from row i0-4, step it+1 from row i0-1;
+    // IQ (adjoint launches): queues of the entries in LDS
     int p1, p2, row1, row2;
+    InjQ q1, q2;
     {
         const InjLists L = inj_lists(&d, a.adjoint);
         p1 = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
         p2 = p1;
         while (p2 < L.n && L.k[p2] < (i0 - 1) * nz) ++p2;
-        row1 = p1 < L.n ? L.xi[p1] : 0x7fffffff;
-        row2 = p2 < L.n ? L.xi[p2] : 0x7fffffff;
+        if constexpr (IQ) {
+            q1.s = iq + 2 * IQ_WORDS * gw;
+            q2.s = q1.s + IQ_WORDS;
+            injq_fill<P>(q1, &d, a.adjoint, a.nt, itx, p1);
+            injq_fill<P>(q2, &d, a.adjoint, a.nt, itx + 1, p2);
+        } else {
+            row1 = p1 < L.n ? L.xi[p1] : 0x7fffffff;
+            row2 = p2 < L.n ? L.xi[p2] : 0x7fffffff;
+        }
     }
     // receivers recorded by this chunk (forward only): rows i0..i1-1; the
     // next two receivers' cells and record rows wait in scalar registers
@@ -641,7 +667,8 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                     for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X0 + k);
                 }
                 vel_row<P, EX, W, G>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                inject<P, W>(dsv, r, j, nz, p1, row1, &d, a.adjoint, a.nt, itx);
+                if constexpr (IQ) inject_q<P, W>(dsv, r, j, nz, q1, &d, a.adjoint, a.nt, itx);
+                else inject<P, W>(dsv, r, j, nz, p1, row1, &d, a.adjoint, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
                 for (int c = 0; c < W; c++) {
@@ -710,7 +737,8 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
                         for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X2 + k);
                     }
                     vel_row<P, EX, W, G>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                    inject<P, W>(dsv, r, j, nz, p2, row2, &d, a.adjoint, a.nt, itx + 1);
+                    if constexpr (IQ) inject_q<P, W>(dsv, r, j, nz, q2, &d, a.adjoint, a.nt, itx + 1);
+                    else inject<P, W>(dsv, r, j, nz, p2, row2, &d, a.adjoint, a.nt, itx + 1);
                     const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
                     for (int c = 0; c < W; c++) {
@@ -824,16 +852,18 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl)
     }
 }
 
-template <int P, bool EX, int W, bool SQ>
+template <int P, bool EX, int W, bool SQ, bool IQ>
 __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 {
-    step2_body<P, EX, W, false, SQ>(a, nullptr);
+    __shared__ int iq[IQ ? 4 * 2 * IQ_WORDS : 1];
+    step2_body<P, EX, W, false, SQ, IQ>(a, nullptr, iq);
 }
 
 // column groups: blocks of up to grp_max wavefronts
-template <int P, bool EX, int W, bool SQ>
+template <int P, bool EX, int W, bool SQ, bool IQ>
 __global__ __launch_bounds__(64 * grp_max(P, W)) void k_step2g(StepArgs a)
 {
     __shared__ float xl[4 * grp_max(P, W) * n_xch(P) * 4];
-    step2_body<P, EX, W, true, SQ>(a, xl);
+    __shared__ int iq[IQ ? grp_max(P, W) * 2 * IQ_WORDS : 1];
+    step2_body<P, EX, W, true, SQ, IQ>(a, xl, iq);
 }

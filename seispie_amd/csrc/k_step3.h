@@ -79,7 +79,7 @@ __device__ __forceinline__ void load3(Row3<P> &p, rsrc_t RS, rsrc_t RM, cflt_p b
     }
 }
 
-template <int P, bool EX, bool SQ>
+template <int P, bool EX, bool SQ, bool IQ>
 __global__ __launch_bounds__(256) void k_step3(StepArgs a)
 {
     constexpr int NS = n_s(P), NV = n_v(P);
@@ -111,7 +111,8 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const rsrc_t RM = make_rsrc(d.mat, (long long)N_MAT * NP4);
     const cflt_p bxp = (cflt_p)d.bx;
 
-    // injection list: sources (forward) or receivers with adjoint sources
+    // injection list: sources (forward) or receivers with adjoint sources;
+    // IQ (adjoint launches): a queue of the entries in LDS
     const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
     const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
     const int ni = a.adjoint ? d.nrec : d.nsrc;
@@ -119,7 +120,15 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const long long nrows = a.adjoint ? d.nrec : d.nsrc;
     int pi = __builtin_amdgcn_readfirstlane(a.adjoint ? item2.y : item2.x);
     while (pi < ni && ik[pi] < (i0 - 1) * nz) ++pi;  // the list pointer starts at row i0-4
-    int src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+    int src_row = 0x7fffffff;
+    __shared__ int iq[IQ ? 4 * IQ_WORDS : 1];
+    InjQ q;
+    if constexpr (IQ) {
+        q.s = iq + IQ_WORDS * (threadIdx.x >> 6);
+        injq_fill<P>(q, &d, a.adjoint, a.nt, itx, pi);
+    } else {
+        src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
+    }
     // receivers recorded by this chunk (forward only): rows i0..i1-1.  The
     // next two receivers' cells and record rows wait in scalar registers, so
     // a row's test uses values loaded rows earlier and the loads of later
@@ -206,9 +215,17 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
                     dsv[2] = (float)((double)dsy + 2.0 * (double)Sc[4]);
                 }
             }
-            if (r == src_row) {
-                // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list
-                // order; every read here is wave-uniform (scalar loads)
+            // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list order
+            if constexpr (IQ) {
+                const int kr = r * nz + j;
+                injq_row<P>(q, r, &d, a.adjoint, a.nt, itx, [&](int kk, float s0, float s1) {
+                    if (kk == kr) {
+                        dsv[0] = add_rn(dsv[0], s0);
+                        if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);
+                    }
+                });
+            } else if (r == src_row) {
+                // every read here is wave-uniform (scalar loads)
                 const int kend = (r + 1) * nz, kr = r * nz + j;
                 while (pi < ni && ik[pi] < kend) {
                     const int kk = ik[pi];

@@ -52,6 +52,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -547,32 +548,48 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     return 0;
 }
 
-// a two-step kernel: strips (k_step2) or column groups (k_step2g) of W cells
-// per lane, exact or fast arithmetic, square cells or not
-template <int P, int W>
-void launch_two(bool group, bool ex, bool sq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+// Launches of the step-kernel instances: one function per combination of
+// the flags (exact or fast arithmetic, square cells or not, adjoint sources
+// queued in LDS or not), picked from a table by the flags' bits.
+using step_launch_t = void (*)(dim3, dim3, hipStream_t, const StepArgs &);
+template <int P, int W, bool G, bool EX, bool SQ, bool IQ>
+void launch_two_k(dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
-    if (group) {
-        if (ex && sq) hipLaunchKernelGGL((k_step2g<P, true, W, true>), grid, block, 0, st, a);
-        else if (ex) hipLaunchKernelGGL((k_step2g<P, true, W, false>), grid, block, 0, st, a);
-        else if (sq) hipLaunchKernelGGL((k_step2g<P, false, W, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((k_step2g<P, false, W, false>), grid, block, 0, st, a);
-    } else {
-        if (ex && sq) hipLaunchKernelGGL((k_step2<P, true, W, true>), grid, block, 0, st, a);
-        else if (ex) hipLaunchKernelGGL((k_step2<P, true, W, false>), grid, block, 0, st, a);
-        else if (sq) hipLaunchKernelGGL((k_step2<P, false, W, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((k_step2<P, false, W, false>), grid, block, 0, st, a);
-    }
+    if constexpr (G) hipLaunchKernelGGL((k_step2g<P, EX, W, SQ, IQ>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_step2<P, EX, W, SQ, IQ>), grid, block, 0, st, a);
+}
+template <int P, bool EX, bool SQ, bool IQ>
+void launch_one_k(dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+{
+    hipLaunchKernelGGL((k_step3<P, EX, SQ, IQ>), grid, block, 0, st, a);
+}
+template <int P, int W, int... I>
+constexpr std::array<step_launch_t, sizeof...(I)> two_table(std::integer_sequence<int, I...>)
+{
+    return {launch_two_k<P, W, (I & 8) != 0, (I & 4) != 0, (I & 2) != 0, (I & 1) != 0>...};
+}
+template <int P, int... I>
+constexpr std::array<step_launch_t, sizeof...(I)> one_table(std::integer_sequence<int, I...>)
+{
+    return {launch_one_k<P, (I & 4) != 0, (I & 2) != 0, (I & 1) != 0>...};
 }
 
-// the one-step kernel k_step3, exact or fast arithmetic, square cells or not
-template <int P>
-void launch_one(bool ex, bool sq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+// a two-step kernel: strips (k_step2) or column groups (k_step2g) of W cells
+// per lane; iq: the instance that reads the injection lists through a queue
+// in LDS (adjoint launches, an adjoint source at every receiver)
+template <int P, int W>
+void launch_two(bool group, bool ex, bool sq, bool iq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
-    if (ex && sq) hipLaunchKernelGGL((k_step3<P, true, true>), grid, block, 0, st, a);
-    else if (ex) hipLaunchKernelGGL((k_step3<P, true, false>), grid, block, 0, st, a);
-    else if (sq) hipLaunchKernelGGL((k_step3<P, false, true>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_step3<P, false, false>), grid, block, 0, st, a);
+    static constexpr auto tab = two_table<P, W>(std::make_integer_sequence<int, 16>{});
+    tab[(group ? 8 : 0) | (ex ? 4 : 0) | (sq ? 2 : 0) | (iq ? 1 : 0)](grid, block, st, a);
+}
+
+// the one-step kernel k_step3
+template <int P>
+void launch_one(bool ex, bool sq, bool iq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+{
+    static constexpr auto tab = one_table<P>(std::make_integer_sequence<int, 8>{});
+    tab[(ex ? 4 : 0) | (sq ? 2 : 0) | (iq ? 1 : 0)](grid, block, st, a);
 }
 
 // One launch from step `it`: `nsteps` (1 or 2) steps of every shot of the
@@ -599,19 +616,20 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
     const bool ex = c->exact;
     const bool sq = c->cur_sq;
+    const bool iq = adjoint != 0;
     if (nsteps == 2 && c->cur_fuse >= 3) {
         a.items = c->d_items_w;
         a.nitems = c->n_items_w;
         grid = dim3((unsigned)c->n_items_w);
         block = dim3((unsigned)(64 * c->grp_waves));
         if (c->cur_fuse == 4) {
-            if (c->physics == SP_SH) launch_two<SP_SH, 2>(true, ex, sq, grid, block, c->stream, a);
-            else launch_two<SP_PSV, 2>(true, ex, sq, grid, block, c->stream, a);
+            if (c->physics == SP_SH) launch_two<SP_SH, 2>(true, ex, sq, iq, grid, block, c->stream, a);
+            else launch_two<SP_PSV, 2>(true, ex, sq, iq, grid, block, c->stream, a);
         } else {
             switch (c->physics) {
-            case SP_SH: launch_two<SP_SH, 1>(true, ex, sq, grid, block, c->stream, a); break;
-            case SP_PSV: launch_two<SP_PSV, 1>(true, ex, sq, grid, block, c->stream, a); break;
-            default: launch_two<SP_SPIN, 1>(true, ex, sq, grid, block, c->stream, a); break;
+            case SP_SH: launch_two<SP_SH, 1>(true, ex, sq, iq, grid, block, c->stream, a); break;
+            case SP_PSV: launch_two<SP_PSV, 1>(true, ex, sq, iq, grid, block, c->stream, a); break;
+            default: launch_two<SP_SPIN, 1>(true, ex, sq, iq, grid, block, c->stream, a); break;
             }
         }
     } else if (nsteps == 2 && c->cur_fuse == 2) {
@@ -619,20 +637,20 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
         a.nitems = c->n_items_w;
         a.wout = 116;
         grid = dim3((unsigned)((c->n_items_w + 3) / 4));
-        if (c->physics == SP_SH) launch_two<SP_SH, 2>(false, ex, sq, grid, block, c->stream, a);
-        else launch_two<SP_PSV, 2>(false, ex, sq, grid, block, c->stream, a);
+        if (c->physics == SP_SH) launch_two<SP_SH, 2>(false, ex, sq, iq, grid, block, c->stream, a);
+        else launch_two<SP_PSV, 2>(false, ex, sq, iq, grid, block, c->stream, a);
     } else if (nsteps == 2) {
         if (!c->cur_fuse) return fail(c, "internal: two-step launch without the two-step geometry");
         switch (c->physics) {
-        case SP_SH: launch_two<SP_SH, 1>(false, ex, sq, grid, block, c->stream, a); break;
-        case SP_PSV: launch_two<SP_PSV, 1>(false, ex, sq, grid, block, c->stream, a); break;
-        default: launch_two<SP_SPIN, 1>(false, ex, sq, grid, block, c->stream, a); break;
+        case SP_SH: launch_two<SP_SH, 1>(false, ex, sq, iq, grid, block, c->stream, a); break;
+        case SP_PSV: launch_two<SP_PSV, 1>(false, ex, sq, iq, grid, block, c->stream, a); break;
+        default: launch_two<SP_SPIN, 1>(false, ex, sq, iq, grid, block, c->stream, a); break;
         }
     } else if (c->cur_kernel == 1) {
         switch (c->physics) {
-        case SP_SH: launch_one<SP_SH>(ex, sq, grid, block, c->stream, a); break;
-        case SP_PSV: launch_one<SP_PSV>(ex, sq, grid, block, c->stream, a); break;
-        default: launch_one<SP_SPIN>(ex, sq, grid, block, c->stream, a); break;
+        case SP_SH: launch_one<SP_SH>(ex, sq, iq, grid, block, c->stream, a); break;
+        case SP_PSV: launch_one<SP_PSV>(ex, sq, iq, grid, block, c->stream, a); break;
+        default: launch_one<SP_SPIN>(ex, sq, iq, grid, block, c->stream, a); break;
         }
     } else {
         switch (c->physics) {

@@ -300,6 +300,50 @@ def test_one_shot_gradient_equals_oracle_bit_for_bit(ctx_factory, physics):
                                       ((vp * vp - 2.0 * vs * vs) * kl + vs * vs * km + kr).astype(np.float32))
 
 
+_KERNEL_NAMES = {_lib.SH: [("mu", _lib.K_MU), ("rho", _lib.K_RHO)],
+                 _lib.PSV: [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO)],
+                 _lib.SPIN: [("lam", _lib.K_LAM), ("mu", _lib.K_MU), ("rho", _lib.K_RHO), ("nu", _lib.K_NU),
+                             ("j", _lib.K_J), ("muc", _lib.K_MUC)]}
+
+
+@pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
+@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4])
+def test_dense_adjoint_sources_equal_oracle_bit_for_bit(ctx_factory, physics, fuse):
+    """Adjoint launches read the adjoint sources through a queue of 64
+    entries per wavefront in LDS.  Three receivers in every inner row (a
+    fourth on an occupied cell every fifth row), given out of grid-index
+    order, and one chunk per shot put about 200 entries in every chunk:
+    the queue refills three times during a march and rows hold several
+    entries, coincident ones included.  The one-shot gradient stays equal to
+    the oracle's for one step per launch (fuse 0), strips of one or two
+    cells per lane and column groups."""
+    prob = _small(physics, nx=70, nz=62, nt=120, nshots=1)
+    g = prob.grids[0]
+    nx, nz = g.nx, g.nz
+    rec = []
+    for i in range(1, nx - 1):
+        rec += [i * nz + 2, i * nz + nz // 2, i * nz + nz - 3]
+        if i % 5 == 0:
+            rec.append(i * nz + nz // 2)
+    rec = np.array(rec, np.int32)[np.random.default_rng(3).permutation(len(rec))]
+    for sh in g.shots:
+        sh.rec_k = rec
+    prob.sae = 10
+    data = _data_for(prob)
+    print("case: dense adjoint sources", physics, "fuse", fuse, "receivers", len(rec), flush=True)
+    ctx = ctx_factory()
+    ctx.set_option(_lib.OPT_FUSE, fuse)
+    ctx.set_option(_lib.OPT_CHUNK, nx)
+    ids = configs.load_problem(ctx, prob)
+    for i, d in zip(ids, data):
+        ctx.set_data(i, d)
+    ctx.gradient(prob.sae)
+    kref, _ = oracle_gradient(prob, data)
+    for nm, w in _KERNEL_NAMES[physics]:
+        assert np.abs(kref[0][nm]).max() > 0, nm
+        np.testing.assert_array_equal(ctx.kernel(0, w), kref[0][nm], err_msg=nm)
+
+
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
 def test_against_the_float64_replay(ctx_factory, physics):
     """The GPU path (float32 storage, the reference's arithmetic) against
