@@ -146,6 +146,15 @@ __device__ __forceinline__ float upd_disp(float u, float v, double dt)
 // the scheduling region per use).
 __device__ __forceinline__ double lsel(bool ok, double v) { return ok ? v : 0.0; }
 __device__ __forceinline__ float lsel(bool ok, float v) { return ok ? v : 0.f; }
+// A wave-uniform condition the compiler has to treat as per-lane: its uses
+// become selects of unconditionally computed values (one v_cndmask each)
+// instead of scalar branches, each of which ends a scheduling region.
+__device__ __forceinline__ bool lane_bool(bool b)
+{
+    int v = b;
+    asm volatile("" : "+v"(v));
+    return v != 0;
+}
 
 // derivative coefficients of a shot in both forms
 struct DCo {
