@@ -414,11 +414,10 @@ __device__ __forceinline__ void inject(float (&dsv)[n_v(P)][W], int r, int j, in
 // the same adds from an injection queue in LDS (adjoint launches, with an
 // adjoint source at every receiver)
 template <int P, int W>
-__device__ __forceinline__ void inject_q(float (&dsv)[n_v(P)][W], int r, int j, int nz, InjQ &q, cdesc_t *dp,
-                                         int adjoint, int nt, int it)
+__device__ __forceinline__ void inject_q(float (&dsv)[n_v(P)][W], int r, int j, int nz, InjQ &q)
 {
     const int kr = r * nz + j;
-    injq_row<P>(q, r, dp, adjoint, nt, it, [&](int kk, float s0, float s1) {
+    injq_row<P>(q, r, [&](int kk, float s0, float s1) {
 #pragma unroll
         for (int c = 0; c < W; c++) {
             if (kk == kr + c) {
@@ -535,7 +534,8 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
     const unsigned vo = outl ? (unsigned)j * 4u : OOB;  // per-lane store offset (OOB: not written)
 
     // injection pointers: step it from row i0-4, step it+1 from row i0-1;
-    // IQ (adjoint launches): queues of the entries in LDS
+    // IQ (launches in which no item reaches more than IQ_CAP entries):
+    // queues of the entries in LDS
     int p1, p2, row1, row2;
     InjQ q1, q2;
     {
@@ -544,10 +544,8 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
         p2 = p1;
         while (p2 < L.n && L.k[p2] < (i0 - 1) * nz) ++p2;
         if constexpr (IQ) {
-            q1.s = iq + 2 * IQ_WORDS * gw;
-            q2.s = q1.s + IQ_WORDS;
-            injq_fill<P>(q1, &d, a.adjoint, a.nt, itx, p1);
-            injq_fill<P>(q2, &d, a.adjoint, a.nt, itx + 1, p2);
+            injq_fill<P>(q1, iq + 2 * IQ_WORDS * gw, &d, a.adjoint, a.nt, itx, p1, item2.w);
+            injq_fill<P>(q2, iq + (2 * gw + 1) * IQ_WORDS, &d, a.adjoint, a.nt, itx + 1, p2, item2.w);
         } else {
             row1 = p1 < L.n ? L.xi[p1] : 0x7fffffff;
             row2 = p2 < L.n ? L.xi[p2] : 0x7fffffff;
@@ -673,7 +671,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
                     for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X0 + k);
                 }
                 vel_row<P, EX, W, G>(dsv, S0[A], S0[B], S0[C], S0[D], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                if constexpr (IQ) inject_q<P, W>(dsv, r, j, nz, q1, &d, a.adjoint, a.nt, itx);
+                if constexpr (IQ) inject_q<P, W>(dsv, r, j, nz, q1);
                 else inject<P, W>(dsv, r, j, nz, p1, row1, &d, a.adjoint, a.nt, itx);
                 const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll
@@ -743,7 +741,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
                         for (int k = 0; k < NZS; k++) nb[k] = nbr(sl1, X2 + k);
                     }
                     vel_row<P, EX, W, G>(dsv, S1[B], S1[C], S1[D], S1[A], xm_ok, zm_ok, KX, KZ, nb, lf, ll);
-                    if constexpr (IQ) inject_q<P, W>(dsv, r, j, nz, q2, &d, a.adjoint, a.nt, itx + 1);
+                    if constexpr (IQ) inject_q<P, W>(dsv, r, j, nz, q2);
                     else inject<P, W>(dsv, r, j, nz, p2, row2, &d, a.adjoint, a.nt, itx + 1);
                     const float bxr = bxp[min(max(r, 0), nx - 1)];
 #pragma unroll

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     const cflt_p bxp = (cflt_p)d.bx;
 
     // injection list: sources (forward) or receivers with adjoint sources;
-    // IQ (adjoint launches): a queue of the entries in LDS
+    // IQ (launches in which no item reaches more than IQ_CAP entries): a
+    // queue of the entries in LDS
     const cint_p ik = (cint_p)(a.adjoint ? d.rec_k : d.src_k);
     const cint_p irow = (cint_p)(a.adjoint ? d.rec_row : d.src_row);
     const int ni = a.adjoint ? d.nrec : d.nsrc;
@@ -124,8 +125,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
     __shared__ int iq[IQ ? 4 * IQ_WORDS : 1];
     InjQ q;
     if constexpr (IQ) {
-        q.s = iq + IQ_WORDS * (threadIdx.x >> 6);
-        injq_fill<P>(q, &d, a.adjoint, a.nt, itx, pi);
+        injq_fill<P>(q, iq + IQ_WORDS * (threadIdx.x >> 6), &d, a.adjoint, a.nt, itx, pi, item2.w);
     } else {
         src_row = pi < ni ? ik[pi] / nz : 0x7fffffff;
     }
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void k_step3(StepArgs a)
             // stf_dsy / stf_dsxz (solver.py:78-93): float32 adds, list order
             if constexpr (IQ) {
                 const int kr = r * nz + j;
-                injq_row<P>(q, r, &d, a.adjoint, a.nt, itx, [&](int kk, float s0, float s1) {
+                injq_row<P>(q, r, [&](int kk, float s0, float s1) {
                     if (kk == kr) {
                         dsv[0] = add_rn(dsv[0], s0);
                         if constexpr (P != SP_SH) dsv[1] = add_rn(dsv[1], s1);

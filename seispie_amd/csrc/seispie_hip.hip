@@ -117,6 +117,7 @@ struct sp_ctx {
     int inter_sae = 0, inter_nsa = 0;    // interactions of the adjoint run being captured
     int cur_fuse = 0;          // the prepared batch uses the two-step strip geometry
     int cur_sq = 0;            // every grid of the prepared batch has square cells (dx = dz)
+    int cur_inj = 0;           // most receivers any work item of the prepared batch reaches
     int group = 0;             // column groups: wavefronts per block (0: 4)
     int grp_waves = 0;         // column groups of the prepared batch (cur_fuse 3, 4): wavefronts per block
     int graph = 0;
@@ -393,6 +394,7 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
     c->cur_kernel = batch_kernel(c, batch);
     c->cur_fuse = fuse_ok(c, batch);
     c->cur_sq = 1;
+    c->cur_inj = 0;
     for (int sb : batch) {
         const Grid &g = c->grids[c->shots[sb].grid];
         c->cur_sq = c->cur_sq && g.dx == g.dz;
@@ -416,11 +418,17 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
             return (int)(std::lower_bound(v.begin(), v.end(), (int)std::max(key, -1LL)) - v.begin());
         };
         items.push_back(make_int4(b, j0, i0, i1));
+        // receivers in the rows the item's march injects at (i0-4 .. i1+4):
+        // adjoint launches queue them in LDS when no item reaches more than
+        // IQ_CAP (the count goes with the item)
+        const int nrec_item = first(sh.rec_sorted, (long long)(i1 + 5) * g.nz) -
+                              first(sh.rec_sorted, (long long)(i0 - 4) * g.nz);
+        c->cur_inj = std::max(c->cur_inj, nrec_item);
         // list pointers from row i0-4 (the two-step kernel's first velocity
         // row; the single-step kernels skip forward to row i0-1)
         items.push_back(make_int4(first(sh.src_sorted, (long long)(i0 - 4) * g.nz),
                                   first(sh.rec_sorted, (long long)(i0 - 4) * g.nz),
-                                  first(sh.rec_sorted, (long long)i0 * g.nz), 0));
+                                  first(sh.rec_sorted, (long long)i0 * g.nz), std::min(nrec_item, IQ_CAP)));
     };
     auto build = [&](int width) {
         W = width;
@@ -576,7 +584,9 @@ constexpr std::array<step_launch_t, sizeof...(I)> one_table(std::integer_sequenc
 
 // a two-step kernel: strips (k_step2) or column groups (k_step2g) of W cells
 // per lane; iq: the instance that reads the injection lists through a queue
-// in LDS (adjoint launches, an adjoint source at every receiver)
+// in LDS (adjoint launches, an adjoint source at every receiver, when no
+// work item reaches more than IQ_CAP of them; forward launches, a source or
+// a few per shot, measured faster with the lists)
 template <int P, int W>
 void launch_two(bool group, bool ex, bool sq, bool iq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
@@ -616,7 +626,7 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     dim3 grid((unsigned)((nitems + 3) / 4)), block(256);
     const bool ex = c->exact;
     const bool sq = c->cur_sq;
-    const bool iq = adjoint != 0;
+    const bool iq = adjoint != 0 && c->cur_inj <= IQ_CAP;
     if (nsteps == 2 && c->cur_fuse >= 3) {
         a.items = c->d_items_w;
         a.nitems = c->n_items_w;

@@ -84,66 +84,84 @@ __device__ __forceinline__ cdesc_t *fresh(cdesc_t *dp)
     return dp;
 }
 
-// Upcoming injection entries in a slice of LDS per wavefront: word l of
-// each of its four 64-word rows holds list entry base + l (grid index, x
-// row, time-function values 0 and 1 at the queue's step), so a row that
-// injects reads them with broadcast LDS reads.  Read from the lists in that
-// row instead, they are a chain of dependent scalar loads (entry, then its
-// time-function row, then the value) that the march waits for in every row
-// holding an entry -- with adjoint sources at receivers every second row,
-// 14% of a spin launch.  Held in VGPRs the queue costs 10-12 of them.  h is
-// the next entry to inject, row its x row (0x7fffffff: none left).
-constexpr int IQ_WORDS = 4 * 64;  // LDS words of one queue
+// Upcoming injection entries in a slice of LDS per wavefront, filled
+// before the march: word l of each of its four 64-word rows holds list
+// entry base + l (grid index, x row, time-function values 0 and 1 at the
+// queue's step), so a row that injects reads them with broadcast LDS
+// reads.  Read from the lists in that row instead, they are a chain of
+// dependent scalar loads (entry, then its time-function row, then the
+// value) that the march waits for in every row holding an entry -- with
+// adjoint sources at receivers every second row, 14% of a spin launch.
+// The host launches the queue instances only when no work item reaches
+// more than IQ_CAP entries, so the march holds no refill: code that
+// refilled the slice, or read the lists past it, inside the march cost
+// 3-5% of every launch, injections or not.  The fill loads only the rounds
+// of 64 entries the item reaches (the count comes with the work item): a
+// fill is latency at the start of the march, 2-4% of a 26-row one.  row is
+// the next entry's x row (0x7fffffff: none left).
+constexpr int IQ_CAP = 128;               // entries a queue holds
+constexpr int IQ_WORDS = 4 * IQ_CAP;      // [k | x row | value 0 | value 1][IQ_CAP]
 struct InjQ {
-    int *s;  // the queue's LDS slice: [k | x row | value 0 | value 1][64]
-    int base, h, row;
+    int *s;  // the queue's LDS slice
+    int l, row;  // the next entry's word, its x row
 };
-// entries from, from + 1, ... at time step `it` into the slice
+// entries from, from + 1, ... at time step `it` into slice s, in rounds of
+// 64: the `count` entries the work item reaches (at most IQ_CAP) and the
+// one after them, whose row ends the item's injections
 template <int P>
-__device__ __forceinline__ void injq_fill(InjQ &q, cdesc_t *dp, int adjoint, int nt, int it, int from)
+__device__ __forceinline__ void injq_fill(InjQ &q, int *s, cdesc_t *dp, int adjoint, int nt, int it, int from,
+                                          int count)
 {
     const InjLists L = inj_lists(dp, adjoint);
-    const int l = (int)(threadIdx.x & 63), e = from + l;
-    int k = 0x7fffffff, xi = 0x7fffffff;
-    float v0 = 0.f, v1 = 0.f;
-    if (e < L.n) {
-        k = L.k[e];
-        xi = L.xi[e];
-        const long long rw = L.row[e];
-        v0 = L.val[rw * nt + it];
-        if constexpr (P != SP_SH) v1 = L.val[(L.nrows + rw) * nt + it];
+    const int lane = (int)(threadIdx.x & 63);
+    int xi0 = 0x7fffffff;
+    const int fill = min(count + 1, IQ_CAP);
+    for (int h = 0; h < fill; h += 64) {
+        const int l = h + lane, e = from + l;
+        int k = 0x7fffffff, xi = 0x7fffffff;
+        float v0 = 0.f, v1 = 0.f;
+        if (e < L.n) {
+            k = L.k[e];
+            xi = L.xi[e];
+            const long long rw = L.row[e];
+            v0 = L.val[rw * nt + it];
+            if constexpr (P != SP_SH) v1 = L.val[(L.nrows + rw) * nt + it];
+        }
+        s[l] = k;
+        s[IQ_CAP + l] = xi;
+        s[2 * IQ_CAP + l] = __float_as_int(v0);
+        if constexpr (P != SP_SH) s[3 * IQ_CAP + l] = __float_as_int(v1);
+        if (h == 0) xi0 = xi;
     }
-    q.s[l] = k;
-    q.s[64 + l] = xi;
-    q.s[128 + l] = __float_as_int(v0);
-    if constexpr (P != SP_SH) q.s[192 + l] = __float_as_int(v1);
-    q.base = q.h = from;
-    q.row = __builtin_amdgcn_readlane(xi, 0);
+    q.s = s;
+    q.l = 0;
+    q.row = __builtin_amdgcn_readlane(xi0, 0);
 }
 // the entries of x row r, in list order (sources: stf_dsy / stf_dsxz,
 // solver.py:78-93): apply(k, value0, value1) for each.  Every lane reads
-// the same LDS word, which lane l of the same wavefront wrote.
+// the same LDS word, which a lane of the same wavefront wrote.
 template <int P, class F>
-__device__ __forceinline__ void injq_row(InjQ &q, int r, cdesc_t *dp, int adjoint, int nt, int it, F &&apply)
+__device__ __forceinline__ void injq_row(InjQ &q, int r, F &&apply)
 {
     if (r != q.row) return;
     do {
-        const int l = q.h - q.base;
         __builtin_amdgcn_wave_barrier();
-        const int kk = q.s[l];
-        const float s0 = __int_as_float(q.s[128 + l]);
+        const int kk = q.s[q.l];
+        const float s0 = __int_as_float(q.s[2 * IQ_CAP + q.l]);
         float s1 = 0.f;
-        if constexpr (P != SP_SH) s1 = __int_as_float(q.s[192 + l]);
+        if constexpr (P != SP_SH) s1 = __int_as_float(q.s[3 * IQ_CAP + q.l]);
         apply(kk, s0, s1);
-        if (++q.h - q.base == 64) injq_fill<P>(q, fresh(dp), adjoint, nt, it, q.h);
-        else q.row = __builtin_amdgcn_readfirstlane(q.s[64 + q.h - q.base]);
+        // the host guarantees no item reaches entry IQ_CAP; the fill covers
+        // the entry after the item's last one
+        q.row = ++q.l < IQ_CAP ? __builtin_amdgcn_readfirstlane(q.s[IQ_CAP + q.l]) : 0x7fffffff;
     } while (q.row == r);
 }
 
 struct StepArgs {
     const ShotDesc *shots;
     const int4 *items;  // per wavefront: (shot, j0, i0, i1) and the first source with k >= (i0-4) nz,
-                        // the first receiver with k >= (i0-4) nz and the first with k >= i0 nz
+                        // the first receiver with k >= (i0-4) nz, the first with k >= i0 nz and
+                        // the receivers in rows i0-4 .. i1+4 (at most IQ_CAP)
     int nitems;
     int it, nt;         // step of this launch (plus *itp when itp is set)
     const int *itp;     // device step counter of a replayed graph, or null

@@ -308,15 +308,18 @@ _KERNEL_NAMES = {_lib.SH: [("mu", _lib.K_MU), ("rho", _lib.K_RHO)],
 
 @pytest.mark.parametrize("physics", [_lib.SH, _lib.PSV, _lib.SPIN])
 @pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4])
-def test_dense_adjoint_sources_equal_oracle_bit_for_bit(ctx_factory, physics, fuse):
-    """Adjoint launches read the adjoint sources through a queue of 64
-    entries per wavefront in LDS.  Three receivers in every inner row (a
-    fourth on an occupied cell every fifth row), given out of grid-index
-    order, and one chunk per shot put about 200 entries in every chunk:
-    the queue refills three times during a march and rows hold several
-    entries, coincident ones included.  The one-shot gradient stays equal to
-    the oracle's for one step per launch (fuse 0), strips of one or two
-    cells per lane and column groups."""
+@pytest.mark.parametrize("chunk", [8, 24, 0])
+def test_dense_adjoint_sources_equal_oracle_bit_for_bit(ctx_factory, physics, fuse, chunk):
+    """Adjoint launches read the adjoint sources through a queue of up to
+    128 entries per wavefront in LDS when no work item reaches more, and
+    from the lists otherwise.  Three receivers in every inner row (a fourth
+    on an occupied cell every fifth row), given out of grid-index order:
+    8-row chunks reach about 54 entries each (one round of 64 in the
+    queue), 24-row chunks about 100 (two rounds), one chunk per shot
+    (chunk 0 here) about 200 (the lists); rows hold several entries,
+    coincident ones included.  The one-shot gradient stays equal to the
+    oracle's for one step per launch (fuse 0), strips of one or two cells
+    per lane and column groups."""
     prob = _small(physics, nx=70, nz=62, nt=120, nshots=1)
     g = prob.grids[0]
     nx, nz = g.nx, g.nz
@@ -330,10 +333,10 @@ def test_dense_adjoint_sources_equal_oracle_bit_for_bit(ctx_factory, physics, fu
         sh.rec_k = rec
     prob.sae = 10
     data = _data_for(prob)
-    print("case: dense adjoint sources", physics, "fuse", fuse, "receivers", len(rec), flush=True)
+    print("case: dense adjoint sources", physics, "fuse", fuse, "chunk", chunk, "receivers", len(rec), flush=True)
     ctx = ctx_factory()
     ctx.set_option(_lib.OPT_FUSE, fuse)
-    ctx.set_option(_lib.OPT_CHUNK, nx)
+    ctx.set_option(_lib.OPT_CHUNK, chunk or nx)
     ids = configs.load_problem(ctx, prob)
     for i, d in zip(ids, data):
         ctx.set_data(i, d)
