@@ -438,11 +438,20 @@ __device__ __forceinline__ void inject_q(float (&dsv)[n_v(P)][W], int r, int j, 
 // m-2, after one barrier per iteration.  Groups that do not reach the
 // column's ends keep 6 halo cells there, as strips do; wavefronts past the
 // column's last cell take part in the barriers only.
-template <int P, bool EX, int W, bool G, bool SQ, bool IQ>
+//
+// M, the launch's mode: STEP_SNAP forward launches that may store adjoint
+// snapshots; STEP_PLAIN launches that store none (most forward launches,
+// and adjoint launches that read the injection lists); STEP_QUEUE adjoint
+// launches with the adjoint sources queued in LDS.  Code for snapshots the
+// launch does not take costs column groups and spin 1-2.4% per launch (two
+// branches per row); SH and P-SV strips run their STEP_SNAP instances for
+// launches without snapshots too (0.5% faster there).
+template <int P, bool EX, int W, bool G, bool SQ, int M>
 __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
 {
     static_assert(W == 1 || (W == 2 && P != SP_SPIN), "two cells per lane: SH and P-SV");
     constexpr int NS = n_s(P), NV = n_v(P);
+    constexpr bool IQ = M == STEP_QUEUE, SNAP = M == STEP_SNAP;
     constexpr int GM = grp_max(P, W), NX = n_xch(P), NZS = n_zs(P), NZV = n_zv(P);
     // strips skip the arithmetic of prologue rows that feed no output row of
     // the chunk (3 of the first 9 iterations' s(n+1) rows, 6 v(n+2) rows, 9
@@ -788,7 +797,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
                 }
                 // adjoint snapshots of either step (solver.py:193-203, 236-242):
                 // [isa][u before the step | v after it][NV]
-                if (snap_v >= 0) {
+                if (SNAP && snap_v >= 0) {
                     const rsrc_t RP = make_rsrc(d.snap + (long long)snap_v * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
@@ -796,7 +805,7 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
                         bstw<W>(RP, ov, (NV + f) * NP4 + orow, W1[D][f]);
                     }
                 }
-                if (snap_u >= 0) {
+                if (SNAP && snap_u >= 0) {
                     const rsrc_t RP = make_rsrc(d.snap + (long long)snap_u * 2 * NV * (NP4 / 4), 2LL * NV * NP4);
 #pragma unroll
                     for (int f = 0; f < NV; f++) {
@@ -856,18 +865,18 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
     }
 }
 
-template <int P, bool EX, int W, bool SQ, bool IQ>
+template <int P, bool EX, int W, bool SQ, int M>
 __global__ __launch_bounds__(256) void k_step2(StepArgs a)
 {
-    __shared__ int iq[IQ ? 4 * 2 * IQ_WORDS : 1];
-    step2_body<P, EX, W, false, SQ, IQ>(a, nullptr, iq);
+    __shared__ int iq[M == STEP_QUEUE ? 4 * 2 * IQ_WORDS : 1];
+    step2_body<P, EX, W, false, SQ, M>(a, nullptr, iq);
 }
 
 // column groups: blocks of up to grp_max wavefronts
-template <int P, bool EX, int W, bool SQ, bool IQ>
+template <int P, bool EX, int W, bool SQ, int M>
 __global__ __launch_bounds__(64 * grp_max(P, W)) void k_step2g(StepArgs a)
 {
     __shared__ float xl[4 * grp_max(P, W) * n_xch(P) * 4];
-    __shared__ int iq[IQ ? grp_max(P, W) * 2 * IQ_WORDS : 1];
-    step2_body<P, EX, W, true, SQ, IQ>(a, xl, iq);
+    __shared__ int iq[M == STEP_QUEUE ? grp_max(P, W) * 2 * IQ_WORDS : 1];
+    step2_body<P, EX, W, true, SQ, M>(a, xl, iq);
 }

@@ -557,24 +557,34 @@ int prepare_batch(sp_ctx *c, const std::vector<int> &batch, int *nitems_out)
 }
 
 // Launches of the step-kernel instances: one function per combination of
-// the flags (exact or fast arithmetic, square cells or not, adjoint sources
-// queued in LDS or not), picked from a table by the flags' bits.
+// the flags (column groups or strips, exact or fast arithmetic, square
+// cells or not, the two-step launch's mode, or for single steps whether
+// the adjoint sources are queued in LDS), picked from a table.
 using step_launch_t = void (*)(dim3, dim3, hipStream_t, const StepArgs &);
-template <int P, int W, bool G, bool EX, bool SQ, bool IQ>
+template <int P, int W, bool G, bool EX, bool SQ, int M>
 void launch_two_k(dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
-    if constexpr (G) hipLaunchKernelGGL((k_step2g<P, EX, W, SQ, IQ>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_step2<P, EX, W, SQ, IQ>), grid, block, 0, st, a);
+    if constexpr (G) hipLaunchKernelGGL((k_step2g<P, EX, W, SQ, M>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_step2<P, EX, W, SQ, M>), grid, block, 0, st, a);
 }
 template <int P, bool EX, bool SQ, bool IQ>
 void launch_one_k(dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
     hipLaunchKernelGGL((k_step3<P, EX, SQ, IQ>), grid, block, 0, st, a);
 }
+// index I = ((G * 2 + EX) * 2 + SQ) * 3 + M.  SH and P-SV strips run their
+// snapshot instances for STEP_PLAIN launches too: without the snapshot code
+// they measured 0.5% slower (column groups 0.9-1.9% and spin 2.3% faster).
+template <int P, bool G, int M>
+constexpr int two_mode()
+{
+    return (M == STEP_PLAIN && !G && P != SP_SPIN) ? STEP_SNAP : M;
+}
 template <int P, int W, int... I>
 constexpr std::array<step_launch_t, sizeof...(I)> two_table(std::integer_sequence<int, I...>)
 {
-    return {launch_two_k<P, W, (I & 8) != 0, (I & 4) != 0, (I & 2) != 0, (I & 1) != 0>...};
+    return {launch_two_k<P, W, (I / 12) != 0, (I / 6) % 2 != 0, (I / 3) % 2 != 0,
+                         two_mode<P, (I / 12) != 0, I % 3>()>...};
 }
 template <int P, int... I>
 constexpr std::array<step_launch_t, sizeof...(I)> one_table(std::integer_sequence<int, I...>)
@@ -583,18 +593,16 @@ constexpr std::array<step_launch_t, sizeof...(I)> one_table(std::integer_sequenc
 }
 
 // a two-step kernel: strips (k_step2) or column groups (k_step2g) of W cells
-// per lane; iq: the instance that reads the injection lists through a queue
-// in LDS (adjoint launches, an adjoint source at every receiver, when no
-// work item reaches more than IQ_CAP of them; forward launches, a source or
-// a few per shot, measured faster with the lists)
+// per lane, in mode m (STEP_SNAP, STEP_PLAIN or STEP_QUEUE)
 template <int P, int W>
-void launch_two(bool group, bool ex, bool sq, bool iq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
+void launch_two(bool group, bool ex, bool sq, int m, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
-    static constexpr auto tab = two_table<P, W>(std::make_integer_sequence<int, 16>{});
-    tab[(group ? 8 : 0) | (ex ? 4 : 0) | (sq ? 2 : 0) | (iq ? 1 : 0)](grid, block, st, a);
+    static constexpr auto tab = two_table<P, W>(std::make_integer_sequence<int, 24>{});
+    tab[((group ? 2 : 0) + (ex ? 1 : 0)) * 6 + (sq ? 3 : 0) + m](grid, block, st, a);
 }
 
-// the one-step kernel k_step3
+// the one-step kernel k_step3; iq: the instance that reads the adjoint
+// sources through a queue in LDS
 template <int P>
 void launch_one(bool ex, bool sq, bool iq, dim3 grid, dim3 block, hipStream_t st, const StepArgs &a)
 {
@@ -627,19 +635,23 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
     const bool ex = c->exact;
     const bool sq = c->cur_sq;
     const bool iq = adjoint != 0 && c->cur_inj <= IQ_CAP;
+    // two-step mode: adjoint sources queued in LDS (iq), snapshots possible
+    // (a snapshot slot given, or looked up per step in a replayed graph), or
+    // neither
+    const int mode = iq ? STEP_QUEUE : (snap_v >= 0 || snap_u >= 0 || a.snap_tab) ? STEP_SNAP : STEP_PLAIN;
     if (nsteps == 2 && c->cur_fuse >= 3) {
         a.items = c->d_items_w;
         a.nitems = c->n_items_w;
         grid = dim3((unsigned)c->n_items_w);
         block = dim3((unsigned)(64 * c->grp_waves));
         if (c->cur_fuse == 4) {
-            if (c->physics == SP_SH) launch_two<SP_SH, 2>(true, ex, sq, iq, grid, block, c->stream, a);
-            else launch_two<SP_PSV, 2>(true, ex, sq, iq, grid, block, c->stream, a);
+            if (c->physics == SP_SH) launch_two<SP_SH, 2>(true, ex, sq, mode, grid, block, c->stream, a);
+            else launch_two<SP_PSV, 2>(true, ex, sq, mode, grid, block, c->stream, a);
         } else {
             switch (c->physics) {
-            case SP_SH: launch_two<SP_SH, 1>(true, ex, sq, iq, grid, block, c->stream, a); break;
-            case SP_PSV: launch_two<SP_PSV, 1>(true, ex, sq, iq, grid, block, c->stream, a); break;
-            default: launch_two<SP_SPIN, 1>(true, ex, sq, iq, grid, block, c->stream, a); break;
+            case SP_SH: launch_two<SP_SH, 1>(true, ex, sq, mode, grid, block, c->stream, a); break;
+            case SP_PSV: launch_two<SP_PSV, 1>(true, ex, sq, mode, grid, block, c->stream, a); break;
+            default: launch_two<SP_SPIN, 1>(true, ex, sq, mode, grid, block, c->stream, a); break;
             }
         }
     } else if (nsteps == 2 && c->cur_fuse == 2) {
@@ -647,14 +659,14 @@ int launch_step(sp_ctx *c, int nitems, int it, int snap_idx, int adjoint, int rd
         a.nitems = c->n_items_w;
         a.wout = 116;
         grid = dim3((unsigned)((c->n_items_w + 3) / 4));
-        if (c->physics == SP_SH) launch_two<SP_SH, 2>(false, ex, sq, iq, grid, block, c->stream, a);
-        else launch_two<SP_PSV, 2>(false, ex, sq, iq, grid, block, c->stream, a);
+        if (c->physics == SP_SH) launch_two<SP_SH, 2>(false, ex, sq, mode, grid, block, c->stream, a);
+        else launch_two<SP_PSV, 2>(false, ex, sq, mode, grid, block, c->stream, a);
     } else if (nsteps == 2) {
         if (!c->cur_fuse) return fail(c, "internal: two-step launch without the two-step geometry");
         switch (c->physics) {
-        case SP_SH: launch_two<SP_SH, 1>(false, ex, sq, iq, grid, block, c->stream, a); break;
-        case SP_PSV: launch_two<SP_PSV, 1>(false, ex, sq, iq, grid, block, c->stream, a); break;
-        default: launch_two<SP_SPIN, 1>(false, ex, sq, iq, grid, block, c->stream, a); break;
+        case SP_SH: launch_two<SP_SH, 1>(false, ex, sq, mode, grid, block, c->stream, a); break;
+        case SP_PSV: launch_two<SP_PSV, 1>(false, ex, sq, mode, grid, block, c->stream, a); break;
+        default: launch_two<SP_SPIN, 1>(false, ex, sq, mode, grid, block, c->stream, a); break;
         }
     } else if (c->cur_kernel == 1) {
         switch (c->physics) {

@@ -100,6 +100,8 @@ __device__ __forceinline__ cdesc_t *fresh(cdesc_t *dp)
 // fill is latency at the start of the march, 2-4% of a 26-row one.  row is
 // the next entry's x row (0x7fffffff: none left).
 constexpr int IQ_CAP = 128;               // entries a queue holds
+// modes of a two-step launch (template parameter M of the kernels)
+constexpr int STEP_SNAP = 0, STEP_PLAIN = 1, STEP_QUEUE = 2;
 constexpr int IQ_WORDS = 4 * IQ_CAP;      // [k | x row | value 0 | value 1][IQ_CAP]
 struct InjQ {
     int *s;  // the queue's LDS slice
