@@ -823,11 +823,18 @@ __device__ __forceinline__ void step2_body(StepArgs a, float *xl, int *iq)
                     const int kr = r * nz + j, cs = nrec * a.nt * 4;
                     do {
                         const int c = nk1 - kr;  // 0 .. W-1 in the lane holding the receiver
-                        const unsigned orc = (outl && c >= 0 && c < W) ? (unsigned)(nr1 * a.nt + itx) * 4u : OOB;
+                        const bool mine = outl && c >= 0 && c < W;
+                        // column groups: only the wavefront holding the
+                        // receiver stores (the group's wavefronts march in
+                        // step, so stores in any of them hold up all; 2% at
+                        // 500^2 x 64); strips measured no faster that way
+                        if (!G || __any(mine)) {
+                            const unsigned orc = mine ? (unsigned)(nr1 * a.nt + itx) * 4u : OOB;
 #pragma unroll
-                        for (int f = 0; f < NV; f++) {
-                            bst(RR, orc, f * cs, (W == 2 && c == 1) ? u1[f][W - 1] : u1[f][0]);
-                            bst(RR, orc, f * cs + 4, (W == 2 && c == 1) ? u2[f][W - 1] : u2[f][0]);
+                            for (int f = 0; f < NV; f++) {
+                                bst(RR, orc, f * cs, (W == 2 && c == 1) ? u1[f][W - 1] : u1[f][0]);
+                                bst(RR, orc, f * cs + 4, (W == 2 && c == 1) ? u2[f][W - 1] : u2[f][0]);
+                            }
                         }
                         ++pr;
                         nk1 = nk2;
