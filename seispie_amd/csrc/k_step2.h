@@ -1,5 +1,5 @@
-// seispie_hip -- k_step2<P, EX, W>, two time steps per launch with W = 1 or 2
-// z cells per lane.
+// seispie_hip -- k_step2<P, EX, W, SQ, M> and k_step2g, two time steps per
+// launch with W = 1 or 2 z cells per lane.
 // Part of seispie_hip.hip: included there, inside its anonymous namespace,
 // after <hip/hip_runtime.h> and the headers before this one.
 #pragma once

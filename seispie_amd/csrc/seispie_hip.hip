@@ -5,10 +5,12 @@
 // one thread per cell, each kernel streaming whole arrays through HBM.
 // Here one launch advances every shot of a batch by one or two time steps:
 //
-//   * k_step2<PHYS, EX, W> (two steps per launch, the default wherever the
-//     state between the steps is not needed; W = 2 holds two z cells per
-//     lane, with 8-byte accesses and 116 outputs per wavefront),
-//     k_step3<PHYS> (one step; the same march with buffer loads and stores)
+//   * k_step2<PHYS, EX, W, SQ, M> (two steps per launch, the default
+//     wherever the state between the steps is not needed; W = 2 holds two z
+//     cells per lane, with 8-byte accesses and 116 outputs per wavefront;
+//     SQ: square cells; M: launch mode), k_step2g (the same march in column
+//     groups that pass edge cells through LDS), k_step3<PHYS, EX, SQ, IQ>
+//     (one step; the same march with buffer loads and stores)
 //     and k_step<PHYS> (plain one-step fallback for grids too large for
 //     32-bit buffer offsets).  In each, a wavefront owns a strip of 64
 //     (or 128) consecutive z cells (z is the contiguous axis,
