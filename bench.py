@@ -531,7 +531,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"] / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(r["achieved"] / HBM_PEAK, 4), "traffic": traffic,
                      "kernel": r["kname"], "launch_ms": round(r["launch_ms"], 4),
-                     "alg_bytes_per_launch": r["bytes_per_launch"], "steps_per_launch": round(r["steps_per_launch"], 3)},
+                     "alg_bytes_per_launch": r["bytes_per_launch"], "steps_per_launch": round(r["steps_per_launch"], 3),
+                     "traffic_rate_GBps": round(traffic / (r["launch_ms"] * 1e-3) / 1e9, 1) if traffic else None,
+                     "note": ("a launch advances steps_per_launch time steps and is credited their algorithmic "
+                              "bytes while it reads and writes the state once; traffic is the HBM bytes it moves "
+                              "(rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, profiles/step_kernel_traffic.json)")},
         "cpu_baseline": cpu,
         "time_to_misfit": ttm,
         "configs": others,
