@@ -208,8 +208,9 @@ int sp_smooth_noalloc(sp_ctx *ctx, float *field_dev, int nx, int nz, int sigma);
  * strips or in column groups of 2, 4 or 8 wavefronts (SP_OPT_FUSE,
  * SP_OPT_GROUP); the chunks are a fixed list plus, for 2 to 6 wavefronts per
  * SIMD, the shortest chunk whose launch fits in that many.  The six fastest
- * are timed again over more steps (at least 3 x steps, about 25 ms) and the
- * fastest of those is kept for later runs and reported; the choice is
+ * are timed again over more steps (at least 3 x steps, about 25 ms), twice,
+ * interleaved, and the one with the best time is kept for later runs and
+ * reported; the choice is
  * remembered for the process and reused for the same physics and grids.
  * Overwrites wavefields and records (every run clears them first).  No
  * reference counterpart (the reference launches one thread per cell with

@@ -2268,16 +2268,18 @@ int sp_autotune(sp_ctx *c, int steps, int *chunk_out, int *order_out)
     }
     std::sort(timed.begin(), timed.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
     const int steps2 = std::min(c->nt, std::max(3 * steps, (int)std::ceil(25.0 * steps / std::max(timed[0].first, 1e-3f))));
-    float best = 1e30f;
-    std::array<int, 4> win = timed[0].second;
-    for (size_t q = 0; q < std::min<size_t>(6, timed.size()); q++) {
-        float ms = 0;
-        if (time_setting(timed[q].second, steps2, &ms)) return 2;
-        if (ms < best) {
-            best = ms;
-            win = timed[q].second;
+    // two interleaved rounds over the finalists, each keeping its best time,
+    // so one slow sample (clock or neighbour noise) does not decide
+    const size_t nfin = std::min<size_t>(6, timed.size());
+    std::vector<float> fin_ms(nfin, 1e30f);
+    for (int round = 0; round < 2; round++)
+        for (size_t q = 0; q < nfin; q++) {
+            float ms = 0;
+            if (time_setting(timed[q].second, steps2, &ms)) return 2;
+            fin_ms[q] = std::min(fin_ms[q], ms);
         }
-    }
+    const size_t wq = (size_t)(std::min_element(fin_ms.begin(), fin_ms.end()) - fin_ms.begin());
+    std::array<int, 4> win = timed[wq].second;
     const int bf = win[0], bg = win[1], bo = win[2], bc = win[3];
     c->chunk = bc;
     c->order = bo;
